@@ -1,0 +1,231 @@
+/*
+ * yaff_nb.h -- C ABI of the MI355X nonbonded engine (libyaff_nb.so).
+ *
+ * Every entry point is asynchronous on the HIP stream it is given, takes
+ * device pointers (allocated by the caller) plus sizes, allocates nothing and
+ * never synchronises, except the two that say so (ynb_nlist_rows and
+ * ynb_sys_plan, which is host-only).  Return value: 0 on success, a negative
+ * YNB_E* code otherwise; ynb_last_error() gives the message.
+ *
+ * Work is batched: a launch covers every system of a ynb_batch.  Atoms of
+ * system s live at [sys.atom0, sys.atom0 + sys.natom) of the padded atom
+ * arrays; atom0 is a multiple of 64 so that one wavefront never holds atoms
+ * of two systems.  A single system is a batch of one.
+ *
+ * Reference interfaces replaced (molmod/yaff):
+ *   ynb_nlist_build + ynb_nlist_rows   <- nlist_build_low   yaff/pes/nlist.h:37-39
+ *                                         (+ nlist_status_* yaff/pes/ext.pyx:359-449)
+ *   ynb_nlist_recompute                <- nlist_recompute_low yaff/pes/nlist.h:41-42
+ *   ynb_pair_compute                   <- pair_pot_compute  yaff/pes/pair_pot.h:61-64
+ *                                         with pair_fn_lj/mm3/exprep/ei and
+ *                                         switch3/hammer (yaff/pes/truncation.h)
+ *   ynb_ewald_reci                     <- compute_ewald_reci yaff/pes/ewald.h:31-34
+ *   ynb_ewald_corr                     <- compute_ewald_corr yaff/pes/ewald.h:39-43
+ *   ynb_verlet_*                       <- VerletIntegrator.propagate
+ *                                         yaff/sampling/verlet.py:339-367
+ */
+#ifndef YAFF_NB_H
+#define YAFF_NB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *ynb_stream; /* hipStream_t */
+
+#define YNB_MAX_POT 4
+#define YNB_WAVE 64
+
+enum { YNB_OK = 0, YNB_EINVAL = -1, YNB_EHIP = -2, YNB_EWORKSPACE = -3, YNB_ECAPACITY = -4 };
+enum { YNB_POT_LJ = 0, YNB_POT_MM3 = 1, YNB_POT_EXPREP = 2, YNB_POT_EI = 3 };
+enum { YNB_TR_NONE = 0, YNB_TR_SWITCH3 = 1, YNB_TR_HAMMER = 2 };
+enum { YNB_FP64 = 0, YNB_MIXED = 1 };
+
+/* Same layout as neigh_row_type (yaff/pes/nlist.h:30-35): 72 bytes. */
+typedef struct ynb_neigh_row {
+  int64_t a, b;
+  double d, dx, dy, dz;
+  int64_t r0, r1, r2;
+} ynb_neigh_row;
+
+/* Same layout as scaling_row_type (yaff/pes/pair_pot.h:46-50). */
+typedef struct ynb_scaling_row {
+  int64_t a, b;
+  double scale;
+  int64_t nbond;
+} ynb_scaling_row;
+
+/* One pair potential of a system (one ForcePartPair). */
+typedef struct ynb_pot {
+  int32_t kind;       /* YNB_POT_* */
+  int32_t trunc;      /* YNB_TR_* */
+  double rcut;
+  double trunc_par;   /* Switch3 width or Hammer tau */
+  double alpha;       /* EI: Ewald alpha (0: bare Coulomb) */
+  double dielectric;  /* EI */
+  int32_t nffatype;   /* ExpRep: number of atom types */
+  int32_t cross0;     /* ExpRep: offset of the amp table in ynb_pairpar.cross; b follows */
+} ynb_pot;
+
+/* One system of a batch.  Filled on the host (ynb_sys_plan completes the
+ * neighbour-list grid) and copied to the device by the caller. */
+typedef struct ynb_sys {
+  double rvecs[9];    /* cell vectors as rows (full 3x3, Cell.update_rvecs) */
+  double gvecs[9];    /* reciprocal rows, no 2 pi */
+  double rc;          /* neighbour-list radius: rcut + skin */
+  double volume;
+  int32_t nvec;       /* 0..3 periodic directions */
+  int32_t natom;
+  int32_t atom0;      /* offset into the padded atom arrays, multiple of 64 */
+  int32_t npot;
+  int32_t rmax[3];    /* image range of nlist.py:132 */
+  int32_t nlow, nhigh;
+  int32_t grid[3];    /* bins per cell vector (ynb_sys_plan) */
+  int32_t range[3];   /* bins searched on each side (ynb_sys_plan) */
+  int32_t bin0;       /* first global bin of this system */
+  int32_t nbin;
+  int32_t stab0, nstab; /* scaling rows of this system in ynb_pairpar.stab */
+  ynb_pot pot[YNB_MAX_POT];
+  /* Ewald reciprocal part (nk == 0: none) */
+  double alpha, dielectric;
+  int32_t k0, nk;         /* k vectors [k0, k0+nk) of ynb_ewald.kvec; k0 and nk multiples of 256 */
+  int32_t row0, nrow;     /* k rows (runs of consecutive g0 at fixed g1, g2) */
+  int32_t seg0, nseg;     /* segments of <= seglen k vectors of one row */
+  int32_t chunk0, nchunk; /* atom chunks of the structure-factor pass */
+  int64_t spart0;         /* offset of this system's partial structure factors, in k entries */
+} ynb_sys;
+
+/* Device tables shared by all launches over one batch. */
+typedef struct ynb_batch {
+  const ynb_sys *sys;        /* device [nsys] */
+  const int32_t *wave_sys;   /* device [npad/64]: system of each wavefront, -1 = none */
+  const int32_t *bin_sys;    /* device [nbin]: system of each bin */
+  int32_t nsys;
+  int32_t nbin;              /* total bins (0 when every system uses the all-pairs path) */
+  int64_t npad;              /* padded atom count, multiple of 64 */
+  int32_t max_bin_atoms;     /* not used by the kernels; informative */
+  int32_t pad_;
+} ynb_batch;
+
+/* Full neighbour list in ELL layout: entry k of atom i is at k*npad + i.
+ * j is the padded index of the partner, img packs the integer translation n
+ * (10 bits per axis, offset 512) with pos_j + n.R the partner image. */
+typedef struct ynb_nlist {
+  int32_t *j;
+  int32_t *img;
+  int32_t *cnt;        /* [npad] entries per atom (also counted past kcap) */
+  int32_t *status;     /* [4]: [0] overflow flag, [1] max count */
+  int32_t kcap;
+  int32_t pad_;
+} ynb_nlist;
+
+/* Per-atom parameters of the pair potentials. */
+typedef struct ynb_pairpar {
+  const double *par;   /* [YNB_MAX_POT][3][npad]: LJ/MM3 sigma, epsilon, onlypauli;
+                          EI charge, radius; ExpRep ffatype id (as double) */
+  const double *cross; /* ExpRep amp/b tables, row-major nffatype^2 each */
+  const ynb_scaling_row *stab; /* scaling rows (a > b, local indices), sorted per system */
+} ynb_pairpar;
+
+/* Ewald reciprocal-space tables and work buffers (all device). */
+typedef struct ynb_ewald {
+  const double *kvec;     /* [nk_tot][4]: kx, ky, kz and c_k = 4 pi/V exp(-k^2/4 alpha^2)/k^2 (0: padding) */
+  const int32_t *krow;    /* [nrow_tot][4]: g1, g2, first g0, count */
+  const int32_t *krow_k;  /* [nrow_tot]: global index of the row's first k */
+  const int32_t *seg;     /* [nseg_tot][4]: first g0, g1, g2, global index of the first k */
+  const int32_t *seg_len; /* [nseg_tot]: k vectors in the segment */
+  const int32_t *chunk;   /* [nchunk_tot][4]: system, first local atom, atom count, group 0 mid/1 low/2 high */
+  const int32_t *work;    /* [nwork][4]: system, first global segment, global chunk, segments (<= 64) */
+  const int32_t *kblk_sys;/* [nk_tot/256]: system of each block of 256 k vectors */
+  int32_t nwork;
+  int32_t nk_tot;
+  int32_t seglen;         /* 4, 8 or 16 */
+  int32_t kchunks;        /* the force pass splits each system's rows in this many parts */
+  int32_t groups;         /* nonzero when some system has nlow > 0 or nhigh < natom */
+  int32_t pad_;
+  double *frac;           /* work [4][npad]: wrapped fractional coordinates and charge */
+  double *step;           /* work [2][npad]: cos, sin of 2 pi f0 */
+  double *spart;          /* work: partial structure factors, 2 doubles per entry */
+  double *coef;           /* work [nk_tot][6]: 2 c_k S(k) of the mid, low and high groups (cos, sin) */
+  double *kpart;          /* work [nk_tot/256][8] */
+  double *gpart;          /* work [kchunks][npad][3] */
+} ynb_ewald;
+
+/* ------------------------------------------------------------------ */
+const char *ynb_version(void);
+const char *ynb_last_error(void);
+
+/* Host only.  Completes sys->grid, sys->range and sys->nbin from the cell,
+ * sys->rc and sys->natom.  Systems with nvec < 3 use the all-pairs path
+ * (nbin = 0). */
+int ynb_sys_plan(ynb_sys *sys);
+
+/* Workspace of ynb_nlist_build in bytes. */
+size_t ynb_nlist_workspace_bytes(int64_t npad, int32_t nbin);
+
+/* Workspace of ynb_nlist_rows for a system of natom atoms and nrows rows. */
+size_t ynb_rows_workspace_bytes(int64_t natom, int64_t nrows);
+
+/* Cell-binned (or all-pairs) neighbour list of every system: the full list of
+ * partners j of each atom i (both directions, periodic self images included)
+ * whose Yaff half row -- built exactly as nlist_build_low builds it -- has
+ * d^2 < rc^2.  Sets status[0] when an atom has more than kcap entries. */
+int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t work_bytes,
+                    const ynb_nlist *nl, ynb_stream stream);
+
+/* Materialises system `s`'s half list as Yaff rows (local atom indices) in
+ * nlist_build_low's order.  Synchronises.  Writes at most `cap` rows to
+ * `rows` (device) and the total to *count (host). */
+int ynb_nlist_rows(const ynb_batch *b, int32_t s, const double *pos, const ynb_nlist *nl,
+                   void *work, size_t work_bytes, ynb_neigh_row *rows, int64_t cap,
+                   int64_t *count, ynb_stream stream);
+
+/* Recomputes d, dx, dy, dz of existing rows from new positions (skin path). */
+int ynb_nlist_recompute(const ynb_sys *h_sys, const double *pos, const double *pos_old,
+                        ynb_neigh_row *rows, int64_t nrow, ynb_stream stream);
+
+/* Pair potentials of the slots in slot_mask, one pass over the list.
+ * gpos (nullable): [npad][3] gradient summed over the slots, written or, with
+ * accumulate != 0, added.  gpos_slots (nullable): [YNB_MAX_POT][npad][3].
+ * out: [nsys][YNB_MAX_POT][10] energy followed by the 3x3 virial.
+ * partial: work of (npad/64) * YNB_MAX_POT * 8 doubles. */
+int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const double *pos,
+                     const ynb_nlist *nl, uint32_t slot_mask, int32_t precision,
+                     double *gpos, int32_t accumulate, double *gpos_slots,
+                     double *partial, double *out, ynb_stream stream);
+
+/* Ewald reciprocal term of every system with nk > 0.
+ * out: [nsys][10] energy followed by the virial; gpos as in ynb_pair_compute. */
+int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const double *pos, const double *charges,
+                   int32_t precision, double *gpos, int32_t accumulate, double *out,
+                   ynb_stream stream);
+
+/* Ewald correction for scaled pairs of one system plus the self term.
+ * Processes the rows in order in one wavefront (deterministic). */
+int ynb_ewald_corr(const ynb_sys *h_sys, const ynb_sys *d_sys, const double *pos, const double *charges,
+                   const ynb_scaling_row *stab, int64_t nstab, double *gpos, int32_t accumulate,
+                   double *out, ynb_stream stream);
+
+/* Velocity Verlet halves (yaff/sampling/verlet.py:345-357): kick_drift does
+ * vel += dt/2 * (-gpos/m); pos += dt * vel; kick does vel += dt/2 * (-gpos/m)
+ * and writes 2*Ekin of each system to ekin2[nsys] via `partial`
+ * (npad/64 doubles). */
+int ynb_verlet_kick_drift(const ynb_batch *b, double *pos, double *vel, const double *gpos,
+                          const double *minv, const double *dt, ynb_stream stream);
+int ynb_verlet_kick(const ynb_batch *b, double *vel, const double *gpos, const double *minv,
+                    const double *masses, const double *dt, double *partial, double *ekin2,
+                    ynb_stream stream);
+
+/* out[s][k] = sum over slots of in[s][slot][k] for k < 10, plus optional
+ * extra [nsys][10] terms; used to total a force-field evaluation on device. */
+int ynb_total(const ynb_batch *b, const double *pair_out, const double *ewald_out, double *total,
+              ynb_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,114 @@
+"""ctypes binding of libyaff_nb.so (include/yaff_nb.h).
+
+The structures below mirror the header field by field.  Loading fails loudly
+when the library has not been built: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libyaff_nb.so')
+
+MAX_POT = 4
+POT_LJ, POT_MM3, POT_EXPREP, POT_EI = 0, 1, 2, 3
+TR_NONE, TR_SWITCH3, TR_HAMMER = 0, 1, 2
+FP64, MIXED = 0, 1
+
+ROW_DTYPE = np.dtype([('a', np.int64), ('b', np.int64), ('d', np.float64), ('dx', np.float64),
+                      ('dy', np.float64), ('dz', np.float64), ('r0', np.int64), ('r1', np.int64),
+                      ('r2', np.int64)])
+SCALING_DTYPE = np.dtype([('a', np.int64), ('b', np.int64), ('scale', np.float64), ('nbond', np.int64)])
+
+
+class YnbPot(C.Structure):
+    _fields_ = [('kind', C.c_int32), ('trunc', C.c_int32), ('rcut', C.c_double), ('trunc_par', C.c_double),
+                ('alpha', C.c_double), ('dielectric', C.c_double), ('nffatype', C.c_int32),
+                ('cross0', C.c_int32)]
+
+
+class YnbSys(C.Structure):
+    _fields_ = [('rvecs', C.c_double * 9), ('gvecs', C.c_double * 9), ('rc', C.c_double),
+                ('volume', C.c_double), ('nvec', C.c_int32), ('natom', C.c_int32), ('atom0', C.c_int32),
+                ('npot', C.c_int32), ('rmax', C.c_int32 * 3), ('nlow', C.c_int32), ('nhigh', C.c_int32),
+                ('grid', C.c_int32 * 3), ('range', C.c_int32 * 3), ('bin0', C.c_int32), ('nbin', C.c_int32),
+                ('stab0', C.c_int32), ('nstab', C.c_int32), ('pot', YnbPot * MAX_POT),
+                ('alpha', C.c_double), ('dielectric', C.c_double), ('k0', C.c_int32), ('nk', C.c_int32),
+                ('row0', C.c_int32), ('nrow', C.c_int32), ('seg0', C.c_int32), ('nseg', C.c_int32),
+                ('chunk0', C.c_int32), ('nchunk', C.c_int32), ('spart0', C.c_int64)]
+
+
+class YnbBatch(C.Structure):
+    _fields_ = [('sys', C.c_void_p), ('wave_sys', C.c_void_p), ('bin_sys', C.c_void_p), ('nsys', C.c_int32),
+                ('nbin', C.c_int32), ('npad', C.c_int64), ('max_bin_atoms', C.c_int32), ('pad_', C.c_int32)]
+
+
+class YnbNlist(C.Structure):
+    _fields_ = [('j', C.c_void_p), ('img', C.c_void_p), ('cnt', C.c_void_p), ('status', C.c_void_p),
+                ('kcap', C.c_int32), ('pad_', C.c_int32)]
+
+
+class YnbPairpar(C.Structure):
+    _fields_ = [('par', C.c_void_p), ('cross', C.c_void_p), ('stab', C.c_void_p)]
+
+
+class YnbEwald(C.Structure):
+    _fields_ = [('kvec', C.c_void_p), ('krow', C.c_void_p), ('krow_k', C.c_void_p), ('seg', C.c_void_p),
+                ('seg_len', C.c_void_p), ('chunk', C.c_void_p), ('work', C.c_void_p), ('kblk_sys', C.c_void_p),
+                ('nwork', C.c_int32), ('nk_tot', C.c_int32), ('seglen', C.c_int32), ('kchunks', C.c_int32),
+                ('groups', C.c_int32), ('pad_', C.c_int32), ('frac', C.c_void_p), ('step', C.c_void_p),
+                ('spart', C.c_void_p), ('coef', C.c_void_p), ('kpart', C.c_void_p), ('gpart', C.c_void_p)]
+
+
+EXPORTS = ['ynb_version', 'ynb_last_error', 'ynb_sys_plan', 'ynb_nlist_workspace_bytes',
+           'ynb_rows_workspace_bytes', 'ynb_nlist_build', 'ynb_nlist_rows', 'ynb_nlist_recompute',
+           'ynb_pair_compute', 'ynb_ewald_reci', 'ynb_ewald_corr', 'ynb_verlet_kick_drift', 'ynb_verlet_kick',
+           'ynb_total']
+
+
+class YnbError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads libyaff_nb.so once; raises OSError when it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError('libyaff_nb.so is not built (%s); run python -m yaff_amd.build' % path)
+    L = C.CDLL(path)
+    vp, i32, i64, sz, P = C.c_void_p, C.c_int32, C.c_int64, C.c_size_t, C.POINTER
+    L.ynb_version.restype = C.c_char_p
+    L.ynb_last_error.restype = C.c_char_p
+    L.ynb_sys_plan.argtypes = [P(YnbSys)]
+    L.ynb_nlist_workspace_bytes.restype = sz
+    L.ynb_nlist_workspace_bytes.argtypes = [i64, i32]
+    L.ynb_rows_workspace_bytes.restype = sz
+    L.ynb_rows_workspace_bytes.argtypes = [i64, i64]
+    L.ynb_nlist_build.argtypes = [P(YnbBatch), vp, vp, sz, P(YnbNlist), vp]
+    L.ynb_nlist_rows.argtypes = [P(YnbBatch), i32, vp, P(YnbNlist), vp, sz, vp, i64, P(i64), vp]
+    L.ynb_nlist_recompute.argtypes = [P(YnbSys), vp, vp, vp, i64, vp]
+    L.ynb_pair_compute.argtypes = [P(YnbBatch), P(YnbPairpar), vp, P(YnbNlist), C.c_uint32, i32, vp, i32, vp,
+                                   vp, vp, vp]
+    L.ynb_ewald_reci.argtypes = [P(YnbBatch), P(YnbEwald), vp, vp, i32, vp, i32, vp, vp]
+    L.ynb_ewald_corr.argtypes = [P(YnbSys), vp, vp, vp, vp, i64, vp, i32, vp, vp]
+    L.ynb_verlet_kick_drift.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp, vp]
+    L.ynb_verlet_kick.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp, vp, vp, vp]
+    L.ynb_total.argtypes = [P(YnbBatch), vp, vp, vp, vp]
+    for name in EXPORTS:
+        if name not in ('ynb_version', 'ynb_last_error', 'ynb_nlist_workspace_bytes', 'ynb_rows_workspace_bytes'):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc):
+    """Raises YnbError with the library's message on a nonzero return code."""
+    if rc != 0:
+        raise YnbError('libyaff_nb: %s (code %d)' % (load().ynb_last_error().decode(), rc))
+    return rc

@@ -1,0 +1,130 @@
+// Shared device helpers of libyaff_nb.so.
+//
+// This library is compiled with -ffp-contract=off: every FP64 expression that
+// restates Yaff's C (yaff/pes/cell.c, nlist.c, pair_pot.c) is evaluated with
+// the same operations in the same order and without fused multiply-adds, so
+// the device reproduces the reference bits (pair membership, d, dx, dy, dz).
+// Code that may contract writes fma() explicitly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/yaff_nb.h"
+
+namespace ynb {
+
+void set_error(const char *fmt, ...);
+
+#define YNB_HIP(expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      ::ynb::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                       __FILE__, __LINE__);                                    \
+      return YNB_EHIP;                                                         \
+    }                                                                          \
+  } while (0)
+
+#define YNB_LAUNCHED()                                                         \
+  do {                                                                         \
+    hipError_t e_ = hipGetLastError();                                         \
+    if (e_ != hipSuccess) {                                                    \
+      ::ynb::set_error("kernel launch failed: %s (%s:%d)",                     \
+                       hipGetErrorString(e_), __FILE__, __LINE__);             \
+      return YNB_EHIP;                                                         \
+    }                                                                          \
+  } while (0)
+
+// Image translations are packed 10 bits per axis with an offset of 512.
+__host__ __device__ inline int32_t pack_img(int n0, int n1, int n2) {
+  return (n0 + 512) | ((n1 + 512) << 10) | ((n2 + 512) << 20);
+}
+__host__ __device__ inline void unpack_img(int32_t c, int n[3]) {
+  n[0] = (c & 1023) - 512;
+  n[1] = ((c >> 10) & 1023) - 512;
+  n[2] = ((c >> 20) & 1023) - 512;
+}
+
+// cell.c:81-106 -- sequential per-axis wrap.  m receives the integer
+// subtracted along each axis (0 beyond nvec).
+__device__ inline void mic_exact(double d[3], const double *rv, const double *gv, int nvec,
+                                 int m[3]) {
+  m[0] = m[1] = m[2] = 0;
+  for (int i = 0; i < 3; ++i) {
+    if (i >= nvec) break;
+    const double x = ceil(gv[3 * i] * d[0] + gv[3 * i + 1] * d[1] + gv[3 * i + 2] * d[2] - 0.5);
+    d[0] -= x * rv[3 * i];
+    d[1] -= x * rv[3 * i + 1];
+    d[2] -= x * rv[3 * i + 2];
+    m[i] = (int)x;
+  }
+}
+
+// cell.c:124-143 -- d += r.R axis by axis.
+__device__ inline void add_vec_exact(double d[3], const double *rv, int nvec, const int r[3]) {
+  for (int i = 0; i < 3; ++i) {
+    if (i >= nvec) break;
+    const double ri = (double)r[i];
+    d[0] += ri * rv[3 * i];
+    d[1] += ri * rv[3 * i + 1];
+    d[2] += ri * rv[3 * i + 2];
+  }
+}
+
+// Membership of an image index in the half space Yaff enumerates
+// (nlist.c:137-180): the origin, then r2 > 0, or r2 == 0 and r1 > 0, or
+// r2 == r1 == 0 and r0 > 0.
+__host__ __device__ inline bool in_half_space(const int r[3]) {
+  if (r[2] != 0) return r[2] > 0;
+  if (r[1] != 0) return r[1] > 0;
+  return r[0] >= 0;
+}
+
+// Yaff's half row for the unordered pair (A >= B) whose partner image is
+// shifted by n_ab cells: delta0 = mic(pos_B - pos_A) (nlist.c:108-113),
+// r = n_ab + m and v = delta0 + r.R (nlist.c:120-123).  v and r are always
+// computed; the return value is false when r lies outside the image box
+// [-rmax, rmax] that the reference scans (such a row is not Yaff's).
+__device__ inline bool yaff_row_vector(const double *pA, const double *pB, bool self,
+                                       const int nab[3], const double *rv, const double *gv,
+                                       int nvec, const int *rmax, double v[3], int r[3]) {
+  int m[3] = {0, 0, 0};
+  if (self) {
+    v[0] = 0.0; v[1] = 0.0; v[2] = 0.0;
+  } else {
+    v[0] = pB[0] - pA[0];
+    v[1] = pB[1] - pA[1];
+    v[2] = pB[2] - pA[2];
+    mic_exact(v, rv, gv, nvec, m);
+  }
+  bool inside = true;
+  for (int i = 0; i < 3; ++i) {
+    r[i] = nab[i] + m[i];
+    if (i < nvec) {
+      if (r[i] > rmax[i] || r[i] < -rmax[i]) inside = false;
+    } else if (r[i] != 0) {
+      inside = false;
+    }
+  }
+  add_vec_exact(v, rv, nvec, r);
+  return inside;
+}
+
+__device__ inline bool nlow_nhigh_ok(int a, int b, int nlow, int nhigh) {
+  return (a >= nlow && b < nhigh) || (b >= nlow && a < nhigh);  // nlist.c:57
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace ynb

@@ -1,0 +1,406 @@
+// Ewald reciprocal space (yaff/pes/ewald.c:32-150) and the scaled-pair /
+// self correction (yaff/pes/ewald.c:278-349).
+//
+// Phases.  With f = gvecs.r the fractional coordinates, k.r = 2 pi g.f, so
+// exp(i k.r) for consecutive g0 at fixed (g1, g2) follows by multiplying with
+// exp(2 pi i f0).  The k vectors are grouped in rows (runs of consecutive g0);
+// a row or segment starts from one sincospi of the reduced phase
+// g.f - rint(g.f) (computed in FP64 in both precisions) and continues by the
+// recurrence.  Fractional coordinates are wrapped to [0, 1), which leaves
+// every phase unchanged modulo 2 pi.
+//
+// Pass 1 (k_ewald_sk): one lane per segment of <= L k vectors, atoms of a
+//   chunk broadcast from LDS; partial S(k) per (chunk, k).
+// Pass 2 (k_ewald_kreduce): one thread per k sums its chunks in a fixed
+//   order per group (atoms below nlow, between, above nhigh), forms the
+//   energy and virial terms of ewald.c:82-127 and the coefficients 2 c S(k).
+// Pass 3 (k_ewald_force): one lane per atom, the system's rows split in
+//   kchunks parts over blockIdx.y; partial gradients per part.
+// Pass 4 (k_ewald_gsum, k_ewald_sysreduce): fixed-order sums.
+// No floating-point atomics: results do not depend on scheduling or on the
+// other systems of a batch.
+#include "common.h"
+
+namespace ynb {
+namespace {
+
+constexpr int CHUNK = 256;  // atoms per structure-factor chunk (LDS)
+
+__global__ void k_ewald_prep(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, const double *pos,
+                             const double *charges, double *frac, double *step) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= npad) return;
+  const int sid = wave_sys[g >> 6];
+  double f[3] = {0.0, 0.0, 0.0}, q = 0.0, c = 1.0, s = 0.0;
+  if (sid >= 0) {
+    const ynb_sys &S = sys[sid];
+    if (g - S.atom0 < S.natom && S.nk > 0) {
+      const double p0 = pos[3 * g], p1 = pos[3 * g + 1], p2 = pos[3 * g + 2];
+      for (int k = 0; k < 3; ++k) {
+        const double x = fma(S.gvecs[3 * k + 2], p2, fma(S.gvecs[3 * k + 1], p1, S.gvecs[3 * k] * p0));
+        f[k] = x - floor(x);
+      }
+      q = charges[g];
+      sincospi(2.0 * f[0], &s, &c);
+    }
+  }
+  frac[g] = f[0];
+  frac[npad + g] = f[1];
+  frac[2 * npad + g] = f[2];
+  frac[3 * npad + g] = q;
+  step[g] = c;
+  step[npad + g] = s;
+}
+
+template <typename T>
+__device__ inline void sincospi_t(double x, T *s, T *c);
+template <>
+__device__ inline void sincospi_t<double>(double x, double *s, double *c) {
+  sincospi(x, s, c);
+}
+template <>
+__device__ inline void sincospi_t<float>(double x, float *s, float *c) {
+  sincospif((float)x, s, c);
+}
+
+// Phase g0 f0 + g1 f1 + g2 f2 in turns reduced to [-1/2, 1/2] in FP64.
+__device__ inline double reduced_phase(int g0, int g1, int g2, double f0, double f1, double f2) {
+  const double t = fma((double)g2, f2, fma((double)g1, f1, (double)g0 * f0));
+  return t - rint(t);
+}
+
+template <int PREC, int L>
+__global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *sys, ynb_ewald ew, int64_t npad) {
+  using T = typename std::conditional<PREC == YNB_FP64, double, float>::type;
+  __shared__ double l_f[3][CHUNK];
+  __shared__ T l_q[CHUNK];
+  __shared__ T l_st[2][CHUNK];
+  const int lane = threadIdx.x;
+  for (int w = blockIdx.x; w < ew.nwork; w += gridDim.x) {
+    const int4 wk = *(const int4 *)(ew.work + 4 * w);
+    const ynb_sys &S = sys[wk.x];
+    const int4 ch = *(const int4 *)(ew.chunk + 4 * wk.z);
+    const int64_t a0 = S.atom0 + ch.y;
+    const int na = ch.z;
+    __syncthreads();
+    for (int t = lane; t < na; t += 64) {
+      const int64_t g = a0 + t;
+      l_f[0][t] = ew.frac[g];
+      l_f[1][t] = ew.frac[npad + g];
+      l_f[2][t] = ew.frac[2 * npad + g];
+      l_q[t] = (T)ew.frac[3 * npad + g];
+      l_st[0][t] = (T)ew.step[g];
+      l_st[1][t] = (T)ew.step[npad + g];
+    }
+    __syncthreads();
+    const bool valid = lane < wk.w;
+    int4 sg = make_int4(0, 0, 0, 0);
+    int len = 0;
+    if (valid) {
+      sg = *(const int4 *)(ew.seg + 4 * (wk.y + lane));
+      len = ew.seg_len[wk.y + lane];
+    }
+    T ar[L], ai[L];
+#pragma unroll
+    for (int m = 0; m < L; ++m) ar[m] = ai[m] = T(0);
+    if (valid) {
+      for (int t = 0; t < na; ++t) {
+        const double ph = reduced_phase(sg.x, sg.y, sg.z, l_f[0][t], l_f[1][t], l_f[2][t]);
+        T s, c;
+        sincospi_t<T>(2.0 * ph, &s, &c);
+        const T q = l_q[t];
+        T zr = q * c, zi = q * s;
+        const T cr = l_st[0][t], ci = l_st[1][t];
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+          ar[m] += zr;
+          ai[m] += zi;
+          const T nr = zr * cr - zi * ci;
+          zi = zr * ci + zi * cr;
+          zr = nr;
+        }
+      }
+      double *out = ew.spart + 2 * (S.spart0 + (int64_t)(wk.z - S.chunk0) * S.nk + (sg.w - S.k0));
+#pragma unroll
+      for (int m = 0; m < L; ++m) {
+        if (m < len) {
+          out[2 * m] = (double)ar[m];
+          out[2 * m + 1] = (double)ai[m];
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_ewald ew) {
+  const int blk = blockIdx.x;
+  const int t = threadIdx.x;
+  const int k = blk * 256 + t;
+  const int sid = ew.kblk_sys[blk];
+  const ynb_sys &S = sys[sid];
+  const int lk = k - S.k0;
+  const double4 kv = *(const double4 *)(ew.kvec + 4 * (int64_t)k);
+  double e = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
+  double cf[6] = {0, 0, 0, 0, 0, 0};
+  if (kv.w != 0.0) {
+    double sc[3] = {0, 0, 0}, ss[3] = {0, 0, 0};
+    for (int c = 0; c < S.nchunk; ++c) {
+      const int grp = ew.chunk[4 * (S.chunk0 + c) + 3];
+      const double *p = ew.spart + 2 * (S.spart0 + (int64_t)c * S.nk + lk);
+      sc[grp] += p[0];
+      ss[grp] += p[1];
+    }
+    // ewald.c:82-95: S0 (mid), S1 (low), S2 (high).
+    double s = sc[0] * (sc[0] + 2.0 * (sc[1] + sc[2])) + ss[0] * (ss[0] + 2.0 * (ss[1] + ss[2]));
+    s += 2.0 * (sc[1] * sc[2] + ss[1] * ss[2]);
+    e = kv.w * s;
+    const double x = 2.0 * kv.w;
+    cf[0] = x * sc[0]; cf[1] = x * ss[0];
+    cf[2] = x * sc[1]; cf[3] = x * ss[1];
+    cf[4] = x * sc[2]; cf[5] = x * ss[2];
+    // ewald.c:113-127
+    const double ksq = kv.x * kv.x + kv.y * kv.y + kv.z * kv.z;
+    const double fac2 = 0.25 / S.alpha / S.alpha;
+    const double c = kv.w * (2.0 * (1.0 / ksq + fac2) * s);
+    v[0] = c * kv.x * kv.x;
+    v[1] = c * kv.y * kv.y;
+    v[2] = c * kv.z * kv.z;
+    v[3] = c * kv.y * kv.x;
+    v[4] = c * kv.z * kv.x;
+    v[5] = c * kv.z * kv.y;
+  }
+  double *o = ew.coef + 6 * (int64_t)k;
+  for (int c = 0; c < 6; ++c) o[c] = cf[c];
+  // Block sum in a fixed tree.
+  __shared__ double red[7][256];
+  red[0][t] = e;
+  for (int c = 0; c < 6; ++c) red[1 + c][t] = v[c];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off)
+      for (int c = 0; c < 7; ++c) red[c][t] += red[c][t + off];
+    __syncthreads();
+  }
+  if (t < 7) ew.kpart[8 * blk + t] = red[t][0];
+}
+
+template <int PREC, bool GROUPS>
+__global__ __launch_bounds__(64) void k_ewald_force(const ynb_sys *sys, const int32_t *wave_sys, ynb_ewald ew,
+                                                    int64_t npad) {
+  using T = typename std::conditional<PREC == YNB_FP64, double, float>::type;
+  const int64_t wave = blockIdx.x;
+  const int kc = blockIdx.y;
+  const int lane = threadIdx.x;
+  const int sid = uniform_int(wave_sys[wave]);
+  if (sid < 0) return;
+  const ynb_sys &S = sys[sid];
+  if (S.nk == 0) return;
+  const int64_t g = wave * 64 + lane;
+  const int li = (int)(g - S.atom0);
+  const bool active = li < S.natom;
+  const double f0 = ew.frac[g], f1 = ew.frac[npad + g], f2 = ew.frac[2 * npad + g];
+  const double q = ew.frac[3 * npad + g];
+  const T cr = (T)ew.step[g], ci = (T)ew.step[npad + g];
+  T wl = T(0), wh = T(0);
+  if (GROUPS) {
+    wh = (li < S.nhigh) ? T(1) : T(0);
+    wl = (li >= S.nlow) ? T(1) : T(0);
+  }
+  const int rb = S.row0 + (int)(((int64_t)S.nrow * kc) / ew.kchunks);
+  const int re = S.row0 + (int)(((int64_t)S.nrow * (kc + 1)) / ew.kchunks);
+  double G[3] = {0.0, 0.0, 0.0};
+  for (int row = rb; row < re; ++row) {
+    const int4 kr = *(const int4 *)(ew.krow + 4 * row);
+    const int kf = ew.krow_k[row];
+    T s, c;
+    sincospi_t<T>(2.0 * reduced_phase(kr.z, kr.x, kr.y, f0, f1, f2), &s, &c);
+    T zr = c, zi = s;
+    T R0 = T(0), R1 = T(0), R2 = T(0);
+    for (int m = 0; m < kr.w; ++m) {
+      const double4 kv = *(const double4 *)(ew.kvec + 4 * (int64_t)(kf + m));
+      const double *cf = ew.coef + 6 * (int64_t)(kf + m);
+      T ac = (T)cf[0], as = (T)cf[1];
+      if (GROUPS) {
+        ac += wh * (T)cf[4] + wl * (T)cf[2];
+        as += wh * (T)cf[5] + wl * (T)cf[3];
+      }
+      // ewald.c:104-111: x = 2c (S_c (-q sin) + S_s (q cos)); q applied below.
+      const T x = as * zr - ac * zi;
+      R0 += (T)kv.x * x;
+      R1 += (T)kv.y * x;
+      R2 += (T)kv.z * x;
+      const T nr = zr * cr - zi * ci;
+      zi = zr * ci + zi * cr;
+      zr = nr;
+    }
+    G[0] += (double)R0;
+    G[1] += (double)R1;
+    G[2] += (double)R2;
+  }
+  const double scale = active ? q / S.dielectric : 0.0;
+  double *o = ew.gpart + ((int64_t)kc * npad + g) * 3;
+  o[0] = G[0] * scale;
+  o[1] = G[1] * scale;
+  o[2] = G[2] * scale;
+}
+
+__global__ void k_ewald_gsum(const ynb_sys *sys, const int32_t *wave_sys, ynb_ewald ew, int64_t npad, double *gpos,
+                             int accumulate) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= npad) return;
+  const int sid = wave_sys[g >> 6];
+  if (sid < 0) return;
+  const ynb_sys &S = sys[sid];
+  if (S.nk == 0 || g - S.atom0 >= S.natom) return;
+  double G[3] = {0.0, 0.0, 0.0};
+  for (int kc = 0; kc < ew.kchunks; ++kc)
+    for (int c = 0; c < 3; ++c) G[c] += ew.gpart[((int64_t)kc * npad + g) * 3 + c];
+  for (int c = 0; c < 3; ++c) {
+    if (accumulate)
+      gpos[3 * g + c] += G[c];
+    else
+      gpos[3 * g + c] = G[c];
+  }
+}
+
+// out[s] = {E, V (3x3)}; ewald.c:128-149 (virial diagonal -= E, then 1/eps_r).
+__global__ void k_ewald_sysreduce(const ynb_sys *sys, int nsys, ynb_ewald ew, double *out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  const ynb_sys &S = sys[s];
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int b = S.k0 / 256; b < (S.k0 + S.nk) / 256; ++b)
+    for (int c = 0; c < 7; ++c) acc[c] += ew.kpart[8 * b + c];
+  const double e = acc[0];
+  const double f = (S.nk > 0) ? 1.0 / S.dielectric : 0.0;
+  double *o = out + 10 * (int64_t)s;
+  o[0] = e * f;
+  o[1] = (acc[1] - e) * f; o[2] = acc[4] * f; o[3] = acc[5] * f;
+  o[4] = acc[4] * f; o[5] = (acc[2] - e) * f; o[6] = acc[6] * f;
+  o[7] = acc[5] * f; o[8] = acc[6] * f; o[9] = (acc[3] - e) * f;
+}
+
+// ewald.c:278-349 for one system, rows processed in order by lane 0 of one
+// wavefront; the self term is a wave-strided sum in a fixed order.
+__global__ void k_ewald_corr(const ynb_sys *sysd, const double *pos, const double *q, const ynb_scaling_row *stab,
+                             int64_t nstab, double *gpos, int accumulate, double *out) {
+  const ynb_sys &S = *sysd;
+  const int lane = threadIdx.x;
+  const double alpha = S.alpha;
+  const double x = alpha / 1.7724538509055160273;  // M_SQRT_PI
+  double self = 0.0;
+  for (int i = S.nlow + lane; i < S.nhigh; i += 64) self += x * q[S.atom0 + i] * q[S.atom0 + i];
+  self = wave_sum(self);
+  const double f = 1.0 / S.dielectric;
+  if (!accumulate && gpos)
+    for (int i = lane; i < S.natom; i += 64)
+      for (int c = 0; c < 3; ++c) gpos[3 * (int64_t)(S.atom0 + i) + c] = 0.0;
+  __syncthreads();
+  if (lane != 0) return;
+  double energy = -self;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t r = 0; r < nstab; ++r) {
+    const int64_t a = stab[r].a, b = stab[r].b;
+    if (a < S.nlow && b < S.nlow) continue;
+    if (a >= S.nhigh && b >= S.nhigh) continue;
+    const int64_t ga = S.atom0 + a, gb = S.atom0 + b;
+    double dl[3] = {pos[3 * gb] - pos[3 * ga], pos[3 * gb + 1] - pos[3 * ga + 1], pos[3 * gb + 2] - pos[3 * ga + 2]};
+    int m[3];
+    mic_exact(dl, S.rvecs, S.gvecs, S.nvec, m);
+    const double d = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+    const double xx = alpha * d;
+    const double e = erf(xx) / d;
+    const double fac = (1 - stab[r].scale) * q[gb] * q[ga];
+    const double gg = -fac * (1.1283791670955125739 * alpha * exp(-xx * xx) - e) / d / d;
+    if (gpos) {
+      for (int c = 0; c < 3; ++c) {
+        gpos[3 * gb + c] += dl[c] * gg * f;
+        gpos[3 * ga + c] -= dl[c] * gg * f;
+      }
+    }
+    v[0] += dl[0] * dl[0] * gg;
+    v[4] += dl[1] * dl[1] * gg;
+    v[8] += dl[2] * dl[2] * gg;
+    const double v01 = dl[1] * dl[0] * gg, v02 = dl[2] * dl[0] * gg, v12 = dl[2] * dl[1] * gg;
+    v[1] += v01; v[3] += v01;
+    v[2] += v02; v[6] += v02;
+    v[5] += v12; v[7] += v12;
+    energy -= fac * e;
+  }
+  out[0] = energy * f;
+  for (int c = 0; c < 9; ++c) out[1 + c] = v[c] * f;
+}
+
+}  // namespace
+}  // namespace ynb
+
+using namespace ynb;
+
+extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const double *pos, const double *charges,
+                              int32_t precision, double *gpos, int32_t accumulate, double *out, ynb_stream stream) {
+  if (!b || !ew || !pos || !charges || !out || b->npad % 64 != 0 || ew->nk_tot % 256 != 0 || ew->kchunks < 1 ||
+      (ew->seglen != 4 && ew->seglen != 8 && ew->seglen != 16) ||
+      (precision != YNB_FP64 && precision != YNB_MIXED)) {
+    set_error("ynb_ewald_reci: invalid arguments");
+    return YNB_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t npad = b->npad;
+  hipLaunchKernelGGL(k_ewald_prep, dim3((unsigned)ceil_div(npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys, npad,
+                     pos, charges, ew->frac, ew->step);
+  YNB_LAUNCHED();
+  if (ew->nwork > 0) {
+    const dim3 grid((unsigned)(ew->nwork < 65535 ? ew->nwork : 65535));
+#define YNB_SK(P, LL) hipLaunchKernelGGL((k_ewald_sk<P, LL>), grid, dim3(64), 0, st, b->sys, *ew, npad)
+    if (precision == YNB_FP64) {
+      if (ew->seglen == 4) YNB_SK(YNB_FP64, 4);
+      else if (ew->seglen == 8) YNB_SK(YNB_FP64, 8);
+      else YNB_SK(YNB_FP64, 16);
+    } else {
+      if (ew->seglen == 4) YNB_SK(YNB_MIXED, 4);
+      else if (ew->seglen == 8) YNB_SK(YNB_MIXED, 8);
+      else YNB_SK(YNB_MIXED, 16);
+    }
+#undef YNB_SK
+    YNB_LAUNCHED();
+  }
+  if (ew->nk_tot > 0) {
+    hipLaunchKernelGGL(k_ewald_kreduce, dim3(ew->nk_tot / 256), dim3(256), 0, st, b->sys, *ew);
+    YNB_LAUNCHED();
+  }
+  const dim3 fgrid((unsigned)(npad / 64), (unsigned)ew->kchunks);
+  if (precision == YNB_FP64) {
+    if (ew->groups)
+      hipLaunchKernelGGL((k_ewald_force<YNB_FP64, true>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
+    else
+      hipLaunchKernelGGL((k_ewald_force<YNB_FP64, false>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
+  } else {
+    if (ew->groups)
+      hipLaunchKernelGGL((k_ewald_force<YNB_MIXED, true>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
+    else
+      hipLaunchKernelGGL((k_ewald_force<YNB_MIXED, false>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
+  }
+  YNB_LAUNCHED();
+  if (gpos) {
+    hipLaunchKernelGGL(k_ewald_gsum, dim3((unsigned)ceil_div(npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys, *ew,
+                       npad, gpos, accumulate);
+    YNB_LAUNCHED();
+  }
+  hipLaunchKernelGGL(k_ewald_sysreduce, dim3((unsigned)ceil_div(b->nsys, 64)), dim3(64), 0, st, b->sys, b->nsys, *ew,
+                     out);
+  YNB_LAUNCHED();
+  return YNB_OK;
+}
+
+extern "C" int ynb_ewald_corr(const ynb_sys *h_sys, const ynb_sys *d_sys, const double *pos, const double *charges,
+                              const ynb_scaling_row *stab, int64_t nstab, double *gpos, int32_t accumulate, double *out,
+                              ynb_stream stream) {
+  if (!h_sys || !d_sys || !pos || !charges || !out || nstab < 0 || (nstab > 0 && !stab)) {
+    set_error("ynb_ewald_corr: invalid arguments");
+    return YNB_EINVAL;
+  }
+  hipLaunchKernelGGL(k_ewald_corr, dim3(1), dim3(64), 0, (hipStream_t)stream, d_sys, pos, charges, stab, nstab, gpos,
+                     accumulate, out);
+  YNB_LAUNCHED();
+  return YNB_OK;
+}

@@ -1,0 +1,586 @@
+// Neighbour lists: cell-binned build for 3-D periodic systems, all-pairs build
+// for 0/1/2-D systems, Yaff-ordered row materialisation and the skin-path
+// recompute.
+//
+// Pair set.  Yaff keeps the half row (A >= B, image r in its half space) when
+// |v|^2 < rc^2 with v = mic(pos_B - pos_A) + r.R evaluated exactly as
+// nlist_build_low does (yaff/pes/nlist.c:30-131).  The binned search below is
+// only a conservative candidate filter (padded radius); every candidate is
+// then re-tested with that exact expression (yaff_row_vector), so the
+// accepted set is the reference's bit for bit, including the image box
+// |r_i| <= rmax_i that limits Yaff in skewed cells.
+#include <hipcub/hipcub.hpp>
+
+#include "common.h"
+
+namespace ynb {
+
+namespace {
+
+struct Work {
+  int32_t *key, *key_sorted, *val, *val_sorted;
+  double4 *xw, *sxw;
+  int4 *wr, *sw;
+  int32_t *bin_start, *bin_end;
+  void *cub;
+  size_t cub_bytes;
+  size_t total;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+size_t cub_sort_bytes(int64_t n) {
+  size_t bytes = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (int32_t *)nullptr, (int32_t *)nullptr,
+                                     (int32_t *)nullptr, (int32_t *)nullptr, (int)n);
+  return bytes;
+}
+
+Work carve(void *base, int64_t npad, int32_t nbin) {
+  Work w;
+  char *p = (char *)base;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char *q = p ? p + off : nullptr;
+    off += align256(bytes);
+    return q;
+  };
+  w.key = (int32_t *)take(4 * npad);
+  w.key_sorted = (int32_t *)take(4 * npad);
+  w.val = (int32_t *)take(4 * npad);
+  w.val_sorted = (int32_t *)take(4 * npad);
+  w.xw = (double4 *)take(32 * npad);
+  w.sxw = (double4 *)take(32 * npad);
+  w.wr = (int4 *)take(16 * npad);
+  w.sw = (int4 *)take(16 * npad);
+  w.bin_start = (int32_t *)take(4 * (size_t)(nbin > 0 ? nbin : 1));
+  w.bin_end = (int32_t *)take(4 * (size_t)(nbin > 0 ? nbin : 1));
+  w.cub_bytes = cub_sort_bytes(npad);
+  w.cub = take(w.cub_bytes);
+  w.total = off;
+  return w;
+}
+
+__device__ inline int floor_div(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Fractional wrap and bin of every atom of the binned systems.
+__global__ void k_bin(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, const double *pos,
+                      int32_t *key, int32_t *val, double4 *xw, int4 *wr) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= npad) return;
+  val[g] = (int32_t)g;
+  key[g] = INT32_MAX;
+  const int sid = wave_sys[g >> 6];
+  if (sid < 0) return;
+  const ynb_sys &S = sys[sid];
+  if (g - S.atom0 >= S.natom || S.nbin == 0) return;
+  const double p0 = pos[3 * g], p1 = pos[3 * g + 1], p2 = pos[3 * g + 2];
+  int c[3], w[3];
+  double x = p0, y = p1, z = p2;
+  for (int k = 0; k < 3; ++k) {
+    const double f = fma(S.gvecs[3 * k + 2], p2, fma(S.gvecs[3 * k + 1], p1, S.gvecs[3 * k] * p0));
+    const double fl = floor(f);
+    int ci = (int)((f - fl) * S.grid[k]);
+    ci = ci < 0 ? 0 : (ci >= S.grid[k] ? S.grid[k] - 1 : ci);
+    c[k] = ci;
+    w[k] = (int)fl;
+    x = fma(-fl, S.rvecs[3 * k], x);
+    y = fma(-fl, S.rvecs[3 * k + 1], y);
+    z = fma(-fl, S.rvecs[3 * k + 2], z);
+  }
+  key[g] = S.bin0 + (c[0] * S.grid[1] + c[1]) * S.grid[2] + c[2];
+  xw[g] = make_double4(x, y, z, 0.0);
+  wr[g] = make_int4(w[0], w[1], w[2], 0);
+}
+
+// Bin ranges in the sorted order and sorted copies of the wrapped atoms.
+__global__ void k_binrange(int64_t npad, const int32_t *key_sorted, const int32_t *val_sorted,
+                           const double4 *xw, const int4 *wr, int32_t *bin_start, int32_t *bin_end,
+                           double4 *sxw, int4 *sw) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= npad) return;
+  const int32_t k = key_sorted[p];
+  if (k == INT32_MAX) return;
+  if (p == 0 || key_sorted[p - 1] != k) bin_start[k] = (int32_t)p;
+  if (p == npad - 1 || key_sorted[p + 1] != k) bin_end[k] = (int32_t)(p + 1);
+  const int32_t g = val_sorted[p];
+  sxw[p] = xw[g];
+  int4 w = wr[g];
+  w.w = g;
+  sw[p] = w;
+}
+
+__device__ inline void store_entry(const ynb_nlist &nl, int64_t npad, int64_t gi, int cnt, int32_t gj,
+                                   int32_t img) {
+  if (cnt < nl.kcap) {
+    nl.j[(int64_t)cnt * npad + gi] = gj;
+    nl.img[(int64_t)cnt * npad + gi] = img;
+  }
+}
+
+__device__ inline void finish_count(const ynb_nlist &nl, int64_t gi, int cnt, bool bad) {
+  nl.cnt[gi] = cnt;
+  if (cnt > nl.kcap) atomicOr(nl.status, 1);
+  if (bad) atomicOr(nl.status, 2);
+  atomicMax(nl.status + 1, cnt);
+}
+
+// One 64-lane block per bin: lanes own the bin's atoms, partner bins are
+// staged through LDS 64 atoms at a time.
+__global__ __launch_bounds__(64) void k_build_binned(const ynb_sys *sys, const int32_t *bin_sys,
+                                                     int64_t npad, const double *pos,
+                                                     const int32_t *bin_start, const int32_t *bin_end,
+                                                     const double4 *sxw, const int4 *sw, ynb_nlist nl) {
+  __shared__ double4 l_xw[64];
+  __shared__ int4 l_w[64];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int sid = bin_sys[b];
+  const ynb_sys &S = sys[sid];
+  const int start = bin_start[b], end = bin_end[b];
+  if (end <= start) return;
+  const int lb = b - S.bin0;
+  const int c2 = lb % S.grid[2], c1 = (lb / S.grid[2]) % S.grid[1], c0 = lb / (S.grid[1] * S.grid[2]);
+  const double rc2 = S.rc * S.rc;
+  const double rcp = S.rc * (1.0 + 1e-8) + 1e-6;
+  const double rcp2 = rcp * rcp;
+  for (int ib = start; ib < end; ib += 64) {
+    const int p = ib + lane;
+    const bool active = p < end;
+    double4 xi = make_double4(0, 0, 0, 0);
+    int4 wi = make_int4(0, 0, 0, -1);
+    if (active) {
+      xi = sxw[p];
+      wi = sw[p];
+    }
+    const int64_t gi = wi.w;
+    const int li = (int)(gi - S.atom0);
+    double pi[3] = {0, 0, 0};
+    if (active) {
+      pi[0] = pos[3 * gi];
+      pi[1] = pos[3 * gi + 1];
+      pi[2] = pos[3 * gi + 2];
+    }
+    int cnt = 0;
+    bool bad = false;
+    for (int d0 = -S.range[0]; d0 <= S.range[0]; ++d0) {
+      int nb0 = c0 + d0, s0 = floor_div(nb0, S.grid[0]);
+      nb0 -= s0 * S.grid[0];
+      for (int d1 = -S.range[1]; d1 <= S.range[1]; ++d1) {
+        int nb1 = c1 + d1, s1 = floor_div(nb1, S.grid[1]);
+        nb1 -= s1 * S.grid[1];
+        for (int d2 = -S.range[2]; d2 <= S.range[2]; ++d2) {
+          int nb2 = c2 + d2, s2 = floor_div(nb2, S.grid[2]);
+          nb2 -= s2 * S.grid[2];
+          const int nbin = S.bin0 + (nb0 * S.grid[1] + nb1) * S.grid[2] + nb2;
+          const double sx = s0 * S.rvecs[0] + s1 * S.rvecs[3] + s2 * S.rvecs[6];
+          const double sy = s0 * S.rvecs[1] + s1 * S.rvecs[4] + s2 * S.rvecs[7];
+          const double sz = s0 * S.rvecs[2] + s1 * S.rvecs[5] + s2 * S.rvecs[8];
+          const int js = bin_start[nbin], je = bin_end[nbin];
+          for (int jb = js; jb < je; jb += 64) {
+            __syncthreads();
+            if (jb + lane < je) {
+              l_xw[lane] = sxw[jb + lane];
+              l_w[lane] = sw[jb + lane];
+            }
+            __syncthreads();
+            if (!active) continue;
+            const int nj = min(64, je - jb);
+            for (int t = 0; t < nj; ++t) {
+              const double4 xj = l_xw[t];
+              const double dx = xj.x + sx - xi.x, dy = xj.y + sy - xi.y, dz = xj.z + sz - xi.z;
+              const double d2 = fma(dx, dx, fma(dy, dy, dz * dz));
+              if (!(d2 < rcp2)) continue;
+              const int4 wj = l_w[t];
+              const int gj = wj.w;
+              const int n[3] = {s0 - wj.x + wi.x, s1 - wj.y + wi.y, s2 - wj.z + wi.z};
+              const bool self = (gj == gi);
+              if (self && n[0] == 0 && n[1] == 0 && n[2] == 0) continue;
+              const int lj = gj - S.atom0;
+              const int A = li > lj ? li : lj, B = li > lj ? lj : li;
+              if (!nlow_nhigh_ok(A, B, S.nlow, S.nhigh)) continue;
+              const bool i_is_a = (li == A);
+              const int nab[3] = {i_is_a ? n[0] : -n[0], i_is_a ? n[1] : -n[1], i_is_a ? n[2] : -n[2]};
+              double pj[3] = {pos[3 * (int64_t)gj], pos[3 * (int64_t)gj + 1], pos[3 * (int64_t)gj + 2]};
+              const double *pA = i_is_a ? pi : pj;
+              const double *pB = i_is_a ? pj : pi;
+              double v[3];
+              int r[3];
+              if (!yaff_row_vector(pA, pB, self, nab, S.rvecs, S.gvecs, 3, S.rmax, v, r)) continue;
+              const double e2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+              if (!(e2 < rc2)) continue;
+              if (n[0] < -511 || n[0] > 511 || n[1] < -511 || n[1] > 511 || n[2] < -511 || n[2] > 511) {
+                bad = true;
+                continue;
+              }
+              store_entry(nl, npad, gi, cnt, gj, pack_img(n[0], n[1], n[2]));
+              ++cnt;
+            }
+          }
+        }
+      }
+    }
+    if (active) finish_count(nl, gi, cnt, bad);
+  }
+}
+
+// All-pairs build for systems without bins (nvec < 3): every partner and
+// every image of the full box [-rmax, rmax]^nvec, tested exactly.
+__global__ void k_build_allpairs(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
+                                 const double *pos, ynb_nlist nl) {
+  const int64_t gi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (gi >= npad) return;
+  const int sid = wave_sys[gi >> 6];
+  if (sid < 0) return;
+  const ynb_sys &S = sys[sid];
+  const int li = (int)(gi - S.atom0);
+  if (li >= S.natom || S.nbin != 0) return;
+  const double rc2 = S.rc * S.rc;
+  const int nvec = S.nvec;
+  const int R0 = nvec > 0 ? S.rmax[0] : 0, R1 = nvec > 1 ? S.rmax[1] : 0, R2 = nvec > 2 ? S.rmax[2] : 0;
+  const double pi[3] = {pos[3 * gi], pos[3 * gi + 1], pos[3 * gi + 2]};
+  int cnt = 0;
+  for (int lj = 0; lj < S.natom; ++lj) {
+    const int64_t gj = S.atom0 + lj;
+    const int A = li > lj ? li : lj, B = li > lj ? lj : li;
+    if (!nlow_nhigh_ok(A, B, S.nlow, S.nhigh)) continue;
+    const bool self = (lj == li);
+    const bool i_is_a = (li == A);
+    const double pj[3] = {pos[3 * gj], pos[3 * gj + 1], pos[3 * gj + 2]};
+    const double *pA = i_is_a ? pi : pj;
+    const double *pB = i_is_a ? pj : pi;
+    double d0[3] = {0.0, 0.0, 0.0};
+    int m[3] = {0, 0, 0};
+    if (!self) {
+      d0[0] = pB[0] - pA[0];
+      d0[1] = pB[1] - pA[1];
+      d0[2] = pB[2] - pA[2];
+      mic_exact(d0, S.rvecs, S.gvecs, nvec, m);
+    }
+    for (int r2 = -R2; r2 <= R2; ++r2)
+      for (int r1 = -R1; r1 <= R1; ++r1)
+        for (int r0 = -R0; r0 <= R0; ++r0) {
+          const int r[3] = {r0, r1, r2};
+          if (self && r0 == 0 && r1 == 0 && r2 == 0) continue;
+          double v[3] = {d0[0], d0[1], d0[2]};
+          add_vec_exact(v, S.rvecs, nvec, r);
+          const double e2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+          if (!(e2 < rc2)) continue;
+          int n[3] = {r0 - m[0], r1 - m[1], r2 - m[2]};
+          if (!i_is_a) {
+            n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2];
+          }
+          store_entry(nl, npad, gi, cnt, (int32_t)gj, pack_img(n[0], n[1], n[2]));
+          ++cnt;
+        }
+  }
+  finish_count(nl, gi, cnt, false);
+}
+
+// ---- Yaff rows ------------------------------------------------------------
+
+__device__ inline bool canonical_entry(int64_t gi, int32_t gj, int32_t img) {
+  if (gj < gi) return true;
+  if (gj > gi) return false;
+  int n[3];
+  unpack_img(img, n);
+  return in_half_space(n);
+}
+
+__global__ void k_rows_count(const ynb_sys *sys, int s, int64_t npad, ynb_nlist nl, int32_t *rowcnt) {
+  const ynb_sys &S = sys[s];
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= S.natom) return;
+  const int64_t gi = S.atom0 + li;
+  const int n = min(nl.cnt[gi], nl.kcap);
+  int c = 0;
+  for (int k = 0; k < n; ++k)
+    c += canonical_entry(gi, nl.j[(int64_t)k * npad + gi], nl.img[(int64_t)k * npad + gi]);
+  rowcnt[li] = c;
+}
+
+__global__ void k_rows_emit(const ynb_sys *sys, int s, int64_t npad, const double *pos, ynb_nlist nl,
+                            const int32_t *rowoff, ynb_neigh_row *tmp, uint64_t *key, int32_t *idx) {
+  const ynb_sys &S = sys[s];
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= S.natom) return;
+  const int64_t gi = S.atom0 + li;
+  const int n = min(nl.cnt[gi], nl.kcap);
+  int64_t o = rowoff[li];
+  const int nv = S.nvec;
+  const int R0 = nv > 0 ? S.rmax[0] : 0, R1 = nv > 1 ? S.rmax[1] : 0, R2 = nv > 2 ? S.rmax[2] : 0;
+  const uint64_t nimg = (uint64_t)(2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
+  const double pi[3] = {pos[3 * gi], pos[3 * gi + 1], pos[3 * gi + 2]};
+  for (int k = 0; k < n; ++k) {
+    const int32_t gj = nl.j[(int64_t)k * npad + gi];
+    const int32_t img = nl.img[(int64_t)k * npad + gi];
+    if (!canonical_entry(gi, gj, img)) continue;
+    int nn[3];
+    unpack_img(img, nn);
+    const int lj = (int)(gj - S.atom0);
+    const bool self = (lj == li);
+    // Canonical entries have i == A (the larger index).
+    const double pj[3] = {pos[3 * (int64_t)gj], pos[3 * (int64_t)gj + 1], pos[3 * (int64_t)gj + 2]};
+    double v[3];
+    int r[3];
+    yaff_row_vector(pi, pj, self, nn, S.rvecs, S.gvecs, nv, S.rmax, v, r);
+    const double d2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    ynb_neigh_row row;
+    const bool plus = in_half_space(r);
+    int rho[3];
+    if (plus) {
+      row.a = li; row.b = lj;
+      row.dx = v[0]; row.dy = v[1]; row.dz = v[2];
+      rho[0] = r[0]; rho[1] = r[1]; rho[2] = r[2];
+    } else {
+      row.a = lj; row.b = li;
+      row.dx = -v[0]; row.dy = -v[1]; row.dz = -v[2];
+      rho[0] = -r[0]; rho[1] = -r[1]; rho[2] = -r[2];
+    }
+    row.d = sqrt(d2);
+    row.r0 = rho[0]; row.r1 = rho[1]; row.r2 = rho[2];
+    tmp[o] = row;
+    // Yaff's loop order (nlist.c:51-120): A, then B, then the image in
+    // odometer order, then sign + before sign -.
+    const uint64_t pidx = (uint64_t)li * (uint64_t)S.natom + (uint64_t)lj;
+    const uint64_t iidx = ((uint64_t)(rho[2] + R2) * (2 * R1 + 1) + (uint64_t)(rho[1] + R1)) * (2 * R0 + 1) +
+                          (uint64_t)(rho[0] + R0);
+    key[o] = (pidx * nimg + iidx) * 2 + (plus ? 0 : 1);
+    idx[o] = (int32_t)o;
+    ++o;
+  }
+}
+
+__global__ void k_rows_gather(int64_t n, const ynb_neigh_row *tmp, const int32_t *idx, ynb_neigh_row *rows) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < n) rows[q] = tmp[idx[q]];
+}
+
+// nlist.c:183-232, one thread per row.
+__global__ void k_recompute(ynb_sys S, const double *pos, const double *pos_old, ynb_neigh_row *rows,
+                            int64_t n) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  ynb_neigh_row w = rows[q];
+  const int64_t a = S.atom0 + w.a, b = S.atom0 + w.b;
+  double d0[3] = {pos_old[3 * b] - pos_old[3 * a], pos_old[3 * b + 1] - pos_old[3 * a + 1],
+                  pos_old[3 * b + 2] - pos_old[3 * a + 2]};
+  int center[3] = {0, 0, 0};
+  for (int i = 0; i < S.nvec; ++i)
+    center[i] = (int)(-ceil(S.gvecs[3 * i] * d0[0] + S.gvecs[3 * i + 1] * d0[1] + S.gvecs[3 * i + 2] * d0[2] - 0.5));
+  d0[0] = pos[3 * b] - pos[3 * a];
+  d0[1] = pos[3 * b + 1] - pos[3 * a + 1];
+  d0[2] = pos[3 * b + 2] - pos[3 * a + 2];
+  add_vec_exact(d0, S.rvecs, S.nvec, center);
+  const int r[3] = {(int)w.r0, (int)w.r1, (int)w.r2};
+  add_vec_exact(d0, S.rvecs, S.nvec, r);
+  w.d = sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);
+  w.dx = d0[0];
+  w.dy = d0[1];
+  w.dz = d0[2];
+  rows[q] = w;
+}
+
+size_t rows_work_layout(int64_t natom, int64_t nrows, char *base, int32_t **rowcnt, int32_t **rowoff,
+                        ynb_neigh_row **tmp, uint64_t **key, uint64_t **key2, int32_t **idx, int32_t **idx2,
+                        void **cub, size_t *cub_bytes) {
+  size_t scan_bytes = 0, sort_bytes = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int32_t *)nullptr, (int32_t *)nullptr, (int)natom);
+  hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                     (int32_t *)nullptr, (int32_t *)nullptr, (int)(nrows > 0 ? nrows : 1));
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char *q = base ? base + off : nullptr;
+    off += align256(bytes);
+    return q;
+  };
+  *rowcnt = (int32_t *)take(4 * (natom + 1));
+  *rowoff = (int32_t *)take(4 * (natom + 1));
+  *tmp = (ynb_neigh_row *)take(sizeof(ynb_neigh_row) * nrows);
+  *key = (uint64_t *)take(8 * nrows);
+  *key2 = (uint64_t *)take(8 * nrows);
+  *idx = (int32_t *)take(4 * nrows);
+  *idx2 = (int32_t *)take(4 * nrows);
+  *cub_bytes = scan_bytes > sort_bytes ? scan_bytes : sort_bytes;
+  *cub = take(*cub_bytes);
+  return off;
+}
+
+}  // namespace
+
+}  // namespace ynb
+
+using namespace ynb;
+
+extern "C" {
+
+int ynb_sys_plan(ynb_sys *s) {
+  if (!s) {
+    set_error("ynb_sys_plan: null system");
+    return YNB_EINVAL;
+  }
+  if (s->natom < 0 || s->rc <= 0.0 || s->nvec < 0 || s->nvec > 3) {
+    set_error("ynb_sys_plan: invalid natom/rc/nvec (%d, %g, %d)", s->natom, s->rc, s->nvec);
+    return YNB_EINVAL;
+  }
+  s->nbin = 0;
+  for (int k = 0; k < 3; ++k) {
+    s->grid[k] = 1;
+    s->range[k] = 0;
+  }
+  if (s->nvec < 3 || s->natom == 0) return YNB_OK;
+  const double rcp = s->rc * (1.0 + 1e-8) + 1e-6;
+  int64_t nbin = 1;
+  for (int k = 0; k < 3; ++k) {
+    const double *g = s->gvecs + 3 * k;
+    const double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);  // 1/plane spacing
+    const double spacing = 1.0 / gn;
+    int grid = (int)floor(spacing / rcp);
+    if (grid < 1) grid = 1;
+    if (grid > 1024) grid = 1024;
+    // Bins searched on each side: the fractional extent of a sphere of
+    // radius rc along axis k is rc*|g_k|.
+    int range = (int)ceil(rcp * gn * grid);
+    if (range < 1) range = 1;
+    s->grid[k] = grid;
+    s->range[k] = range;
+    nbin *= grid;
+  }
+  // Keep at least a few atoms per bin on average.
+  while (nbin > 1 && nbin * 4 > (int64_t)s->natom * 8) {
+    int kmax = 0;
+    for (int k = 1; k < 3; ++k)
+      if (s->grid[k] > s->grid[kmax]) kmax = k;
+    if (s->grid[kmax] == 1) break;
+    s->grid[kmax] = (s->grid[kmax] + 1) / 2;
+    const double *g = s->gvecs + 3 * kmax;
+    const double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    s->range[kmax] = (int)ceil(rcp * gn * s->grid[kmax]);
+    if (s->range[kmax] < 1) s->range[kmax] = 1;
+    nbin = (int64_t)s->grid[0] * s->grid[1] * s->grid[2];
+  }
+  s->nbin = (int32_t)nbin;
+  return YNB_OK;
+}
+
+size_t ynb_nlist_workspace_bytes(int64_t npad, int32_t nbin) { return carve(nullptr, npad, nbin).total; }
+
+int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t work_bytes,
+                    const ynb_nlist *nl, ynb_stream stream) {
+  if (!b || !nl || !pos || b->npad % 64 != 0 || nl->kcap < 1) {
+    set_error("ynb_nlist_build: invalid arguments");
+    return YNB_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  Work w = carve(work, b->npad, b->nbin);
+  if (work_bytes < w.total || !work) {
+    set_error("ynb_nlist_build: workspace of %zu bytes, need %zu", work_bytes, w.total);
+    return YNB_EWORKSPACE;
+  }
+  const int64_t npad = b->npad;
+  const int tb = 256;
+  const int nb = (int)ceil_div(npad, tb);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, nl->status, 4, 0);
+  YNB_LAUNCHED();
+  if (b->nbin > 0) {
+    const int nbb = (int)ceil_div(b->nbin, tb);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nbb), dim3(tb), 0, st, w.bin_start, (int64_t)b->nbin, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nbb), dim3(tb), 0, st, w.bin_end, (int64_t)b->nbin, 0);
+    hipLaunchKernelGGL(k_bin, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, w.key, w.val, w.xw,
+                       w.wr);
+    YNB_LAUNCHED();
+    size_t cb = w.cub_bytes;
+    YNB_HIP(hipcub::DeviceRadixSort::SortPairs(w.cub, cb, w.key, w.key_sorted, w.val, w.val_sorted,
+                                               (int)npad, 0, 32, st));
+    hipLaunchKernelGGL(k_binrange, dim3(nb), dim3(tb), 0, st, npad, w.key_sorted, w.val_sorted, w.xw, w.wr,
+                       w.bin_start, w.bin_end, w.sxw, w.sw);
+    YNB_LAUNCHED();
+    hipLaunchKernelGGL(k_build_binned, dim3(b->nbin), dim3(64), 0, st, b->sys, b->bin_sys, npad, pos,
+                       w.bin_start, w.bin_end, w.sxw, w.sw, *nl);
+    YNB_LAUNCHED();
+  }
+  hipLaunchKernelGGL(k_build_allpairs, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, *nl);
+  YNB_LAUNCHED();
+  return YNB_OK;
+}
+
+size_t ynb_rows_workspace_bytes(int64_t natom, int64_t nrows) {
+  int32_t *a, *c, *e, *f;
+  ynb_neigh_row *t;
+  uint64_t *k, *k2;
+  void *cub;
+  size_t cb;
+  return rows_work_layout(natom, nrows, nullptr, &a, &c, &t, &k, &k2, &e, &f, &cub, &cb);
+}
+
+int ynb_nlist_rows(const ynb_batch *b, int32_t s, const double *pos, const ynb_nlist *nl, void *work,
+                   size_t work_bytes, ynb_neigh_row *rows, int64_t cap, int64_t *count, ynb_stream stream) {
+  if (!b || !nl || !count || s < 0 || s >= b->nsys) {
+    set_error("ynb_nlist_rows: invalid arguments");
+    return YNB_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  ynb_sys S;
+  YNB_HIP(hipMemcpyAsync(&S, b->sys + s, sizeof(ynb_sys), hipMemcpyDeviceToHost, st));
+  YNB_HIP(hipStreamSynchronize(st));
+  const int64_t natom = S.natom;
+  const int64_t nrows_cap = cap > 0 ? cap : 0;
+  int32_t *rowcnt, *rowoff, *idx, *idx2;
+  ynb_neigh_row *tmp;
+  uint64_t *key, *key2;
+  void *cub;
+  size_t cub_bytes;
+  const size_t need = rows_work_layout(natom, nrows_cap, (char *)work, &rowcnt, &rowoff, &tmp, &key, &key2, &idx,
+                                       &idx2, &cub, &cub_bytes);
+  if (!work || work_bytes < need) {
+    set_error("ynb_nlist_rows: workspace of %zu bytes, need %zu", work_bytes, need);
+    return YNB_EWORKSPACE;
+  }
+  if (natom == 0) {
+    *count = 0;
+    return YNB_OK;
+  }
+  const int tb = 256;
+  const int nb = (int)ceil_div(natom, tb);
+  hipLaunchKernelGGL(k_rows_count, dim3(nb), dim3(tb), 0, st, b->sys, s, b->npad, *nl, rowcnt);
+  YNB_LAUNCHED();
+  size_t cb = cub_bytes;
+  YNB_HIP(hipcub::DeviceScan::ExclusiveSum(cub, cb, rowcnt, rowoff, (int)natom, st));
+  int32_t last[2];
+  YNB_HIP(hipMemcpyAsync(&last[0], rowoff + natom - 1, 4, hipMemcpyDeviceToHost, st));
+  YNB_HIP(hipMemcpyAsync(&last[1], rowcnt + natom - 1, 4, hipMemcpyDeviceToHost, st));
+  YNB_HIP(hipStreamSynchronize(st));
+  const int64_t total = (int64_t)last[0] + last[1];
+  *count = total;
+  if (total > nrows_cap || !rows) return YNB_OK;
+  hipLaunchKernelGGL(k_rows_emit, dim3(nb), dim3(tb), 0, st, b->sys, s, b->npad, pos, *nl, rowoff, tmp, key, idx);
+  YNB_LAUNCHED();
+  if (total > 0) {
+    cb = cub_bytes;
+    YNB_HIP(hipcub::DeviceRadixSort::SortPairs(cub, cb, key, key2, idx, idx2, (int)total, 0, 64, st));
+    hipLaunchKernelGGL(k_rows_gather, dim3((int)ceil_div(total, tb)), dim3(tb), 0, st, total, tmp, idx2, rows);
+    YNB_LAUNCHED();
+  }
+  YNB_HIP(hipStreamSynchronize(st));
+  return YNB_OK;
+}
+
+int ynb_nlist_recompute(const ynb_sys *h_sys, const double *pos, const double *pos_old, ynb_neigh_row *rows,
+                        int64_t nrow, ynb_stream stream) {
+  if (!h_sys || nrow < 0) {
+    set_error("ynb_nlist_recompute: invalid arguments");
+    return YNB_EINVAL;
+  }
+  if (nrow == 0) return YNB_OK;
+  hipLaunchKernelGGL(k_recompute, dim3((int)ceil_div(nrow, 256)), dim3(256), 0, (hipStream_t)stream, *h_sys, pos,
+                     pos_old, rows, nrow);
+  YNB_LAUNCHED();
+  return YNB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,350 @@
+// Pair potentials over the full neighbour list, all potential slots of a
+// system in one pass (Yaff makes one pass per ForcePartPair,
+// yaff/pes/pair_pot.c:84-176).
+//
+// One lane per atom i walks its ELL entries; both directions of a pair are in
+// the list, so the lane accumulates its own gradient in registers without
+// atomics and energy/virial get a factor 1/2.  Per-wave sums of energy and
+// virial go to `partial`; k_pair_reduce adds the waves of each system in a
+// fixed order, so results are deterministic and independent of the other
+// systems of the batch.
+//
+// FP64 mode rebuilds every row vector exactly as nlist_build_low does
+// (yaff_row_vector), so d, the cutoff test d < rcut and the scaling lookup on
+// r == 0 rows see Yaff's bits.  Mixed mode takes the image vector
+// pos_j - pos_i + n.R in FP64 and evaluates the potentials in FP32,
+// accumulating gradient, energy and virial in FP64.
+#include "common.h"
+
+namespace ynb {
+namespace {
+
+constexpr double TWO_DIV_SQRT_PI = 1.1283791670955125739;  // yaff/pes/constants.h:31
+
+template <typename T>
+struct PotEval {
+  // Returns the energy; g receives dE/dd / d (Yaff's vg).
+  __device__ static T lj(T sa, T sb, T ea, T eb, T d, T &g) {  // pair_pot.c:277-301
+    const T sigma = T(0.5) * (sa + sb);
+    const T eps = sqrt(ea * eb);
+    T x = sigma / d;
+    x *= x;
+    x *= x * x;
+    g = T(24.0) * eps / d / d * x * (T(1.0) - T(2.0) * x);
+    return T(4.0) * eps * (x * (x - T(1.0)));
+  }
+  __device__ static T mm3(T sa, T sb, T ea, T eb, int pauli, T d, T &g) {  // pair_pot.c:321-352
+    const T sigma = sa + sb;
+    const T eps = sqrt(ea * eb);
+    T x = sigma / d;
+    const T e = T(1.84e5) * exp(T(-12.0) / x);
+    if (pauli != 0) {
+      g = eps / d * (T(-12.0) / sigma * e);
+      return eps * e;
+    }
+    x *= x;
+    x *= T(2.25) * x * x;
+    g = eps / d * (T(-12.0) / sigma * e + T(6.0) / d * x);
+    return eps * (e - x);
+  }
+  __device__ static T exprep(T amp, T b, T d, T &g) {  // pair_pot.c:473-489
+    if (amp == T(0) || b == T(0)) {
+      g = T(0);
+      return T(0);
+    }
+    const T e = amp * exp(-b * d);
+    g = -e * b / d;
+    return e;
+  }
+  __device__ static T ei(T qa, T qb, T ra, T rb, T alpha, T diel, T d, T &g) {  // pair_pot.c:711-751
+    const T qq = (qa * qb) / diel;
+    const T rab = sqrt(ra * ra + rb * rb);
+    T e;
+    if (alpha > T(0)) {
+      const T x = alpha * d;
+      if (rab > T(0)) {
+        const T y = d / rab;
+        e = qq / d * (erfc(x) - erfc(y));
+        g = (T(TWO_DIV_SQRT_PI) * (exp(-y * y) / rab - exp(-x * x) * alpha) * qq - e) / (d * d);
+      } else {
+        e = qq * erfc(x) / d;
+        g = (-T(TWO_DIV_SQRT_PI) * alpha * exp(-x * x) * qq - e) / (d * d);
+      }
+    } else {
+      if (rab > T(0)) {
+        const T y = d / rab;
+        e = qq / d * erf(y);
+        g = (T(TWO_DIV_SQRT_PI) / rab * exp(-y * y) * qq - e) / (d * d);
+      } else {
+        e = qq / d;
+        g = -e / (d * d);
+      }
+    }
+    return e;
+  }
+  __device__ static T switch3(T d, T rcut, T w, T &hg) {  // truncation.c:59-76
+    if (d < rcut) {
+      T x = rcut - d;
+      if (x > w) {
+        hg = T(0);
+        return T(1);
+      }
+      x /= w;
+      hg = T(-6) * x * (T(1) - x) / w;
+      return (T(3) - T(2) * x) * x * x;
+    }
+    hg = T(0);
+    return T(0);
+  }
+  __device__ static T hammer(T d, T rcut, T tau, T &hg) {  // truncation.c:30-41
+    if (d < rcut) {
+      const T x = d - rcut;
+      const T h = exp(tau / x);
+      hg = -h * tau / x / x;
+      return h;
+    }
+    hg = T(0);
+    return T(0);
+  }
+};
+
+// pair_pot.c:65-81 as a binary search over the system's sorted rows.
+__device__ inline double stab_scale(const ynb_scaling_row *st, int n, int64_t a, int64_t b) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const ynb_scaling_row &r = st[mid];
+    if (r.a < a || (r.a == a && r.b < b))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo < n && st[lo].a == a && st[lo].b == b) return st[lo].scale;
+  return 1.0;
+}
+
+template <int PREC, bool SLOTS>
+__global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
+                                              ynb_pairpar pp, const double *pos, ynb_nlist nl, uint32_t mask,
+                                              double *gpos, int accumulate, double *gpos_slots,
+                                              double *partial) {
+  using T = typename std::conditional<PREC == YNB_FP64, double, float>::type;
+  const int64_t gi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t wave = gi >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= npad / 64) return;  // npad is a multiple of 64: whole waves leave
+  const int sid = uniform_int(wave_sys[wave]);
+  if (sid < 0) return;
+  const ynb_sys &S = sys[sid];
+  const int li = (int)(gi - S.atom0);
+  const bool active = li < S.natom;
+  const int npot = S.npot;
+
+  double acc_e[YNB_MAX_POT], acc_v[YNB_MAX_POT][6], acc_g[SLOTS ? YNB_MAX_POT : 1][3];
+  for (int p = 0; p < YNB_MAX_POT; ++p) {
+    acc_e[p] = 0.0;
+    for (int c = 0; c < 6; ++c) acc_v[p][c] = 0.0;
+  }
+  for (int p = 0; p < (SLOTS ? YNB_MAX_POT : 1); ++p) acc_g[p][0] = acc_g[p][1] = acc_g[p][2] = 0.0;
+
+  if (active) {
+    const double pi[3] = {pos[3 * gi], pos[3 * gi + 1], pos[3 * gi + 2]};
+    double par_i[YNB_MAX_POT][2];
+    for (int p = 0; p < YNB_MAX_POT; ++p) {
+      par_i[p][0] = (p < npot) ? pp.par[((int64_t)p * 3 + 0) * npad + gi] : 0.0;
+      par_i[p][1] = (p < npot) ? pp.par[((int64_t)p * 3 + 1) * npad + gi] : 0.0;
+    }
+    const int n = min(nl.cnt[gi], nl.kcap);
+    for (int k = 0; k < n; ++k) {
+      const int64_t gj = nl.j[(int64_t)k * npad + gi];
+      int nn[3];
+      unpack_img(nl.img[(int64_t)k * npad + gi], nn);
+      const int lj = (int)(gj - S.atom0);
+      const bool self = (gj == gi);
+      const double pj[3] = {pos[3 * gj], pos[3 * gj + 1], pos[3 * gj + 2]};
+      double u[3];  // from atom i to the partner image
+      bool r_zero = false;
+      int A = li > lj ? li : lj, B = li > lj ? lj : li;
+      if (PREC == YNB_FP64) {
+        const bool i_is_a = (li == A);
+        const int nab[3] = {i_is_a ? nn[0] : -nn[0], i_is_a ? nn[1] : -nn[1], i_is_a ? nn[2] : -nn[2]};
+        double v[3];
+        int r[3];
+        yaff_row_vector(i_is_a ? pi : pj, i_is_a ? pj : pi, self, nab, S.rvecs, S.gvecs, S.nvec, S.rmax, v, r);
+        r_zero = !self && r[0] == 0 && r[1] == 0 && r[2] == 0;
+        u[0] = i_is_a ? v[0] : -v[0];
+        u[1] = i_is_a ? v[1] : -v[1];
+        u[2] = i_is_a ? v[2] : -v[2];
+      } else {
+        if (S.nstab > 0 && !self) {
+          // Scaling applies to r == 0 rows only: r = n_ab + mic shift.
+          const bool i_is_a = (li == A);
+          const int nab[3] = {i_is_a ? nn[0] : -nn[0], i_is_a ? nn[1] : -nn[1], i_is_a ? nn[2] : -nn[2]};
+          double v[3];
+          int r[3];
+          yaff_row_vector(i_is_a ? pi : pj, i_is_a ? pj : pi, self, nab, S.rvecs, S.gvecs, S.nvec, S.rmax, v, r);
+          r_zero = r[0] == 0 && r[1] == 0 && r[2] == 0;
+        }
+        u[0] = pj[0] - pi[0];
+        u[1] = pj[1] - pi[1];
+        u[2] = pj[2] - pi[2];
+        for (int a = 0; a < 3; ++a) {
+          const double na = (double)nn[a];
+          u[0] = fma(na, S.rvecs[3 * a], u[0]);
+          u[1] = fma(na, S.rvecs[3 * a + 1], u[1]);
+          u[2] = fma(na, S.rvecs[3 * a + 2], u[2]);
+        }
+      }
+      const double d2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+      const double dd = sqrt(d2);
+      const T d = (T)dd;
+      for (int p = 0; p < YNB_MAX_POT; ++p) {
+        if (p >= npot || !((mask >> p) & 1u)) continue;
+        const ynb_pot &P = S.pot[p];
+        if (!(dd < P.rcut)) continue;
+        double s = 1.0;
+        if (S.nstab > 0 && r_zero) s = stab_scale(pp.stab + S.stab0, S.nstab, A, B);
+        if (s == 0.0) continue;
+        const int64_t gjj = gj;
+        T g = T(0), e;
+        switch (P.kind) {
+          case YNB_POT_LJ:
+            e = PotEval<T>::lj((T)par_i[p][0], (T)pp.par[((int64_t)p * 3) * npad + gjj], (T)par_i[p][1],
+                               (T)pp.par[((int64_t)p * 3 + 1) * npad + gjj], d, g);
+            break;
+          case YNB_POT_MM3: {
+            const int pa = (int)pp.par[((int64_t)p * 3 + 2) * npad + gi];
+            const int pb = (int)pp.par[((int64_t)p * 3 + 2) * npad + gjj];
+            e = PotEval<T>::mm3((T)par_i[p][0], (T)pp.par[((int64_t)p * 3) * npad + gjj], (T)par_i[p][1],
+                                (T)pp.par[((int64_t)p * 3 + 1) * npad + gjj], pa + pb, d, g);
+            break;
+          }
+          case YNB_POT_EXPREP: {
+            const int ta = (int)par_i[p][0];
+            const int tb = (int)pp.par[((int64_t)p * 3) * npad + gjj];
+            const int nf = P.nffatype;
+            const int64_t ci = P.cross0 + (int64_t)(ta * nf + tb);  // tables are symmetric
+            e = PotEval<T>::exprep((T)pp.cross[ci], (T)pp.cross[ci + (int64_t)nf * nf], d, g);
+            break;
+          }
+          default:
+            e = PotEval<T>::ei((T)par_i[p][0], (T)pp.par[((int64_t)p * 3) * npad + gjj], (T)par_i[p][1],
+                               (T)pp.par[((int64_t)p * 3 + 1) * npad + gjj], (T)P.alpha, (T)P.dielectric, d, g);
+            break;
+        }
+        if (P.trunc != YNB_TR_NONE && (e != T(0) || g != T(0))) {  // pair_pot.c:130-139
+          T hg;
+          const T h = (P.trunc == YNB_TR_SWITCH3) ? PotEval<T>::switch3(d, (T)P.rcut, (T)P.trunc_par, hg)
+                                                   : PotEval<T>::hammer(d, (T)P.rcut, (T)P.trunc_par, hg);
+          g = g * h + e * hg / d;
+          e *= h;
+        }
+        const double vg = (double)g * s;
+        acc_e[p] += s * (double)e;
+        const int gs = SLOTS ? p : 0;
+        acc_g[gs][0] -= u[0] * vg;
+        acc_g[gs][1] -= u[1] * vg;
+        acc_g[gs][2] -= u[2] * vg;
+        acc_v[p][0] += u[0] * (u[0] * vg);
+        acc_v[p][1] += u[1] * (u[1] * vg);
+        acc_v[p][2] += u[2] * (u[2] * vg);
+        acc_v[p][3] += u[0] * (u[1] * vg);
+        acc_v[p][4] += u[0] * (u[2] * vg);
+        acc_v[p][5] += u[1] * (u[2] * vg);
+      }
+    }
+    if (gpos) {
+      double G[3] = {0.0, 0.0, 0.0};
+      for (int p = 0; p < (SLOTS ? YNB_MAX_POT : 1); ++p)
+        for (int c = 0; c < 3; ++c) G[c] += acc_g[p][c];
+      for (int c = 0; c < 3; ++c) {
+        if (accumulate)
+          gpos[3 * gi + c] += G[c];
+        else
+          gpos[3 * gi + c] = G[c];
+      }
+    }
+    if (SLOTS && gpos_slots) {
+      for (int p = 0; p < YNB_MAX_POT; ++p)
+        for (int c = 0; c < 3; ++c) gpos_slots[((int64_t)p * npad + gi) * 3 + c] = acc_g[p][c];
+    }
+  }
+  // Wave sums of energy and virial (lanes past natom contribute zeros).
+  for (int p = 0; p < YNB_MAX_POT; ++p) {
+    if (p >= npot) break;
+    const double e = wave_sum(acc_e[p]);
+    double v[6];
+    for (int c = 0; c < 6; ++c) v[c] = wave_sum(acc_v[p][c]);
+    if (lane == 0) {
+      double *o = partial + (wave * YNB_MAX_POT + p) * 8;
+      o[0] = e;
+      for (int c = 0; c < 6; ++c) o[1 + c] = v[c];
+      o[7] = 0.0;
+    }
+  }
+}
+
+// One block per system; thread t < 32 sums value t over the system's waves in
+// order.  out[s][p] = {E, V00 V01 V02 V10 V11 V12 V20 V21 V22} with the 1/2 of
+// the full list applied.
+__global__ void k_pair_reduce(const ynb_sys *sys, const double *partial, double *out) {
+  const int s = blockIdx.x;
+  const ynb_sys &S = sys[s];
+  const int t = threadIdx.x;
+  __shared__ double red[YNB_MAX_POT * 8];
+  if (t < YNB_MAX_POT * 8) {
+    const int p = t / 8;
+    double acc = 0.0;
+    if (p < S.npot) {
+      const int64_t w0 = S.atom0 / 64, w1 = (S.atom0 + S.natom + 63) / 64;
+      for (int64_t w = w0; w < w1; ++w) acc += partial[(w * YNB_MAX_POT) * 8 + t];
+    }
+    red[t] = 0.5 * acc;
+  }
+  __syncthreads();
+  if (t < YNB_MAX_POT) {
+    const double *r = red + t * 8;
+    double *o = out + ((int64_t)s * YNB_MAX_POT + t) * 10;
+    o[0] = r[0];
+    o[1] = r[1]; o[2] = r[4]; o[3] = r[5];
+    o[4] = r[4]; o[5] = r[2]; o[6] = r[6];
+    o[7] = r[5]; o[8] = r[6]; o[9] = r[3];
+  }
+}
+
+}  // namespace
+}  // namespace ynb
+
+using namespace ynb;
+
+extern "C" int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const double *pos, const ynb_nlist *nl,
+                                uint32_t slot_mask, int32_t precision, double *gpos, int32_t accumulate,
+                                double *gpos_slots, double *partial, double *out, ynb_stream stream) {
+  if (!b || !pp || !nl || !pos || !partial || !out || b->npad % 64 != 0 ||
+      (precision != YNB_FP64 && precision != YNB_MIXED)) {
+    set_error("ynb_pair_compute: invalid arguments");
+    return YNB_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid((unsigned)ceil_div(b->npad, 256)), block(256);
+  const bool slots = gpos_slots != nullptr;
+  if (precision == YNB_FP64) {
+    if (slots)
+      hipLaunchKernelGGL((k_pair<YNB_FP64, true>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
+                         slot_mask, gpos, accumulate, gpos_slots, partial);
+    else
+      hipLaunchKernelGGL((k_pair<YNB_FP64, false>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
+                         slot_mask, gpos, accumulate, gpos_slots, partial);
+  } else {
+    if (slots)
+      hipLaunchKernelGGL((k_pair<YNB_MIXED, true>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
+                         slot_mask, gpos, accumulate, gpos_slots, partial);
+    else
+      hipLaunchKernelGGL((k_pair<YNB_MIXED, false>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
+                         slot_mask, gpos, accumulate, gpos_slots, partial);
+  }
+  YNB_LAUNCHED();
+  hipLaunchKernelGGL(k_pair_reduce, dim3(b->nsys), dim3(64), 0, st, b->sys, partial, out);
+  YNB_LAUNCHED();
+  return YNB_OK;
+}

@@ -1,0 +1,504 @@
+"""Batched device engine: lays out systems in HBM and drives libyaff_nb.so.
+
+A batch is any number of independent systems.  Their atoms are concatenated in
+padded ranges (each system starts at a multiple of 64 atoms, so one wavefront
+never spans two systems); every launch covers the whole batch.  A single
+system -- the drop-in ForceField path -- is a batch of one.
+
+Device memory is held in torch tensors (plumbing only); every kernel is a
+hand-written HIP kernel of libyaff_nb.so launched on torch's current stream,
+so a sequence of engine calls can be captured in a HIP graph.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import (YnbSys, YnbBatch, YnbNlist, YnbPairpar, YnbEwald, MAX_POT, FP64, MIXED, TR_NONE,
+                   ROW_DTYPE, SCALING_DTYPE, check)
+
+__all__ = ['PotSpec', 'EwaldSpec', 'NBSystem', 'Engine', 'ewald_kvectors']
+
+TWO_PI = 6.2831853071795864769
+FOUR_PI = 12.5663706143591729538
+CHUNK = 256          # atoms per structure-factor chunk (k_ewald_sk LDS tile)
+KBLOCK = 256         # k vectors per reduction block
+
+
+class PotSpec(object):
+    """One pair potential of a system.  p0/p1/p2 are per-atom parameters:
+    LJ/MM3 sigma, epsilon (MM3: p2 = onlypauli); EI charge, radius; ExpRep
+    ffatype id with amp_cross/b_cross tables."""
+
+    def __init__(self, kind, rcut, trunc=TR_NONE, trunc_par=0.0, p0=None, p1=None, p2=None, alpha=0.0,
+                 dielectric=1.0, amp_cross=None, b_cross=None):
+        self.kind, self.rcut, self.trunc, self.trunc_par = int(kind), float(rcut), int(trunc), float(trunc_par)
+        self.p0, self.p1, self.p2 = p0, p1, p2
+        self.alpha, self.dielectric = float(alpha), float(dielectric)
+        self.amp_cross, self.b_cross = amp_cross, b_cross
+
+
+class EwaldSpec(object):
+    def __init__(self, alpha, gcut, dielectric=1.0, nlow=0, nhigh=-1):
+        self.alpha, self.gcut, self.dielectric = float(alpha), float(gcut), float(dielectric)
+        self.nlow, self.nhigh = int(nlow), int(nhigh)
+
+
+class NBSystem(object):
+    """Input of the engine for one system.  ``rc`` is the neighbour-list radius
+    (largest pair cutoff plus skin); rmax defaults to nlist.py:132."""
+
+    def __init__(self, pos, cell, pots, rc=None, rmax=None, nlow=0, nhigh=-1, stab=None, ewald=None,
+                 charges=None, masses=None):
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64)
+        self.cell = cell
+        self.pots = list(pots)
+        if len(self.pots) > MAX_POT:
+            raise ValueError('at most %d pair potentials per system' % MAX_POT)
+        self.rc = float(rc if rc is not None else max([p.rcut for p in self.pots] + [0.0]))
+        if rmax is None:
+            rmax = np.ceil(self.rc/cell.rspacings - 0.5).astype(np.int64) if cell.nvec > 0 else np.zeros(0, int)
+        self.rmax = np.asarray(rmax, dtype=np.int64)
+        self.nlow = int(nlow)
+        self.nhigh = len(self.pos) if nhigh == -1 else int(nhigh)
+        self.stab = np.zeros(0, SCALING_DTYPE) if stab is None else np.asarray(stab, dtype=SCALING_DTYPE)
+        self.ewald = ewald
+        self.charges = None if charges is None else np.asarray(charges, dtype=np.float64)
+        self.masses = None if masses is None else np.asarray(masses, dtype=np.float64)
+
+    @property
+    def natom(self):
+        return len(self.pos)
+
+
+def ewald_kvectors(cell, alpha, gcut):
+    """The half-space k vectors compute_ewald_reci visits (ewald.c:41-58), with
+    the same FP64 expressions for k and |k|^2 so that the cutoff test picks the
+    same set.  Returned in row order (g2, g1 major, g0 ascending) with
+    c_k = 4 pi/V exp(-k^2/4 alpha^2)/k^2."""
+    gmax = np.ceil(gcut/cell.gspacings - 0.5).astype(np.int64)
+    kv = TWO_PI*cell.gvecs_full
+    g0, g1, g2 = np.meshgrid(np.arange(-gmax[0], gmax[0] + 1), np.arange(-gmax[1], gmax[1] + 1),
+                             np.arange(0, gmax[2] + 1), indexing='ij')
+    g = np.stack([g0.ravel(), g1.ravel(), g2.ravel()], axis=1)
+    order = np.lexsort((g[:, 0], g[:, 1], g[:, 2]))
+    g = g[order]
+    half = (g[:, 2] > 0) | ((g[:, 2] == 0) & ((g[:, 1] > 0) | ((g[:, 1] == 0) & (g[:, 0] > 0))))
+    g = g[half]
+    gf = g.astype(np.float64)
+    kx = gf[:, 0]*kv[0, 0] + gf[:, 1]*kv[1, 0] + gf[:, 2]*kv[2, 0]
+    ky = gf[:, 0]*kv[0, 1] + gf[:, 1]*kv[1, 1] + gf[:, 2]*kv[2, 1]
+    kz = gf[:, 0]*kv[0, 2] + gf[:, 1]*kv[1, 2] + gf[:, 2]*kv[2, 2]
+    ksq = kx*kx + ky*ky + kz*kz
+    gc = gcut*TWO_PI
+    gc *= gc
+    keep = ~(ksq > gc)
+    g, kx, ky, kz, ksq = g[keep], kx[keep], ky[keep], kz[keep], ksq[keep]
+    fac1 = FOUR_PI/cell.volume
+    fac2 = 0.25/alpha/alpha
+    c = fac1*np.exp(-ksq*fac2)/ksq
+    return gmax, g, np.stack([kx, ky, kz, c], axis=1)
+
+
+def _runs(g):
+    """Rows: maximal runs of consecutive g0 at fixed (g1, g2)."""
+    if len(g) == 0:
+        return np.zeros((0, 2), np.int64)
+    brk = np.ones(len(g), bool)
+    brk[1:] = (g[1:, 1] != g[:-1, 1]) | (g[1:, 2] != g[:-1, 2]) | (g[1:, 0] != g[:-1, 0] + 1)
+    starts = np.nonzero(brk)[0]
+    ends = np.append(starts[1:], len(g))
+    return np.stack([starts, ends - starts], axis=1)
+
+
+def _round_up(x, m):
+    return ((x + m - 1)//m)*m
+
+
+class Engine(object):
+    """Device state and launches for a batch of NBSystem objects."""
+
+    def __init__(self, systems, precision='fp64', seglen=None, kchunks=None, kcap=None, device='cuda'):
+        self.lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError('yaff_amd.Engine needs a HIP device (torch.cuda.is_available() is False)')
+        self.device = torch.device(device)
+        self.systems = list(systems)
+        self.precision = {'fp64': FP64, 'mixed': MIXED}[precision] if isinstance(precision, str) else int(precision)
+        self.nsys = len(self.systems)
+        self._layout()
+        self._plan_sys(seglen, kchunks)
+        self._alloc(kcap)
+        self.upload_positions()
+        self.nbuilds = 0
+
+    # ------------------------------------------------------------------
+    def _layout(self):
+        self.atom0 = []
+        off = 0
+        for s in self.systems:
+            self.atom0.append(off)
+            off += _round_up(max(s.natom, 1), 64)
+        self.npad = max(off, 64)
+        self.wave_sys = np.full(self.npad//64, -1, np.int32)
+        for i, s in enumerate(self.systems):
+            a0 = self.atom0[i]
+            self.wave_sys[a0//64:(a0 + max(s.natom, 1) + 63)//64] = i
+
+    def _plan_sys(self, seglen, kchunks):
+        L = self.lib
+        self.hsys = (YnbSys*self.nsys)()
+        par = np.zeros((MAX_POT, 3, self.npad), np.float64)
+        cross = []
+        stabs = []
+        nbin = 0
+        stab_off = 0
+        self.has_ewald = any(s.ewald is not None for s in self.systems)
+        self.groups = 0
+        self.seglen = seglen
+        for i, s in enumerate(self.systems):
+            h = self.hsys[i]
+            cell = s.cell
+            for k in range(9):
+                h.rvecs[k] = float(cell.rvecs_full.ravel()[k])
+                h.gvecs[k] = float(cell.gvecs_full.ravel()[k])
+            h.rc = s.rc
+            h.volume = cell.volume
+            h.nvec = cell.nvec
+            h.natom = s.natom
+            h.atom0 = self.atom0[i]
+            h.npot = len(s.pots)
+            for k in range(3):
+                h.rmax[k] = int(s.rmax[k]) if k < cell.nvec else 0
+            h.nlow, h.nhigh = s.nlow, s.nhigh
+            check(L.ynb_sys_plan(h))
+            h.bin0 = nbin
+            nbin += h.nbin
+            h.stab0, h.nstab = stab_off, len(s.stab)
+            stab_off += len(s.stab)
+            stabs.append(s.stab)
+            a0 = self.atom0[i]
+            for p, pot in enumerate(s.pots):
+                hp = h.pot[p]
+                hp.kind, hp.trunc, hp.rcut, hp.trunc_par = pot.kind, pot.trunc, pot.rcut, pot.trunc_par
+                hp.alpha, hp.dielectric = pot.alpha, pot.dielectric
+                for q, arr in enumerate((pot.p0, pot.p1, pot.p2)):
+                    if arr is not None:
+                        par[p, q, a0:a0 + s.natom] = np.asarray(arr, dtype=np.float64)
+                if pot.amp_cross is not None:
+                    nff = pot.amp_cross.shape[0]
+                    hp.nffatype = nff
+                    hp.cross0 = sum(len(c) for c in cross)
+                    cross.append(np.concatenate([np.ascontiguousarray(pot.amp_cross, dtype=float).ravel(),
+                                                 np.ascontiguousarray(pot.b_cross, dtype=float).ravel()]))
+        self.nbin = nbin
+        self.bin_sys = np.zeros(max(nbin, 1), np.int32)
+        for i in range(self.nsys):
+            self.bin_sys[self.hsys[i].bin0:self.hsys[i].bin0 + self.hsys[i].nbin] = i
+        self.par_h = par
+        self.cross_h = np.concatenate(cross) if cross else np.zeros(1)
+        self.stab_h = np.concatenate(stabs) if stab_off > 0 else np.zeros(1, SCALING_DTYPE)
+        if self.has_ewald:
+            self._plan_ewald(seglen, kchunks)
+
+    def _plan_ewald(self, seglen, kchunks):
+        """k rows, segments, atom chunks and the work list of k_ewald_sk."""
+        nk_all = 0
+        for s in self.systems:
+            if s.ewald is not None:
+                _, g, _ = ewald_kvectors(s.cell, s.ewald.alpha, s.ewald.gcut)
+                nk_all = max(nk_all, len(g))
+        if seglen is None:
+            seglen = 16 if nk_all > 20000 else 8
+        self.seglen = seglen
+        kvec, krow, krow_k, seg, seg_len, chunk, work, kblk_sys = [], [], [], [], [], [], [], []
+        k0 = row0 = seg0 = chunk0 = 0
+        spart0 = 0
+        self.gmax = []
+        for i, s in enumerate(self.systems):
+            h = self.hsys[i]
+            if s.ewald is None or s.natom == 0:
+                h.nk = 0
+                h.k0 = k0
+                self.gmax.append(None)
+                continue
+            ew = s.ewald
+            if s.cell.nvec != 3:
+                raise ValueError('Ewald reciprocal needs a 3D periodic cell')
+            gmax, g, kv = ewald_kvectors(s.cell, ew.alpha, ew.gcut)
+            self.gmax.append(gmax)
+            nk = len(g)
+            nkp = max(_round_up(nk, KBLOCK), KBLOCK)
+            kpad = np.zeros((nkp, 4))
+            kpad[:nk] = kv
+            kvec.append(kpad)
+            kblk_sys.extend([i]*(nkp//KBLOCK))
+            rows = _runs(g)
+            for (st, cnt) in rows:
+                krow.append((g[st, 1], g[st, 2], g[st, 0], cnt))
+                krow_k.append(k0 + st)
+            segs = []
+            for (st, cnt) in rows:
+                for off in range(0, cnt, seglen):
+                    ln = min(seglen, cnt - off)
+                    segs.append((g[st + off, 0], g[st, 1], g[st, 2], k0 + st + off, ln))
+            for x in segs:
+                seg.append(x[:4])
+                seg_len.append(x[4])
+            nhigh = s.natom if ew.nhigh == -1 else ew.nhigh
+            nlow = ew.nlow
+            if nlow > 0 or nhigh < s.natom:
+                self.groups = 1
+            chunks = []
+            for (lo, hi, grp) in ((0, nlow, 1), (nlow, nhigh, 0), (nhigh, s.natom, 2)):
+                for a in range(lo, hi, CHUNK):
+                    chunks.append((i, a, min(CHUNK, hi - a), grp))
+            chunk.extend(chunks)
+            nseg = len(segs)
+            for sb in range(0, nseg, 64):
+                for c in range(len(chunks)):
+                    work.append((i, seg0 + sb, chunk0 + c, min(64, nseg - sb)))
+            h.alpha, h.dielectric = ew.alpha, ew.dielectric
+            h.nlow, h.nhigh = (h.nlow, h.nhigh)
+            h.k0, h.nk = k0, nkp
+            h.row0, h.nrow = row0, len(rows)
+            h.seg0, h.nseg = seg0, nseg
+            h.chunk0, h.nchunk = chunk0, len(chunks)
+            h.spart0 = spart0
+            spart0 += len(chunks)*nkp
+            k0 += nkp
+            row0 += len(rows)
+            seg0 += nseg
+            chunk0 += len(chunks)
+        self.ewald_h = dict(
+            kvec=np.concatenate(kvec) if kvec else np.zeros((KBLOCK, 4)),
+            krow=np.array(krow, np.int32).reshape(-1, 4) if krow else np.zeros((1, 4), np.int32),
+            krow_k=np.array(krow_k, np.int32) if krow_k else np.zeros(1, np.int32),
+            seg=np.array(seg, np.int32).reshape(-1, 4) if seg else np.zeros((1, 4), np.int32),
+            seg_len=np.array(seg_len, np.int32) if seg_len else np.zeros(1, np.int32),
+            chunk=np.array(chunk, np.int32).reshape(-1, 4) if chunk else np.zeros((1, 4), np.int32),
+            work=np.array(work, np.int32).reshape(-1, 4) if work else np.zeros((1, 4), np.int32),
+            kblk_sys=np.array(kblk_sys, np.int32) if kblk_sys else np.zeros(1, np.int32),
+        )
+        self.nk_tot = k0
+        self.nwork = len(work)
+        self.spart_entries = max(spart0, 1)
+        if kchunks is None:
+            nwaves = self.npad//64
+            kchunks = max(1, min(64, int(math.ceil(4096.0/max(nwaves, 1)))))
+            maxrow = max([self.hsys[i].nrow for i in range(self.nsys)] + [1])
+            kchunks = min(kchunks, maxrow)
+        self.kchunks = kchunks
+
+    # ------------------------------------------------------------------
+    def _t(self, arr, dtype=None):
+        a = np.ascontiguousarray(arr)
+        t = torch.from_numpy(a.copy())
+        return t.to(self.device)
+
+    def _alloc(self, kcap):
+        L = self.lib
+        dev = self.device
+        self.sys_d = self._t(np.frombuffer(bytes(self.hsys), np.uint8))
+        self.wave_sys_d = self._t(self.wave_sys)
+        self.bin_sys_d = self._t(self.bin_sys)
+        self.batch = YnbBatch(self.sys_d.data_ptr(), self.wave_sys_d.data_ptr(), self.bin_sys_d.data_ptr(),
+                              self.nsys, self.nbin, self.npad, 0, 0)
+        self.pos_d = torch.zeros((self.npad, 3), dtype=torch.float64, device=dev)
+        self.gpos_d = torch.zeros((self.npad, 3), dtype=torch.float64, device=dev)
+        self.par_d = self._t(self.par_h)
+        self.cross_d = self._t(self.cross_h)
+        self.stab_d = self._t(np.frombuffer(self.stab_h.tobytes(), np.uint8))
+        self.pairpar = YnbPairpar(self.par_d.data_ptr(), self.cross_d.data_ptr(), self.stab_d.data_ptr())
+        # neighbour list
+        self.nl_work_bytes = L.ynb_nlist_workspace_bytes(self.npad, self.nbin)
+        self.nl_work_d = torch.empty(self.nl_work_bytes, dtype=torch.uint8, device=dev)
+        self.cnt_d = torch.zeros(self.npad, dtype=torch.int32, device=dev)
+        self.status_d = torch.zeros(4, dtype=torch.int32, device=dev)
+        self._alloc_nlist(kcap if kcap is not None else self._estimate_kcap())
+        self.partial_d = torch.zeros((self.npad//64)*MAX_POT*8, dtype=torch.float64, device=dev)
+        self.pair_out_d = torch.zeros((self.nsys, MAX_POT, 10), dtype=torch.float64, device=dev)
+        self.ewald_out_d = torch.zeros((self.nsys, 10), dtype=torch.float64, device=dev)
+        self.total_d = torch.zeros((self.nsys, 10), dtype=torch.float64, device=dev)
+        self.gpos_slots_d = None
+        charges = np.zeros(self.npad)
+        for i, s in enumerate(self.systems):
+            if s.charges is not None:
+                charges[self.atom0[i]:self.atom0[i] + s.natom] = s.charges
+        self.charges_d = self._t(charges)
+        if self.has_ewald:
+            e = self.ewald_h
+            self.ew_tabs = {k: self._t(v) for k, v in e.items()}
+            self.ew_frac_d = torch.zeros(4*self.npad, dtype=torch.float64, device=dev)
+            self.ew_step_d = torch.zeros(2*self.npad, dtype=torch.float64, device=dev)
+            self.ew_spart_d = torch.zeros(2*self.spart_entries, dtype=torch.float64, device=dev)
+            self.ew_coef_d = torch.zeros(6*max(self.nk_tot, KBLOCK), dtype=torch.float64, device=dev)
+            self.ew_kpart_d = torch.zeros(8*max(self.nk_tot//KBLOCK, 1), dtype=torch.float64, device=dev)
+            self.ew_gpart_d = torch.zeros(self.kchunks*self.npad*3, dtype=torch.float64, device=dev)
+            t = self.ew_tabs
+            self.ewald = YnbEwald(t['kvec'].data_ptr(), t['krow'].data_ptr(), t['krow_k'].data_ptr(),
+                                  t['seg'].data_ptr(), t['seg_len'].data_ptr(), t['chunk'].data_ptr(),
+                                  t['work'].data_ptr(), t['kblk_sys'].data_ptr(), self.nwork, self.nk_tot,
+                                  self.seglen, self.kchunks, self.groups, 0, self.ew_frac_d.data_ptr(),
+                                  self.ew_step_d.data_ptr(), self.ew_spart_d.data_ptr(), self.ew_coef_d.data_ptr(),
+                                  self.ew_kpart_d.data_ptr(), self.ew_gpart_d.data_ptr())
+        # MD state
+        masses = np.ones(self.npad)
+        for i, s in enumerate(self.systems):
+            if s.masses is not None:
+                masses[self.atom0[i]:self.atom0[i] + s.natom] = s.masses
+        self.masses_d = self._t(masses)
+        self.minv_d = self._t(1.0/masses)
+        self.vel_d = torch.zeros((self.npad, 3), dtype=torch.float64, device=dev)
+        self.dt_d = torch.zeros(self.nsys, dtype=torch.float64, device=dev)
+        self.ekin2_d = torch.zeros(self.nsys, dtype=torch.float64, device=dev)
+        self.md_partial_d = torch.zeros(self.npad//64, dtype=torch.float64, device=dev)
+
+    def _estimate_kcap(self):
+        est = 16
+        for s in self.systems:
+            if s.natom == 0:
+                continue
+            if s.cell.nvec == 3:
+                rho = s.natom/s.cell.volume
+                n = 4.0/3.0*math.pi*s.rc**3*rho
+            else:
+                nimg = int(np.prod(2*s.rmax + 1)) if len(s.rmax) else 1
+                n = s.natom*nimg
+            est = max(est, int(1.3*n) + 32)
+        return est
+
+    def _alloc_nlist(self, kcap):
+        self.kcap = int(kcap)
+        self.nbr_j_d = torch.zeros(self.kcap*self.npad, dtype=torch.int32, device=self.device)
+        self.nbr_img_d = torch.zeros(self.kcap*self.npad, dtype=torch.int32, device=self.device)
+        self.nlist = YnbNlist(self.nbr_j_d.data_ptr(), self.nbr_img_d.data_ptr(), self.cnt_d.data_ptr(),
+                              self.status_d.data_ptr(), self.kcap, 0)
+
+    # ------------------------------------------------------------------
+    @staticmethod
+    def stream():
+        return C_stream(torch.cuda.current_stream().cuda_stream)
+
+    def upload_positions(self, pos_list=None):
+        """Copies host positions (list per system, default: the systems' own)."""
+        host = np.zeros((self.npad, 3))
+        for i, s in enumerate(self.systems):
+            p = s.pos if pos_list is None else pos_list[i]
+            host[self.atom0[i]:self.atom0[i] + s.natom] = p
+        self.pos_d.copy_(torch.from_numpy(host), non_blocking=False)
+
+    def set_system_positions(self, i, pos):
+        a0 = self.atom0[i]
+        self.pos_d[a0:a0 + len(pos)].copy_(torch.from_numpy(np.ascontiguousarray(pos, dtype=np.float64)))
+
+    def build_nlist(self):
+        check(self.lib.ynb_nlist_build(self.batch, self.pos_d.data_ptr(), self.nl_work_d.data_ptr(),
+                                       self.nl_work_bytes, self.nlist, self.stream()))
+        self.nbuilds += 1
+
+    def nlist_status(self):
+        """(flags, max entries per atom) of the last build; synchronises."""
+        st = self.status_d.cpu().numpy()
+        return int(st[0]), int(st[1])
+
+    def ensure_nlist(self):
+        """Builds the list, growing the ELL capacity until every atom fits."""
+        self.build_nlist()
+        flags, mx = self.nlist_status()
+        if flags & 2:
+            raise RuntimeError('neighbour list: an image translation exceeds 511 cells')
+        if flags & 1:
+            self._alloc_nlist(int(mx*1.1) + 8)
+            self.build_nlist()
+            flags, mx = self.nlist_status()
+            if flags:
+                raise RuntimeError('neighbour list: capacity still exceeded')
+
+    def pair(self, mask=0xF, gpos=True, accumulate=False, slots=False):
+        if slots and self.gpos_slots_d is None:
+            self.gpos_slots_d = torch.zeros((MAX_POT, self.npad, 3), dtype=torch.float64, device=self.device)
+        check(self.lib.ynb_pair_compute(self.batch, self.pairpar, self.pos_d.data_ptr(), self.nlist, mask,
+                                        self.precision, self.gpos_d.data_ptr() if gpos else None,
+                                        1 if accumulate else 0,
+                                        self.gpos_slots_d.data_ptr() if slots else None,
+                                        self.partial_d.data_ptr(), self.pair_out_d.data_ptr(), self.stream()))
+
+    def ewald_reci(self, gpos=True, accumulate=True):
+        if not self.has_ewald:
+            self.ewald_out_d.zero_()
+            return
+        check(self.lib.ynb_ewald_reci(self.batch, self.ewald, self.pos_d.data_ptr(), self.charges_d.data_ptr(),
+                                      self.precision, self.gpos_d.data_ptr() if gpos else None,
+                                      1 if accumulate else 0, self.ewald_out_d.data_ptr(), self.stream()))
+
+    def total(self):
+        check(self.lib.ynb_total(self.batch, self.pair_out_d.data_ptr(),
+                                 self.ewald_out_d.data_ptr() if self.has_ewald else None,
+                                 self.total_d.data_ptr(), self.stream()))
+
+    def compute(self, build=True, gpos=True):
+        """Neighbour list, pair potentials and Ewald reciprocal for every
+        system; energy+virial totals in total_d, gradient in gpos_d.
+        Asynchronous (call nlist_status() to validate the list)."""
+        if build:
+            self.build_nlist()
+        self.pair(gpos=gpos, accumulate=False)
+        self.ewald_reci(gpos=gpos, accumulate=True)
+        self.total()
+
+    # ---- MD (velocity Verlet, yaff/sampling/verlet.py:339-367) ----------
+    def set_velocities(self, vel_list):
+        host = np.zeros((self.npad, 3))
+        for i, s in enumerate(self.systems):
+            host[self.atom0[i]:self.atom0[i] + s.natom] = vel_list[i]
+        self.vel_d.copy_(torch.from_numpy(host))
+
+    def set_timestep(self, dt):
+        dt = np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.nsys,))
+        self.dt_d.copy_(torch.from_numpy(np.ascontiguousarray(dt)))
+
+    def md_step(self):
+        """kick, drift, neighbour list, forces, kick: one Verlet step of every
+        system, on device, without synchronisation."""
+        st = self.stream()
+        check(self.lib.ynb_verlet_kick_drift(self.batch, self.pos_d.data_ptr(), self.vel_d.data_ptr(),
+                                             self.gpos_d.data_ptr(), self.minv_d.data_ptr(), self.dt_d.data_ptr(),
+                                             st))
+        self.compute(build=True)
+        check(self.lib.ynb_verlet_kick(self.batch, self.vel_d.data_ptr(), self.gpos_d.data_ptr(),
+                                       self.minv_d.data_ptr(), self.masses_d.data_ptr(), self.dt_d.data_ptr(),
+                                       self.md_partial_d.data_ptr(), self.ekin2_d.data_ptr(), st))
+
+    # ---- host views ---------------------------------------------------
+    def results(self, i=None):
+        """(energy, gpos, vtens) of system i (or lists for all); synchronises."""
+        tot = self.total_d.cpu().numpy()
+        g = self.gpos_d.cpu().numpy()
+        out = []
+        for s in range(self.nsys) if i is None else [i]:
+            a0, n = self.atom0[s], self.systems[s].natom
+            out.append((tot[s, 0], g[a0:a0 + n].copy(), tot[s, 1:].reshape(3, 3).copy()))
+        return out if i is None else out[0]
+
+    def rows(self, i=0):
+        """Yaff rows (neigh_dtype) of system i in nlist_build_low's order."""
+        L = self.lib
+        natom = self.systems[i].natom
+        count = _lib.C.c_int64(0)
+        wbytes = L.ynb_rows_workspace_bytes(natom, 0)
+        work = torch.empty(max(wbytes, 1), dtype=torch.uint8, device=self.device)
+        check(L.ynb_nlist_rows(self.batch, i, self.pos_d.data_ptr(), self.nlist, work.data_ptr(), wbytes, None, 0,
+                               _lib.C.byref(count), self.stream()))
+        n = count.value
+        wbytes = L.ynb_rows_workspace_bytes(natom, n)
+        work = torch.empty(max(wbytes, 1), dtype=torch.uint8, device=self.device)
+        rows = torch.empty(max(n, 1)*ROW_DTYPE.itemsize, dtype=torch.uint8, device=self.device)
+        check(L.ynb_nlist_rows(self.batch, i, self.pos_d.data_ptr(), self.nlist, work.data_ptr(), wbytes,
+                               rows.data_ptr(), n, _lib.C.byref(count), self.stream()))
+        return np.frombuffer(rows.cpu().numpy().tobytes()[:n*ROW_DTYPE.itemsize], ROW_DTYPE).copy()
+
+
+def C_stream(handle):
+    return _lib.C.c_void_p(handle)

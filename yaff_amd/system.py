@@ -1,0 +1,112 @@
+"""Minimal ``System`` with the attributes the nonbonded path reads.
+
+Mirrors the constructor of yaff.System (yaff/system.py:50-170) for the fields
+used by NeighborList, the pair potentials and the Ewald parts, the bond-derived
+neighbour sets (yaff/system.py:199-234) and ``supercell`` (yaff/system.py:841-935).
+"""
+import numpy as np
+
+from .cell import Cell
+
+__all__ = ['System']
+
+
+class System(object):
+    def __init__(self, numbers, pos, ffatypes=None, ffatype_ids=None, bonds=None, rvecs=None,
+                 charges=None, radii=None, masses=None, **kwargs):
+        self.numbers = np.asarray(numbers)
+        self.pos = np.ascontiguousarray(pos, dtype=float).copy()
+        if self.pos.ndim != 2 or self.pos.shape[1] != 3 or len(self.pos) != len(self.numbers):
+            raise TypeError('pos must have shape (natom, 3).')
+        self.ffatypes = None if ffatypes is None else np.asarray(ffatypes)
+        if ffatypes is not None and ffatype_ids is None:
+            # Yaff assigns ids in order of first appearance when ffatypes are per atom.
+            uniq = []
+            ids = []
+            for t in ffatypes:
+                if t not in uniq:
+                    uniq.append(t)
+                ids.append(uniq.index(t))
+            self.ffatypes = np.array(uniq)
+            ffatype_ids = np.array(ids)
+        self.ffatype_ids = None if ffatype_ids is None else np.asarray(ffatype_ids, dtype=np.int64)
+        self.bonds = None if bonds is None else np.asarray(bonds, dtype=np.int64).reshape(-1, 2)
+        self.cell = Cell(None if rvecs is None else np.ascontiguousarray(rvecs, dtype=float))
+        self.charges = None if charges is None else np.asarray(charges, dtype=float).copy()
+        self.radii = None if radii is None else np.asarray(radii, dtype=float).copy()
+        self.masses = None if masses is None else np.asarray(masses, dtype=float).copy()
+        self._init_derived_bonds()
+
+    @property
+    def natom(self):
+        return len(self.pos)
+
+    def _init_derived_bonds(self):
+        """yaff/system.py:199-234"""
+        n = self.natom
+        n1 = {i: set() for i in range(n)}
+        for i0, i1 in (self.bonds if self.bonds is not None else []):
+            n1[int(i0)].add(int(i1))
+            n1[int(i1)].add(int(i0))
+        n2 = {i: set() for i in range(n)}
+        for i0, s0 in n1.items():
+            for i1 in s0:
+                for i2 in n1[i1]:
+                    if i2 > i0 and i2 not in n1[i0]:
+                        n2[i0].add(i2)
+                        n2[i2].add(i0)
+        n3 = {i: set() for i in range(n)}
+        for i0, s0 in n1.items():
+            for i1 in s0:
+                for i3 in n2[i1]:
+                    if i3 != i0 and i3 not in n1[i0] and i3 not in n2[i0]:
+                        n3[i0].add(i3)
+                        n3[i3].add(i0)
+        n4 = {i: set() for i in range(n)}
+        for i0, s0 in n1.items():
+            for i1 in s0:
+                for i4 in n3[i1]:
+                    if i4 != i0 and i4 not in n1[i0] and i4 not in n2[i0] and i4 not in n3[i0]:
+                        n4[i0].add(i4)
+                        n4[i4].add(i0)
+        self.neighs1, self.neighs2, self.neighs3, self.neighs4 = n1, n2, n3, n4
+
+    def supercell(self, *reps):
+        """yaff/system.py:841-935"""
+        if self.cell.nvec == 0:
+            raise TypeError('Can not create a supercell of a non-periodic system.')
+        if self.cell.nvec != len(reps):
+            raise TypeError('The number of repetitions must match the number of cell vectors.')
+        nrep = int(np.prod(reps))
+        kw = {}
+        for name in ('numbers', 'ffatype_ids', 'charges', 'radii', 'masses'):
+            v = getattr(self, name)
+            if v is not None:
+                kw[name] = np.tile(v, nrep)
+        if self.ffatypes is not None:
+            kw['ffatypes_unique'] = self.ffatypes.copy()
+        rvecs = self.cell.rvecs*np.array(reps)[:, None]
+        pos = np.zeros((self.natom*nrep, 3))
+        offsets = {}
+        start = 0
+        for img in np.ndindex(reps):
+            pos[start:start + self.natom] = self.pos + np.dot(img, self.cell.rvecs)
+            offsets[img] = start
+            start += self.natom
+        bonds = None
+        if self.bonds is not None:
+            rel = []
+            for i0, i1 in self.bonds:
+                frac = np.dot(self.cell.gvecs, self.pos[i0] - self.pos[i1])
+                rel.append(np.ceil(frac - 0.5))
+            bonds = np.zeros((len(self.bonds)*nrep, 2), int)
+            k = 0
+            for img0 in np.ndindex(reps):
+                for ib, (i0, i1) in enumerate(self.bonds):
+                    img1 = tuple(int((img0[c] + rel[ib][c]) % reps[c]) for c in range(len(reps)))
+                    bonds[k] = offsets[img0] + i0, offsets[img1] + i1
+                    k += 1
+        ffu = kw.pop('ffatypes_unique', None)
+        s = System(kw.pop('numbers'), pos, bonds=bonds, rvecs=rvecs, **kw)
+        s.ffatypes = ffu
+        return s
