@@ -16,7 +16,7 @@
  *   ynb_nlist_build + ynb_nlist_rows   <- nlist_build_low   yaff/pes/nlist.h:37-39
  *                                         (+ nlist_status_* yaff/pes/ext.pyx:359-449)
  *   ynb_nlist_recompute                <- nlist_recompute_low yaff/pes/nlist.h:41-42
- *   ynb_pair_compute                   <- pair_pot_compute  yaff/pes/pair_pot.h:61-64
+ *   ynb_pair_compute, ynb_pair_rows    <- pair_pot_compute  yaff/pes/pair_pot.h:61-64
  *                                         with pair_fn_lj/mm3/exprep/ei and
  *                                         switch3/hammer (yaff/pes/truncation.h)
  *   ynb_ewald_reci                     <- compute_ewald_reci yaff/pes/ewald.h:31-34
@@ -68,6 +68,8 @@ typedef struct ynb_pot {
   double dielectric;  /* EI */
   int32_t nffatype;   /* ExpRep: number of atom types */
   int32_t cross0;     /* ExpRep: offset of the amp table in ynb_pairpar.cross; b follows */
+  int32_t stab0;      /* scaling rows of this potential: [stab0, stab0+nstab) of ynb_pairpar.stab */
+  int32_t nstab;
 } ynb_pot;
 
 /* One system of a batch.  Filled on the host (ynb_sys_plan completes the
@@ -87,7 +89,8 @@ typedef struct ynb_sys {
   int32_t range[3];   /* bins searched on each side (ynb_sys_plan) */
   int32_t bin0;       /* first global bin of this system */
   int32_t nbin;
-  int32_t stab0, nstab; /* scaling rows of this system in ynb_pairpar.stab */
+  int32_t any_stab;   /* nonzero when a potential of the system has scaling rows */
+  int32_t pad0_;
   ynb_pot pot[YNB_MAX_POT];
   /* Ewald reciprocal part (nk == 0: none) */
   double alpha, dielectric;
@@ -158,6 +161,14 @@ typedef struct ynb_ewald {
 const char *ynb_version(void);
 const char *ynb_last_error(void);
 
+/* Kernel timing: while enabled, each kernel launched by the library is
+ * bracketed by two HIP events (created once, on the first enable) recorded on
+ * its own stream.  ynb_profile_read synchronises on launch i's end event and
+ * returns its kernel name and duration.  Enabling resets the record. */
+int ynb_profile_enable(int on);
+int ynb_profile_count(void);
+int ynb_profile_read(int i, const char **name, float *ms);
+
 /* Host only.  Completes sys->grid, sys->range and sys->nbin from the cell,
  * sys->rc and sys->natom.  Systems with nvec < 3 use the all-pairs path
  * (nbin = 0). */
@@ -196,6 +207,14 @@ int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const double *po
                      const ynb_nlist *nl, uint32_t slot_mask, int32_t precision,
                      double *gpos, int32_t accumulate, double *gpos_slots,
                      double *partial, double *out, ynb_stream stream);
+
+/* pair_pot_compute over explicit Yaff rows (local atom indices) for one
+ * potential: gpos (nullable) [natom][3] is added to with atomics; out[10] =
+ * energy and virial; partial: ceil(nrow/256)*8 doubles.  Parameter arrays are
+ * per atom (p1/p2/cross as in ynb_pairpar, nullable when unused). */
+int ynb_pair_rows(const ynb_neigh_row *rows, int64_t nrow, const ynb_scaling_row *stab, int64_t nstab,
+                  const ynb_pot *pot, const double *p0, const double *p1, const double *p2,
+                  const double *cross, double *gpos, double *partial, double *out, ynb_stream stream);
 
 /* Ewald reciprocal term of every system with nk > 0.
  * out: [nsys][10] energy followed by the virial; gpos as in ynb_pair_compute. */

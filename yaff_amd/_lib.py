@@ -25,7 +25,7 @@ SCALING_DTYPE = np.dtype([('a', np.int64), ('b', np.int64), ('scale', np.float64
 class YnbPot(C.Structure):
     _fields_ = [('kind', C.c_int32), ('trunc', C.c_int32), ('rcut', C.c_double), ('trunc_par', C.c_double),
                 ('alpha', C.c_double), ('dielectric', C.c_double), ('nffatype', C.c_int32),
-                ('cross0', C.c_int32)]
+                ('cross0', C.c_int32), ('stab0', C.c_int32), ('nstab', C.c_int32)]
 
 
 class YnbSys(C.Structure):
@@ -33,7 +33,7 @@ class YnbSys(C.Structure):
                 ('volume', C.c_double), ('nvec', C.c_int32), ('natom', C.c_int32), ('atom0', C.c_int32),
                 ('npot', C.c_int32), ('rmax', C.c_int32 * 3), ('nlow', C.c_int32), ('nhigh', C.c_int32),
                 ('grid', C.c_int32 * 3), ('range', C.c_int32 * 3), ('bin0', C.c_int32), ('nbin', C.c_int32),
-                ('stab0', C.c_int32), ('nstab', C.c_int32), ('pot', YnbPot * MAX_POT),
+                ('any_stab', C.c_int32), ('pad0_', C.c_int32), ('pot', YnbPot * MAX_POT),
                 ('alpha', C.c_double), ('dielectric', C.c_double), ('k0', C.c_int32), ('nk', C.c_int32),
                 ('row0', C.c_int32), ('nrow', C.c_int32), ('seg0', C.c_int32), ('nseg', C.c_int32),
                 ('chunk0', C.c_int32), ('nchunk', C.c_int32), ('spart0', C.c_int64)]
@@ -63,8 +63,8 @@ class YnbEwald(C.Structure):
 
 EXPORTS = ['ynb_version', 'ynb_last_error', 'ynb_sys_plan', 'ynb_nlist_workspace_bytes',
            'ynb_rows_workspace_bytes', 'ynb_nlist_build', 'ynb_nlist_rows', 'ynb_nlist_recompute',
-           'ynb_pair_compute', 'ynb_ewald_reci', 'ynb_ewald_corr', 'ynb_verlet_kick_drift', 'ynb_verlet_kick',
-           'ynb_total']
+           'ynb_pair_compute', 'ynb_pair_rows', 'ynb_ewald_reci', 'ynb_ewald_corr', 'ynb_verlet_kick_drift', 'ynb_verlet_kick',
+           'ynb_total', 'ynb_profile_enable', 'ynb_profile_count', 'ynb_profile_read']
 
 
 class YnbError(RuntimeError):
@@ -95,11 +95,14 @@ def load(path=LIB_PATH):
     L.ynb_nlist_recompute.argtypes = [P(YnbSys), vp, vp, vp, i64, vp]
     L.ynb_pair_compute.argtypes = [P(YnbBatch), P(YnbPairpar), vp, P(YnbNlist), C.c_uint32, i32, vp, i32, vp,
                                    vp, vp, vp]
+    L.ynb_pair_rows.argtypes = [vp, i64, vp, i64, P(YnbPot), vp, vp, vp, vp, vp, vp, vp, vp]
     L.ynb_ewald_reci.argtypes = [P(YnbBatch), P(YnbEwald), vp, vp, i32, vp, i32, vp, vp]
     L.ynb_ewald_corr.argtypes = [P(YnbSys), vp, vp, vp, vp, i64, vp, i32, vp, vp]
     L.ynb_verlet_kick_drift.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp, vp]
     L.ynb_verlet_kick.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp, vp, vp, vp]
     L.ynb_total.argtypes = [P(YnbBatch), vp, vp, vp, vp]
+    L.ynb_profile_enable.argtypes = [C.c_int]
+    L.ynb_profile_read.argtypes = [C.c_int, P(C.c_char_p), P(C.c_float)]
     for name in EXPORTS:
         if name not in ('ynb_version', 'ynb_last_error', 'ynb_nlist_workspace_bytes', 'ynb_rows_workspace_bytes'):
             getattr(L, name).restype = C.c_int
@@ -112,3 +115,19 @@ def check(rc):
     if rc != 0:
         raise YnbError('libyaff_nb: %s (code %d)' % (load().ynb_last_error().decode(), rc))
     return rc
+
+
+def profile_enable(on=True):
+    check(load().ynb_profile_enable(1 if on else 0))
+
+
+def profile_read():
+    """[(kernel name, milliseconds)] of the launches since profile_enable(True)."""
+    L = load()
+    out = []
+    name = C.c_char_p()
+    ms = C.c_float()
+    for i in range(L.ynb_profile_count()):
+        check(L.ynb_profile_read(i, C.byref(name), C.byref(ms)))
+        out.append((name.value.decode(), ms.value))
+    return out

@@ -37,6 +37,19 @@ void set_error(const char *fmt, ...);
     }                                                                          \
   } while (0)
 
+// Kernel timing with HIP events (ynb_profile_*): when enabled, every launch
+// made through YNB_LAUNCH is bracketed by two pre-created events recorded on
+// the launch's own stream.
+void prof_before(const char *name, hipStream_t st);
+void prof_after(hipStream_t st);
+
+#define YNB_LAUNCH(name, kernel, grid, block, shmem, stream, ...)               \
+  do {                                                                         \
+    ::ynb::prof_before(name, stream);                                          \
+    hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);       \
+    ::ynb::prof_after(stream);                                                 \
+  } while (0)
+
 // Image translations are packed 10 bits per axis with an offset of 512.
 __host__ __device__ inline int32_t pack_img(int n0, int n1, int n2) {
   return (n0 + 512) | ((n1 + 512) << 10) | ((n2 + 512) << 20);

@@ -351,7 +351,7 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
   YNB_LAUNCHED();
   if (ew->nwork > 0) {
     const dim3 grid((unsigned)(ew->nwork < 65535 ? ew->nwork : 65535));
-#define YNB_SK(P, LL) hipLaunchKernelGGL((k_ewald_sk<P, LL>), grid, dim3(64), 0, st, b->sys, *ew, npad)
+#define YNB_SK(P, LL) YNB_LAUNCH("ewald_sk", (k_ewald_sk<P, LL>), grid, dim3(64), 0, st, b->sys, *ew, npad)
     if (precision == YNB_FP64) {
       if (ew->seglen == 4) YNB_SK(YNB_FP64, 4);
       else if (ew->seglen == 8) YNB_SK(YNB_FP64, 8);
@@ -365,20 +365,20 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
     YNB_LAUNCHED();
   }
   if (ew->nk_tot > 0) {
-    hipLaunchKernelGGL(k_ewald_kreduce, dim3(ew->nk_tot / 256), dim3(256), 0, st, b->sys, *ew);
+    YNB_LAUNCH("ewald_kreduce", k_ewald_kreduce, dim3(ew->nk_tot / 256), dim3(256), 0, st, b->sys, *ew);
     YNB_LAUNCHED();
   }
   const dim3 fgrid((unsigned)(npad / 64), (unsigned)ew->kchunks);
   if (precision == YNB_FP64) {
     if (ew->groups)
-      hipLaunchKernelGGL((k_ewald_force<YNB_FP64, true>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
+      YNB_LAUNCH("ewald_force", (k_ewald_force<YNB_FP64, true>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
     else
-      hipLaunchKernelGGL((k_ewald_force<YNB_FP64, false>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
+      YNB_LAUNCH("ewald_force", (k_ewald_force<YNB_FP64, false>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
   } else {
     if (ew->groups)
-      hipLaunchKernelGGL((k_ewald_force<YNB_MIXED, true>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
+      YNB_LAUNCH("ewald_force", (k_ewald_force<YNB_MIXED, true>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
     else
-      hipLaunchKernelGGL((k_ewald_force<YNB_MIXED, false>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
+      YNB_LAUNCH("ewald_force", (k_ewald_force<YNB_MIXED, false>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
   }
   YNB_LAUNCHED();
   if (gpos) {

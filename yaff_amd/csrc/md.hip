@@ -18,6 +18,31 @@ void set_error(const char *fmt, ...) {
 
 namespace {
 
+constexpr int PROF_MAX = 8192;
+struct Prof {
+  bool on = false;
+  int n = 0;
+  const char *name[PROF_MAX];
+  hipEvent_t ev[2 * PROF_MAX];
+  bool created = false;
+} g_prof;
+
+}  // namespace
+
+void prof_before(const char *name, hipStream_t st) {
+  if (!g_prof.on || g_prof.n >= PROF_MAX) return;
+  g_prof.name[g_prof.n] = name;
+  hipEventRecord(g_prof.ev[2 * g_prof.n], st);
+}
+
+void prof_after(hipStream_t st) {
+  if (!g_prof.on || g_prof.n >= PROF_MAX) return;
+  hipEventRecord(g_prof.ev[2 * g_prof.n + 1], st);
+  ++g_prof.n;
+}
+
+namespace {
+
 // acc = -gpos/m; vel += dt/2 acc; pos += dt vel  (verlet.py:344-347)
 __global__ void k_kick_drift(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, double *pos, double *vel,
                              const double *gpos, const double *minv, const double *dt) {
@@ -94,14 +119,37 @@ const char *ynb_version(void) { return "yaff_nb 0.1 gfx950"; }
 
 const char *ynb_last_error(void) { return g_err; }
 
+int ynb_profile_enable(int on) {
+  if (on && !g_prof.created) {
+    for (int i = 0; i < 2 * PROF_MAX; ++i) YNB_HIP(hipEventCreate(&g_prof.ev[i]));
+    g_prof.created = true;
+  }
+  g_prof.on = on != 0;
+  g_prof.n = 0;
+  return YNB_OK;
+}
+
+int ynb_profile_count(void) { return g_prof.n; }
+
+int ynb_profile_read(int i, const char **name, float *ms) {
+  if (i < 0 || i >= g_prof.n) {
+    set_error("ynb_profile_read: index %d out of range", i);
+    return YNB_EINVAL;
+  }
+  YNB_HIP(hipEventSynchronize(g_prof.ev[2 * i + 1]));
+  YNB_HIP(hipEventElapsedTime(ms, g_prof.ev[2 * i], g_prof.ev[2 * i + 1]));
+  *name = g_prof.name[i];
+  return YNB_OK;
+}
+
 int ynb_verlet_kick_drift(const ynb_batch *b, double *pos, double *vel, const double *gpos, const double *minv,
                           const double *dt, ynb_stream stream) {
   if (!b || !pos || !vel || !gpos || !minv || !dt) {
     set_error("ynb_verlet_kick_drift: invalid arguments");
     return YNB_EINVAL;
   }
-  hipLaunchKernelGGL(k_kick_drift, dim3((unsigned)ceil_div(b->npad, 256)), dim3(256), 0, (hipStream_t)stream, b->sys,
-                     b->wave_sys, b->npad, pos, vel, gpos, minv, dt);
+  YNB_LAUNCH("verlet_kick_drift", k_kick_drift, dim3((unsigned)ceil_div(b->npad, 256)), dim3(256), 0,
+             (hipStream_t)stream, b->sys, b->wave_sys, b->npad, pos, vel, gpos, minv, dt);
   YNB_LAUNCHED();
   return YNB_OK;
 }
@@ -113,8 +161,8 @@ int ynb_verlet_kick(const ynb_batch *b, double *vel, const double *gpos, const d
     return YNB_EINVAL;
   }
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_kick, dim3((unsigned)ceil_div(b->npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys, b->npad,
-                     vel, gpos, minv, masses, dt, partial);
+  YNB_LAUNCH("verlet_kick", k_kick, dim3((unsigned)ceil_div(b->npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys,
+             b->npad, vel, gpos, minv, masses, dt, partial);
   YNB_LAUNCHED();
   hipLaunchKernelGGL(k_wave_to_sys, dim3((unsigned)ceil_div(b->nsys, 64)), dim3(64), 0, st, b->sys, b->nsys, partial,
                      ekin2);

@@ -338,9 +338,21 @@ __global__ void k_rows_emit(const ynb_sys *sys, int s, int64_t npad, const doubl
       row.dx = v[0]; row.dy = v[1]; row.dz = v[2];
       rho[0] = r[0]; rho[1] = r[1]; rho[2] = r[2];
     } else {
-      row.a = lj; row.b = li;
-      row.dx = -v[0]; row.dy = -v[1]; row.dz = -v[2];
+      // Yaff's sign -1 row: (-delta0) + rho.R with rho = -r (nlist.c:120-123),
+      // evaluated as such so that signed zeros match as well.
       rho[0] = -r[0]; rho[1] = -r[1]; rho[2] = -r[2];
+      double d0[3] = {0.0, 0.0, 0.0};
+      int m[3];
+      if (!self) {
+        d0[0] = pj[0] - pi[0];
+        d0[1] = pj[1] - pi[1];
+        d0[2] = pj[2] - pi[2];
+        mic_exact(d0, S.rvecs, S.gvecs, nv, m);
+      }
+      double w[3] = {-d0[0], -d0[1], -d0[2]};
+      add_vec_exact(w, S.rvecs, nv, rho);
+      row.a = lj; row.b = li;
+      row.dx = w[0]; row.dy = w[1]; row.dz = w[2];
     }
     row.d = sqrt(d2);
     row.r0 = rho[0]; row.r1 = rho[1]; row.r2 = rho[2];
@@ -491,8 +503,8 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
     const int nbb = (int)ceil_div(b->nbin, tb);
     hipLaunchKernelGGL(k_fill_i32, dim3(nbb), dim3(tb), 0, st, w.bin_start, (int64_t)b->nbin, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3(nbb), dim3(tb), 0, st, w.bin_end, (int64_t)b->nbin, 0);
-    hipLaunchKernelGGL(k_bin, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, w.key, w.val, w.xw,
-                       w.wr);
+    YNB_LAUNCH("nlist_bin", k_bin, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, w.key, w.val, w.xw,
+               w.wr);
     YNB_LAUNCHED();
     size_t cb = w.cub_bytes;
     YNB_HIP(hipcub::DeviceRadixSort::SortPairs(w.cub, cb, w.key, w.key_sorted, w.val, w.val_sorted,
@@ -500,11 +512,12 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
     hipLaunchKernelGGL(k_binrange, dim3(nb), dim3(tb), 0, st, npad, w.key_sorted, w.val_sorted, w.xw, w.wr,
                        w.bin_start, w.bin_end, w.sxw, w.sw);
     YNB_LAUNCHED();
-    hipLaunchKernelGGL(k_build_binned, dim3(b->nbin), dim3(64), 0, st, b->sys, b->bin_sys, npad, pos,
-                       w.bin_start, w.bin_end, w.sxw, w.sw, *nl);
+    YNB_LAUNCH("nlist_build_binned", k_build_binned, dim3(b->nbin), dim3(64), 0, st, b->sys, b->bin_sys, npad, pos,
+               w.bin_start, w.bin_end, w.sxw, w.sw, *nl);
     YNB_LAUNCHED();
   }
-  hipLaunchKernelGGL(k_build_allpairs, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, *nl);
+  YNB_LAUNCH("nlist_build_allpairs", k_build_allpairs, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos,
+             *nl);
   YNB_LAUNCHED();
   return YNB_OK;
 }

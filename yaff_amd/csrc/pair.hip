@@ -176,7 +176,7 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
         u[1] = i_is_a ? v[1] : -v[1];
         u[2] = i_is_a ? v[2] : -v[2];
       } else {
-        if (S.nstab > 0 && !self) {
+        if (S.any_stab && !self) {
           // Scaling applies to r == 0 rows only: r = n_ab + mic shift.
           const bool i_is_a = (li == A);
           const int nab[3] = {i_is_a ? nn[0] : -nn[0], i_is_a ? nn[1] : -nn[1], i_is_a ? nn[2] : -nn[2]};
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
         const ynb_pot &P = S.pot[p];
         if (!(dd < P.rcut)) continue;
         double s = 1.0;
-        if (S.nstab > 0 && r_zero) s = stab_scale(pp.stab + S.stab0, S.nstab, A, B);
+        if (P.nstab > 0 && r_zero) s = stab_scale(pp.stab + P.stab0, P.nstab, A, B);
         if (s == 0.0) continue;
         const int64_t gjj = gj;
         T g = T(0), e;
@@ -312,6 +312,98 @@ __global__ void k_pair_reduce(const ynb_sys *sys, const double *partial, double 
   }
 }
 
+
+// pair_pot_compute over explicit Yaff rows (yaff/pes/pair_pot.c:84-176):
+// one thread per row, rows in any order.  Gradients are scattered with FP64
+// atomics; energy and virial are reduced per block and then in order.
+__global__ __launch_bounds__(256) void k_pair_rows(const ynb_neigh_row *rows, int64_t nrow,
+                                                   const ynb_scaling_row *stab, int64_t nstab, ynb_pot P,
+                                                   const double *p0, const double *p1, const double *p2,
+                                                   const double *cross, double *gpos, double *partial) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double e = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
+  if (q < nrow) {
+    const ynb_neigh_row w = rows[q];
+    if (w.d < P.rcut) {
+      double s = 1.0;
+      if (w.r0 == 0 && w.r1 == 0 && w.r2 == 0 && nstab > 0) s = stab_scale(stab, (int)nstab, w.a, w.b);
+      if (s != 0.0) {
+        double g = 0.0, pe;
+        const double d = w.d;
+        switch (P.kind) {
+          case YNB_POT_LJ: pe = PotEval<double>::lj(p0[w.a], p0[w.b], p1[w.a], p1[w.b], d, g); break;
+          case YNB_POT_MM3:
+            pe = PotEval<double>::mm3(p0[w.a], p0[w.b], p1[w.a], p1[w.b], (int)p2[w.a] + (int)p2[w.b], d, g);
+            break;
+          case YNB_POT_EXPREP: {
+            const int nf = P.nffatype;
+            const int64_t ci = P.cross0 + (int64_t)((int)p0[w.a] * nf + (int)p0[w.b]);
+            pe = PotEval<double>::exprep(cross[ci], cross[ci + (int64_t)nf * nf], d, g);
+            break;
+          }
+          default: pe = PotEval<double>::ei(p0[w.a], p0[w.b], p1[w.a], p1[w.b], P.alpha, P.dielectric, d, g); break;
+        }
+        if (gpos == nullptr) {
+          if (P.trunc != YNB_TR_NONE && pe != 0.0) {
+            double hg;
+            pe *= (P.trunc == YNB_TR_SWITCH3) ? PotEval<double>::switch3(d, P.rcut, P.trunc_par, hg)
+                                              : PotEval<double>::hammer(d, P.rcut, P.trunc_par, hg);
+          }
+        } else if (P.trunc != YNB_TR_NONE && (pe != 0.0 || g != 0.0)) {
+          double hg;
+          const double h = (P.trunc == YNB_TR_SWITCH3) ? PotEval<double>::switch3(d, P.rcut, P.trunc_par, hg)
+                                                        : PotEval<double>::hammer(d, P.rcut, P.trunc_par, hg);
+          g = g * h + pe * hg / d;
+          pe *= h;
+        }
+        g *= s;
+        e = s * pe;
+        if (gpos) {
+          const double dl[3] = {w.dx, w.dy, w.dz};
+          for (int c = 0; c < 3; ++c) {
+            const double h = dl[c] * g;
+            atomicAdd(gpos + 3 * w.b + c, h);
+            atomicAdd(gpos + 3 * w.a + c, -h);
+          }
+          v[0] = w.dx * (w.dx * g);
+          v[1] = w.dy * (w.dy * g);
+          v[2] = w.dz * (w.dz * g);
+          v[3] = w.dx * (w.dy * g);
+          v[4] = w.dx * (w.dz * g);
+          v[5] = w.dy * (w.dz * g);
+        }
+      }
+    }
+  }
+  __shared__ double red[7][256];
+  const int t = threadIdx.x;
+  red[0][t] = e;
+  for (int c = 0; c < 6; ++c) red[1 + c][t] = v[c];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off)
+      for (int c = 0; c < 7; ++c) red[c][t] += red[c][t + off];
+    __syncthreads();
+  }
+  if (t < 7) partial[8 * (int64_t)blockIdx.x + t] = red[t][0];
+}
+
+__global__ void k_pair_rows_reduce(int64_t nblk, const double *partial, double *out) {
+  const int t = threadIdx.x;
+  if (t >= 7) return;
+  double acc = 0.0;
+  for (int64_t b = 0; b < nblk; ++b) acc += partial[8 * b + t];
+  __shared__ double r[7];
+  r[t] = acc;
+  __syncwarp();
+  if (t == 0) {
+    out[0] = r[0];
+    out[1] = r[1]; out[2] = r[4]; out[3] = r[5];
+    out[4] = r[4]; out[5] = r[2]; out[6] = r[6];
+    out[7] = r[5]; out[8] = r[6]; out[9] = r[3];
+  }
+}
+
 }  // namespace
 }  // namespace ynb
 
@@ -330,21 +422,38 @@ extern "C" int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const
   const bool slots = gpos_slots != nullptr;
   if (precision == YNB_FP64) {
     if (slots)
-      hipLaunchKernelGGL((k_pair<YNB_FP64, true>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
-                         slot_mask, gpos, accumulate, gpos_slots, partial);
+      YNB_LAUNCH("pair", (k_pair<YNB_FP64, true>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
+                 slot_mask, gpos, accumulate, gpos_slots, partial);
     else
-      hipLaunchKernelGGL((k_pair<YNB_FP64, false>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
-                         slot_mask, gpos, accumulate, gpos_slots, partial);
+      YNB_LAUNCH("pair", (k_pair<YNB_FP64, false>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
+                 slot_mask, gpos, accumulate, gpos_slots, partial);
   } else {
     if (slots)
-      hipLaunchKernelGGL((k_pair<YNB_MIXED, true>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
-                         slot_mask, gpos, accumulate, gpos_slots, partial);
+      YNB_LAUNCH("pair", (k_pair<YNB_MIXED, true>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
+                 slot_mask, gpos, accumulate, gpos_slots, partial);
     else
-      hipLaunchKernelGGL((k_pair<YNB_MIXED, false>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
-                         slot_mask, gpos, accumulate, gpos_slots, partial);
+      YNB_LAUNCH("pair", (k_pair<YNB_MIXED, false>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
+                 slot_mask, gpos, accumulate, gpos_slots, partial);
   }
   YNB_LAUNCHED();
   hipLaunchKernelGGL(k_pair_reduce, dim3(b->nsys), dim3(64), 0, st, b->sys, partial, out);
+  YNB_LAUNCHED();
+  return YNB_OK;
+}
+
+extern "C" int ynb_pair_rows(const ynb_neigh_row *rows, int64_t nrow, const ynb_scaling_row *stab, int64_t nstab,
+                             const ynb_pot *pot, const double *p0, const double *p1, const double *p2,
+                             const double *cross, double *gpos, double *partial, double *out, ynb_stream stream) {
+  if (!pot || nrow < 0 || !partial || !out || (nrow > 0 && !rows) || !p0) {
+    set_error("ynb_pair_rows: invalid arguments");
+    return YNB_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nblk = nrow > 0 ? ceil_div(nrow, 256) : 1;
+  hipLaunchKernelGGL(k_pair_rows, dim3((unsigned)nblk), dim3(256), 0, st, rows, nrow, stab, nstab, *pot, p0,
+                     p1 ? p1 : p0, p2 ? p2 : p0, cross ? cross : p0, gpos, partial);
+  YNB_LAUNCHED();
+  hipLaunchKernelGGL(k_pair_rows_reduce, dim3(1), dim3(64), 0, st, nblk, partial, out);
   YNB_LAUNCHED();
   return YNB_OK;
 }

@@ -29,20 +29,25 @@ KBLOCK = 256         # k vectors per reduction block
 class PotSpec(object):
     """One pair potential of a system.  p0/p1/p2 are per-atom parameters:
     LJ/MM3 sigma, epsilon (MM3: p2 = onlypauli); EI charge, radius; ExpRep
-    ffatype id with amp_cross/b_cross tables."""
+    ffatype id with amp_cross/b_cross tables.  ``stab`` is this potential's
+    scaling table (default: the system's)."""
 
     def __init__(self, kind, rcut, trunc=TR_NONE, trunc_par=0.0, p0=None, p1=None, p2=None, alpha=0.0,
-                 dielectric=1.0, amp_cross=None, b_cross=None):
+                 dielectric=1.0, amp_cross=None, b_cross=None, stab=None):
         self.kind, self.rcut, self.trunc, self.trunc_par = int(kind), float(rcut), int(trunc), float(trunc_par)
+        self.stab = None if stab is None else np.asarray(stab, dtype=SCALING_DTYPE)
         self.p0, self.p1, self.p2 = p0, p1, p2
         self.alpha, self.dielectric = float(alpha), float(dielectric)
         self.amp_cross, self.b_cross = amp_cross, b_cross
 
 
 class EwaldSpec(object):
-    def __init__(self, alpha, gcut, dielectric=1.0, nlow=0, nhigh=-1):
+    """Reciprocal-space settings; gmax defaults to ff.py:386 for the cell."""
+
+    def __init__(self, alpha, gcut, dielectric=1.0, nlow=0, nhigh=-1, gmax=None):
         self.alpha, self.gcut, self.dielectric = float(alpha), float(gcut), float(dielectric)
         self.nlow, self.nhigh = int(nlow), int(nhigh)
+        self.gmax = None if gmax is None else np.asarray(gmax, dtype=np.int64)
 
 
 class NBSystem(object):
@@ -72,12 +77,13 @@ class NBSystem(object):
         return len(self.pos)
 
 
-def ewald_kvectors(cell, alpha, gcut):
+def ewald_kvectors(cell, alpha, gcut, gmax=None):
     """The half-space k vectors compute_ewald_reci visits (ewald.c:41-58), with
     the same FP64 expressions for k and |k|^2 so that the cutoff test picks the
     same set.  Returned in row order (g2, g1 major, g0 ascending) with
     c_k = 4 pi/V exp(-k^2/4 alpha^2)/k^2."""
-    gmax = np.ceil(gcut/cell.gspacings - 0.5).astype(np.int64)
+    if gmax is None:
+        gmax = np.ceil(gcut/cell.gspacings - 0.5).astype(np.int64)
     kv = TWO_PI*cell.gvecs_full
     g0, g1, g2 = np.meshgrid(np.arange(-gmax[0], gmax[0] + 1), np.arange(-gmax[1], gmax[1] + 1),
                              np.arange(0, gmax[2] + 1), indexing='ij')
@@ -175,12 +181,15 @@ class Engine(object):
             check(L.ynb_sys_plan(h))
             h.bin0 = nbin
             nbin += h.nbin
-            h.stab0, h.nstab = stab_off, len(s.stab)
-            stab_off += len(s.stab)
-            stabs.append(s.stab)
             a0 = self.atom0[i]
             for p, pot in enumerate(s.pots):
                 hp = h.pot[p]
+                st = s.stab if pot.stab is None else pot.stab
+                hp.stab0, hp.nstab = stab_off, len(st)
+                stab_off += len(st)
+                if len(st):
+                    stabs.append(np.asarray(st, dtype=SCALING_DTYPE))
+                    h.any_stab = 1
                 hp.kind, hp.trunc, hp.rcut, hp.trunc_par = pot.kind, pot.trunc, pot.rcut, pot.trunc_par
                 hp.alpha, hp.dielectric = pot.alpha, pot.dielectric
                 for q, arr in enumerate((pot.p0, pot.p1, pot.p2)):
@@ -198,7 +207,7 @@ class Engine(object):
             self.bin_sys[self.hsys[i].bin0:self.hsys[i].bin0 + self.hsys[i].nbin] = i
         self.par_h = par
         self.cross_h = np.concatenate(cross) if cross else np.zeros(1)
-        self.stab_h = np.concatenate(stabs) if stab_off > 0 else np.zeros(1, SCALING_DTYPE)
+        self.stab_h = np.concatenate(stabs) if stabs else np.zeros(1, SCALING_DTYPE)
         if self.has_ewald:
             self._plan_ewald(seglen, kchunks)
 
@@ -207,7 +216,7 @@ class Engine(object):
         nk_all = 0
         for s in self.systems:
             if s.ewald is not None:
-                _, g, _ = ewald_kvectors(s.cell, s.ewald.alpha, s.ewald.gcut)
+                _, g, _ = ewald_kvectors(s.cell, s.ewald.alpha, s.ewald.gcut, s.ewald.gmax)
                 nk_all = max(nk_all, len(g))
         if seglen is None:
             seglen = 16 if nk_all > 20000 else 8
@@ -226,7 +235,7 @@ class Engine(object):
             ew = s.ewald
             if s.cell.nvec != 3:
                 raise ValueError('Ewald reciprocal needs a 3D periodic cell')
-            gmax, g, kv = ewald_kvectors(s.cell, ew.alpha, ew.gcut)
+            gmax, g, kv = ewald_kvectors(s.cell, ew.alpha, ew.gcut, ew.gmax)
             self.gmax.append(gmax)
             nk = len(g)
             nkp = max(_round_up(nk, KBLOCK), KBLOCK)
