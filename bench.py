@@ -1,0 +1,262 @@
+#!/usr/bin/env python
+"""Benchmark of the MI355X nonbonded engine.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (the headline value): the 6912-atom rocksalt system (12x12x6
+conventional cells, a = 5.64 A, +-0.05 A jitter, seed 0) with ExpRep + LJ +
+real-space EI (rcut 11 A, alpha 0.28/A) and the Ewald reciprocal term
+(gcut 0.32/A), in FP64.  One step is one velocity-Verlet MD step at
+dt = 1 fs done on the device: kick/drift, neighbour-list rebuild (skin 0),
+all pair potentials, Ewald reciprocal, kick.  Inputs are resident in HBM; the
+timed region holds K steps between two synchronisations.
+
+value = atom-force-evals/s = natom * K / elapsed.
+
+Also reported on the same line: per-kernel durations from HIP events recorded
+on the launch stream over the timed region, the roofline of the dominant
+kernel, the mixed-precision rate on the same system with its energy / force /
+virial error against the FP64 result, and the CPU baseline (the oracle's FP64
+restatement of Yaff's C path on one host core, bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+# Flop-equivalents per unit of work (add/sub/mul 1, FMA 2, div/sqrt/exp/erfc/
+# sin/cos 1), read off the reference C expressions.
+FLOP_PAIR_TAIL = 57          # pair_pot.c:141-171 (scaling, gpos, vtens, energy)
+FLOP_POT = {'lj': 19, 'ei': 18, 'exprep': 5, 'mm3': 22}
+FLOP_TRUNC = 8               # chain rule pair_pot.c:134-138
+FLOP_SWITCH3 = (1, 10)       # flat zone, switching zone
+FLOP_EWALD_SK = 11           # ewald.c:62-76 per (k, atom): k.r 5, q cos/q sin 2, sums 2, sin, cos
+FLOP_EWALD_FORCE = 9         # ewald.c:104-111 per (k, atom)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-mixed', action='store_true')
+    ap.add_argument('--cpu-evals', type=int, default=2)
+    return ap.parse_args()
+
+
+def rocksalt_system(reps=(12, 12, 6), seed=0, angles=(90.0, 90.0, 90.0)):
+    from yaff_amd import workloads
+    from yaff_amd.engine import NBSystem, PotSpec, EwaldSpec
+    from yaff_amd import _lib
+    s = workloads.rocksalt(reps, seed=seed, angles=angles)
+    sp = s.spec
+    n = s.natom
+    pots = [PotSpec(_lib.POT_EXPREP, sp.rcut, p0=s.ffatype_ids.astype(float), amp_cross=np.full((2, 2), sp.exprep_a),
+                    b_cross=np.full((2, 2), sp.exprep_b)),
+            PotSpec(_lib.POT_LJ, sp.rcut, p0=np.full(n, sp.lj_sigma), p1=np.full(n, sp.lj_epsilon)),
+            PotSpec(_lib.POT_EI, sp.rcut, p0=s.charges, alpha=sp.alpha)]
+    nb = NBSystem(s.pos, s.cell, pots, rc=sp.rcut, ewald=EwaldSpec(sp.alpha, sp.gcut), charges=s.charges,
+                  masses=s.masses)
+    return s, nb
+
+
+def maxwell_boltzmann(masses, temp, seed):
+    """yaff/sampling/utils.py:42-75 with scalevel0."""
+    from yaff_amd.units import boltzmann
+    rng = np.random.default_rng(seed)
+    v = rng.normal(0, 1, (len(masses), 3))*np.sqrt(boltzmann*temp/masses).reshape(-1, 1)
+    t = (v**2*masses.reshape(-1, 1)).mean()/boltzmann
+    return v*np.sqrt(temp/t)
+
+
+def pair_flops(engine, s_idx=0):
+    """Algorithmic flop-equivalents of one pair launch: half pairs within each
+    potential's cutoff times its per-pair count."""
+    import torch
+    eng = engine
+    sys_ = eng.systems[s_idx]
+    n = sys_.natom
+    cnt = eng.cnt_d[:n].cpu().numpy()
+    half_pairs = int(cnt.sum())//2
+    total = 0
+    names = {0: 'lj', 1: 'mm3', 2: 'exprep', 3: 'ei'}
+    for p in sys_.pots:
+        f = FLOP_POT[names[p.kind]] + FLOP_PAIR_TAIL
+        if p.trunc:
+            f += FLOP_TRUNC + FLOP_SWITCH3[1]
+        total += half_pairs*f
+    return total, half_pairs
+
+
+def run_steps(eng, K):
+    import torch
+    for _ in range(K):
+        eng.md_step()
+
+
+def timed(eng, K, profile=True):
+    import torch
+    from yaff_amd import _lib
+    torch.cuda.synchronize()
+    if profile:
+        _lib.profile_enable(True)
+    t0 = time.perf_counter()
+    run_steps(eng, K)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = _lib.profile_read() if profile else []
+    if profile:
+        _lib.profile_enable(False)
+    return t1 - t0, prof
+
+
+def kernel_table(prof, K):
+    tab = {}
+    for name, ms in prof:
+        t = tab.setdefault(name, [0, 0.0])
+        t[0] += 1
+        t[1] += ms
+    return {k: {'launches': v[0], 'avg_ms': v[1]/v[0], 'ms_per_step': v[1]/K} for k, v in tab.items()}
+
+
+def cpu_baseline(s, evals):
+    """The oracle (FP64 restatement of Yaff's C path, one host core) on the same
+    system: neighbour list, three pair potentials and the Ewald reciprocal."""
+    from oracle import oracle as orc
+    O = orc.Oracle()
+    sp = s.spec
+    n = s.natom
+    cell = O.cell(s.cell.rvecs)
+    rmax = orc.rmax_of(cell, sp.rcut)
+    gmax = orc.gmax_of(cell, sp.gcut)
+    pots = [orc.Pot(orc.EXPREP, sp.rcut, ffatype_ids=s.ffatype_ids, amp_cross=np.full((2, 2), sp.exprep_a),
+                    b_cross=np.full((2, 2), sp.exprep_b)),
+            orc.Pot(orc.LJ, sp.rcut, p0=np.full(n, sp.lj_sigma), p1=np.full(n, sp.lj_epsilon)),
+            orc.Pot(orc.EI, sp.rcut, p0=s.charges, alpha=sp.alpha)]
+    t0 = time.perf_counter()
+    res = None
+    for _ in range(evals):
+        rows = O.nlist_build(s.pos, cell, sp.rcut, rmax)
+        e = 0.0
+        g = np.zeros((n, 3))
+        v = np.zeros((3, 3))
+        for p in pots:
+            pe, pg, pv = O.pair_compute(rows, None, p, n)
+            e += pe
+            g += pg
+            v += pv
+        pe, pg, pv = O.ewald_reci(s.pos, s.charges, cell, sp.alpha, gmax, sp.gcut)
+        res = (e + pe, g + pg, v + pv)
+    t = time.perf_counter() - t0
+    return n*evals/t, t, res
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: every box here has a single MI355X and no run uses a second; '
+              'run with --gpus 1 (see DESIGN.md for how the path shards).', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from yaff_amd.engine import Engine
+    from yaff_amd.units import femtosecond
+    K, W = args.steps, args.warmup
+
+    s, nb = rocksalt_system()
+    n = s.natom
+    eng = Engine([nb], precision='fp64')
+    eng.ensure_nlist()
+    eng.compute(build=False)
+    eng.set_velocities([maxwell_boltzmann(s.masses, 300.0, 1)])
+    eng.set_timestep(1.0*femtosecond)
+    run_steps(eng, W)
+    torch.cuda.synchronize()
+    flags, maxcnt = eng.nlist_status()
+    if flags:
+        eng.ensure_nlist()
+    dt, prof = timed(eng, K)
+    flags, _ = eng.nlist_status()
+    if flags:
+        raise RuntimeError('neighbour list overflowed during the timed steps')
+    value = n*K/dt
+    kt = kernel_table(prof, K)
+
+    # roofline of the dominant kernel
+    nk_real = int((eng.ewald_h['kvec'][:, 3] != 0).sum())
+    pflops, half_pairs = pair_flops(eng)
+    algo = {'ewald_sk': nk_real*n*FLOP_EWALD_SK, 'ewald_force': nk_real*n*FLOP_EWALD_FORCE, 'pair': pflops}
+    dom = max(kt, key=lambda k: kt[k]['ms_per_step'])
+    for k, v in kt.items():
+        if k in algo:
+            v['achieved_tflops'] = algo[k]/(v['avg_ms']*1e-3)/1e12
+            v['frac_fp64_peak'] = v['achieved_tflops']/FP64_PEAK_TFLOPS
+    if dom in algo:
+        ach = algo[dom]/(kt[dom]['avg_ms']*1e-3)/1e12
+        roofline = {'kernel': dom, 'bound': 'valu', 'achieved': ach, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                    'frac': ach/FP64_PEAK_TFLOPS, 'traffic': None,
+                    'algorithmic_flop_per_launch': algo[dom]}
+    else:
+        roofline = {'kernel': dom, 'bound': 'hbm', 'achieved': None, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                    'frac': None, 'traffic': None}
+
+    out = {'metric': 'atom-force-evals/s (FP64 vs mixed @ RMS force err<1e-5) + p99 served latency',
+           'value': value, 'unit': 'atom-force-evals/s', 'n_gpus': 1, 'steps': K, 'warmup': W,
+           'ms_per_step': dt/K*1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+           'dtype': 'f64', 'data': 'synthetic',
+           'config': {'workload': 'rocksalt 6912 atoms (12x12x6 cells), ExpRep+LJ+EI rcut 11 A + Ewald reci '
+                                  'gcut 0.32/A, FP64, velocity-Verlet MD step with nlist rebuild',
+                      'natom': n, 'half_pairs': half_pairs, 'kvectors': nk_real, 'parallelism': 'single GPU'},
+           'roofline': roofline, 'kernels': kt}
+
+    # mixed precision on the same system: rate and error against FP64
+    if not args.no_mixed:
+        eng.upload_positions()
+        eng.ensure_nlist()
+        eng.compute(build=False)
+        e64, g64, v64 = eng.results(0)
+        engm = Engine([nb], precision='mixed')
+        engm.ensure_nlist()
+        engm.compute(build=False)
+        em, gm, vm = engm.results(0)
+        engm.set_velocities([maxwell_boltzmann(s.masses, 300.0, 1)])
+        engm.set_timestep(1.0*femtosecond)
+        run_steps(engm, W)
+        dtm, profm = timed(engm, K)
+        rms = np.sqrt(((gm - g64)**2).sum(axis=1).mean())/np.sqrt((g64**2).sum(axis=1).mean())
+        out['mixed'] = {'value': n*K/dtm, 'ms_per_step': dtm/K*1e3,
+                        'energy_rel_err': abs(em - e64)/abs(e64),
+                        'force_rms_rel_err': float(rms),
+                        'force_max_abs_err': float(np.abs(gm - g64).max()),
+                        'virial_rel_err': float(np.abs(vm - v64).max()/np.abs(v64).max()),
+                        'kernels': kernel_table(profm, K)}
+
+    if not args.no_cpu_baseline:
+        try:
+            rate, t, res = cpu_baseline(s, args.cpu_evals)
+            out['cpu_baseline'] = {'value': rate, 'unit': 'atom-force-evals/s', 'cores': 1, 'kind': 'port',
+                                   'sample': '%d full FP64 evaluations (nlist + ExpRep/LJ/EI + Ewald reci) of the '
+                                             '6912-atom system, %.1f s' % (args.cpu_evals, t)}
+            eng.upload_positions()
+            eng.ensure_nlist()
+            eng.compute(build=False)
+            e, g, v = eng.results(0)
+            out['cpu_baseline']['gpu_vs_oracle_energy_rel_err'] = abs(e - res[0])/abs(res[0])
+            out['cpu_baseline']['gpu_vs_oracle_force_max_rel_err'] = float(np.abs(g - res[1]).max()/np.abs(res[1]).max())
+        except OSError as exc:
+            out['cpu_baseline'] = {'value': None, 'unit': 'atom-force-evals/s', 'cores': 1, 'kind': 'port',
+                                   'sample': 'unavailable: %s' % exc}
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

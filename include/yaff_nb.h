@@ -90,7 +90,7 @@ typedef struct ynb_sys {
   int32_t bin0;       /* first global bin of this system */
   int32_t nbin;
   int32_t any_stab;   /* nonzero when a potential of the system has scaling rows */
-  int32_t pad0_;
+  int32_t box_safe;   /* ynb_sys_plan: Yaff's image box cannot reject a pair within rc */
   ynb_pot pot[YNB_MAX_POT];
   /* Ewald reciprocal part (nk == 0: none) */
   double alpha, dielectric;
@@ -109,11 +109,12 @@ typedef struct ynb_batch {
   int32_t nsys;
   int32_t nbin;              /* total bins (0 when every system uses the all-pairs path) */
   int64_t npad;              /* padded atom count, multiple of 64 */
-  int32_t max_bin_atoms;     /* not used by the kernels; informative */
+  int32_t max_bin_atoms;     /* upper bound of atoms per bin (sizes the build grid; 0: default) */
   int32_t pad_;
 } ynb_batch;
 
-/* Full neighbour list in ELL layout: entry k of atom i is at k*npad + i.
+/* Full neighbour list, one row of kcap entries per atom: entry k of atom i is
+ * at i*kcap + k.
  * j is the padded index of the partner, img packs the integer translation n
  * (10 bits per axis, offset 512) with pos_j + n.R the partner image. */
 typedef struct ynb_nlist {
@@ -202,7 +203,7 @@ int ynb_nlist_recompute(const ynb_sys *h_sys, const double *pos, const double *p
  * gpos (nullable): [npad][3] gradient summed over the slots, written or, with
  * accumulate != 0, added.  gpos_slots (nullable): [YNB_MAX_POT][npad][3].
  * out: [nsys][YNB_MAX_POT][10] energy followed by the 3x3 virial.
- * partial: work of (npad/64) * YNB_MAX_POT * 8 doubles. */
+ * partial: work of npad/4 * YNB_MAX_POT * 8 doubles (16 lanes per atom). */
 int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const double *pos,
                      const ynb_nlist *nl, uint32_t slot_mask, int32_t precision,
                      double *gpos, int32_t accumulate, double *gpos_slots,

@@ -111,3 +111,33 @@ def test_ewald_reci_fp64(name):
     assert abs(out[0] - e) <= TOL*abs(e), (out[0], e)
     assert rel(g, REF[name + '/reci/gpos']) <= TOL
     assert rel(out[1:].reshape(3, 3), REF[name + '/reci/vtens']) <= TOL
+
+
+# Mixed precision: FP64 arithmetic with FP32 exp/erfc/sincos on FP64-reduced
+# arguments.  Tolerances: RMS force error < 1e-5 of the RMS force (the
+# north-star bound), energy and virial within 1e-6 relative.
+MIXED_FORCE_RMS_TOL = 1e-5
+MIXED_REL_TOL = 1e-6
+
+
+@pytest.mark.parametrize('name', ['rocksalt144_tri', 'water32', 'lj108_tri', 'quartz222'])
+def test_mixed_precision_error(name):
+    print('case', name, flush=True)
+    c = CASES[name]
+    n = len(c['pos'])
+    eng = _engine(c, precision='mixed')
+    eng.ensure_nlist()
+    eng.compute(build=False)
+    e, g, v = eng.results(0)
+    e_ref = sum(float(REF['%s/pot%d/e' % (name, ip)]) for ip in range(len(c['pots'])))
+    g_ref = sum(REF['%s/pot%d/gpos' % (name, ip)] for ip in range(len(c['pots'])))
+    v_ref = sum(REF['%s/pot%d/vtens' % (name, ip)] for ip in range(len(c['pots'])))
+    if c['ewald'] is not None:
+        e_ref += float(REF[name + '/reci/e'])
+        g_ref = g_ref + REF[name + '/reci/gpos']
+        v_ref = v_ref + REF[name + '/reci/vtens']
+    rms = np.sqrt(((g - g_ref)**2).sum(axis=1).mean())/np.sqrt((g_ref**2).sum(axis=1).mean())
+    print('mixed', name, 'energy', abs(e - e_ref)/abs(e_ref), 'force rms', rms, flush=True)
+    assert abs(e - e_ref) <= MIXED_REL_TOL*abs(e_ref)
+    assert rms <= MIXED_FORCE_RMS_TOL
+    assert np.abs(v - v_ref).max() <= MIXED_REL_TOL*np.abs(v_ref).max()

@@ -33,7 +33,7 @@ class YnbSys(C.Structure):
                 ('volume', C.c_double), ('nvec', C.c_int32), ('natom', C.c_int32), ('atom0', C.c_int32),
                 ('npot', C.c_int32), ('rmax', C.c_int32 * 3), ('nlow', C.c_int32), ('nhigh', C.c_int32),
                 ('grid', C.c_int32 * 3), ('range', C.c_int32 * 3), ('bin0', C.c_int32), ('nbin', C.c_int32),
-                ('any_stab', C.c_int32), ('pad0_', C.c_int32), ('pot', YnbPot * MAX_POT),
+                ('any_stab', C.c_int32), ('box_safe', C.c_int32), ('pot', YnbPot * MAX_POT),
                 ('alpha', C.c_double), ('dielectric', C.c_double), ('k0', C.c_int32), ('nk', C.c_int32),
                 ('row0', C.c_int32), ('nrow', C.c_int32), ('seg0', C.c_int32), ('nseg', C.c_int32),
                 ('chunk0', C.c_int32), ('nchunk', C.c_int32), ('spart0', C.c_int64)]

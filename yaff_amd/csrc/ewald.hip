@@ -52,15 +52,19 @@ __global__ void k_ewald_prep(const ynb_sys *sys, const int32_t *wave_sys, int64_
   step[npad + g] = s;
 }
 
-template <typename T>
-__device__ inline void sincospi_t(double x, T *s, T *c);
-template <>
-__device__ inline void sincospi_t<double>(double x, double *s, double *c) {
-  sincospi(x, s, c);
-}
-template <>
-__device__ inline void sincospi_t<float>(double x, float *s, float *c) {
-  sincospif((float)x, s, c);
+// Seed of a recurrence: sin and cos of pi x for x = 2 (phase in turns) in
+// [-1, 1].  Mixed mode evaluates it in FP32 on the FP64-reduced argument;
+// the recurrences and all sums stay in FP64 in both modes.
+template <int PREC>
+__device__ inline void seed_sincospi(double x, double *s, double *c) {
+  if (PREC == YNB_FP64) {
+    sincospi(x, s, c);
+  } else {
+    float sf, cf;
+    sincospif((float)x, &sf, &cf);
+    *s = sf;
+    *c = cf;
+  }
 }
 
 // Phase g0 f0 + g1 f1 + g2 f2 in turns reduced to [-1/2, 1/2] in FP64.
@@ -71,7 +75,7 @@ __device__ inline double reduced_phase(int g0, int g1, int g2, double f0, double
 
 template <int PREC, int L>
 __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *sys, ynb_ewald ew, int64_t npad) {
-  using T = typename std::conditional<PREC == YNB_FP64, double, float>::type;
+  using T = double;
   __shared__ double l_f[3][CHUNK];
   __shared__ T l_q[CHUNK];
   __shared__ T l_st[2][CHUNK];
@@ -107,7 +111,7 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *sys, ynb_ewald e
       for (int t = 0; t < na; ++t) {
         const double ph = reduced_phase(sg.x, sg.y, sg.z, l_f[0][t], l_f[1][t], l_f[2][t]);
         T s, c;
-        sincospi_t<T>(2.0 * ph, &s, &c);
+        seed_sincospi<PREC>(2.0 * ph, &s, &c);
         const T q = l_q[t];
         T zr = q * c, zi = q * s;
         const T cr = l_st[0][t], ci = l_st[1][t];
@@ -187,7 +191,7 @@ __global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_e
 template <int PREC, bool GROUPS>
 __global__ __launch_bounds__(64) void k_ewald_force(const ynb_sys *sys, const int32_t *wave_sys, ynb_ewald ew,
                                                     int64_t npad) {
-  using T = typename std::conditional<PREC == YNB_FP64, double, float>::type;
+  using T = double;
   const int64_t wave = blockIdx.x;
   const int kc = blockIdx.y;
   const int lane = threadIdx.x;
@@ -213,7 +217,7 @@ __global__ __launch_bounds__(64) void k_ewald_force(const ynb_sys *sys, const in
     const int4 kr = *(const int4 *)(ew.krow + 4 * row);
     const int kf = ew.krow_k[row];
     T s, c;
-    sincospi_t<T>(2.0 * reduced_phase(kr.z, kr.x, kr.y, f0, f1, f2), &s, &c);
+    seed_sincospi<PREC>(2.0 * reduced_phase(kr.z, kr.x, kr.y, f0, f1, f2), &s, &c);
     T zr = c, zi = s;
     T R0 = T(0), R1 = T(0), R2 = T(0);
     for (int m = 0; m < kr.w; ++m) {

@@ -118,8 +118,8 @@ __global__ void k_binrange(int64_t npad, const int32_t *key_sorted, const int32_
 __device__ inline void store_entry(const ynb_nlist &nl, int64_t npad, int64_t gi, int cnt, int32_t gj,
                                    int32_t img) {
   if (cnt < nl.kcap) {
-    nl.j[(int64_t)cnt * npad + gi] = gj;
-    nl.img[(int64_t)cnt * npad + gi] = img;
+    nl.j[gi * nl.kcap + cnt] = gj;
+    nl.img[gi * nl.kcap + cnt] = img;
   }
 }
 
@@ -130,35 +130,60 @@ __device__ inline void finish_count(const ynb_nlist &nl, int64_t gi, int cnt, bo
   atomicMax(nl.status + 1, cnt);
 }
 
-// One 64-lane block per bin: lanes own the bin's atoms, partner bins are
-// staged through LDS 64 atoms at a time.
-__global__ __launch_bounds__(64) void k_build_binned(const ynb_sys *sys, const int32_t *bin_sys,
-                                                     int64_t npad, const double *pos,
-                                                     const int32_t *bin_start, const int32_t *bin_end,
-                                                     const double4 *sxw, const int4 *sw, ynb_nlist nl) {
-  __shared__ double4 l_xw[64];
-  __shared__ int4 l_w[64];
+// Lanes per atom in the binned build and the pair kernel.
+constexpr int G = 16;
+constexpr int BA = 256 / G;   // atoms per block
+constexpr int TILE = 256;     // partner atoms staged in LDS per pass
+
+// Segmented exclusive prefix sum over the G lanes of one atom.
+__device__ inline int group_exclusive_scan(int x, int s, int *total) {
+  int incl = x;
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const int y = __shfl_up(incl, d, G);
+    if (s >= d) incl += y;
+  }
+  *total = __shfl(incl, G - 1, G);
+  return incl - x;
+}
+
+// Block (bin b, chunk of BA atoms of the bin): G lanes per atom split the
+// partner candidates; partner bins are staged through LDS TILE atoms at a
+// time.  Two passes: count, then write at the lane's prefix offset, so that
+// every atom's row is in a fixed order (partner bins in offset order, partner
+// atoms in sorted order, interleaved by lane).
+//
+// Acceptance: the cheap FP64 distance on wrapped coordinates decides unless
+// it lies within a relative 1e-9 of rc^2 (or the image box could bind, see
+// ynb_sys_plan), where the exact Yaff expression decides.
+__global__ __launch_bounds__(256) void k_build_binned(const ynb_sys *sys, const int32_t *bin_sys, int64_t npad,
+                                                      const double *pos, const int32_t *bin_start,
+                                                      const int32_t *bin_end, const double4 *sxw, const int4 *sw,
+                                                      ynb_nlist nl) {
+  __shared__ double l_x[TILE], l_y[TILE], l_z[TILE];
+  __shared__ int4 l_w[TILE];
   const int b = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
+  const int al = tid / G, s = tid % G;
   const int sid = bin_sys[b];
   const ynb_sys &S = sys[sid];
   const int start = bin_start[b], end = bin_end[b];
-  if (end <= start) return;
+  if (start + (int)blockIdx.y * BA >= end) return;
   const int lb = b - S.bin0;
   const int c2 = lb % S.grid[2], c1 = (lb / S.grid[2]) % S.grid[1], c0 = lb / (S.grid[1] * S.grid[2]);
   const double rc2 = S.rc * S.rc;
-  const double rcp = S.rc * (1.0 + 1e-8) + 1e-6;
-  const double rcp2 = rcp * rcp;
-  for (int ib = start; ib < end; ib += 64) {
-    const int p = ib + lane;
+  const double rc2_hi = rc2 * (1.0 + 1e-9), rc2_lo = rc2 * (1.0 - 1e-9);
+  const bool fast_ok = S.box_safe != 0;
+  for (int ib = start + (int)blockIdx.y * BA; ib < end; ib += (int)gridDim.y * BA) {
+    const int p = ib + al;
     const bool active = p < end;
     double4 xi = make_double4(0, 0, 0, 0);
-    int4 wi = make_int4(0, 0, 0, -1);
+    int4 wi = make_int4(0, 0, 0, 0);
     if (active) {
       xi = sxw[p];
       wi = sw[p];
     }
-    const int64_t gi = wi.w;
+    const int64_t gi = active ? wi.w : 0;
     const int li = (int)(gi - S.atom0);
     double pi[3] = {0, 0, 0};
     if (active) {
@@ -166,66 +191,83 @@ __global__ __launch_bounds__(64) void k_build_binned(const ynb_sys *sys, const i
       pi[1] = pos[3 * gi + 1];
       pi[2] = pos[3 * gi + 2];
     }
-    int cnt = 0;
+    int cnt = 0, off = 0, total = 0;
     bool bad = false;
-    for (int d0 = -S.range[0]; d0 <= S.range[0]; ++d0) {
-      int nb0 = c0 + d0, s0 = floor_div(nb0, S.grid[0]);
-      nb0 -= s0 * S.grid[0];
-      for (int d1 = -S.range[1]; d1 <= S.range[1]; ++d1) {
-        int nb1 = c1 + d1, s1 = floor_div(nb1, S.grid[1]);
-        nb1 -= s1 * S.grid[1];
-        for (int d2 = -S.range[2]; d2 <= S.range[2]; ++d2) {
-          int nb2 = c2 + d2, s2 = floor_div(nb2, S.grid[2]);
-          nb2 -= s2 * S.grid[2];
-          const int nbin = S.bin0 + (nb0 * S.grid[1] + nb1) * S.grid[2] + nb2;
-          const double sx = s0 * S.rvecs[0] + s1 * S.rvecs[3] + s2 * S.rvecs[6];
-          const double sy = s0 * S.rvecs[1] + s1 * S.rvecs[4] + s2 * S.rvecs[7];
-          const double sz = s0 * S.rvecs[2] + s1 * S.rvecs[5] + s2 * S.rvecs[8];
-          const int js = bin_start[nbin], je = bin_end[nbin];
-          for (int jb = js; jb < je; jb += 64) {
-            __syncthreads();
-            if (jb + lane < je) {
-              l_xw[lane] = sxw[jb + lane];
-              l_w[lane] = sw[jb + lane];
-            }
-            __syncthreads();
-            if (!active) continue;
-            const int nj = min(64, je - jb);
-            for (int t = 0; t < nj; ++t) {
-              const double4 xj = l_xw[t];
-              const double dx = xj.x + sx - xi.x, dy = xj.y + sy - xi.y, dz = xj.z + sz - xi.z;
-              const double d2 = fma(dx, dx, fma(dy, dy, dz * dz));
-              if (!(d2 < rcp2)) continue;
-              const int4 wj = l_w[t];
-              const int gj = wj.w;
-              const int n[3] = {s0 - wj.x + wi.x, s1 - wj.y + wi.y, s2 - wj.z + wi.z};
-              const bool self = (gj == gi);
-              if (self && n[0] == 0 && n[1] == 0 && n[2] == 0) continue;
-              const int lj = gj - S.atom0;
-              const int A = li > lj ? li : lj, B = li > lj ? lj : li;
-              if (!nlow_nhigh_ok(A, B, S.nlow, S.nhigh)) continue;
-              const bool i_is_a = (li == A);
-              const int nab[3] = {i_is_a ? n[0] : -n[0], i_is_a ? n[1] : -n[1], i_is_a ? n[2] : -n[2]};
-              double pj[3] = {pos[3 * (int64_t)gj], pos[3 * (int64_t)gj + 1], pos[3 * (int64_t)gj + 2]};
-              const double *pA = i_is_a ? pi : pj;
-              const double *pB = i_is_a ? pj : pi;
-              double v[3];
-              int r[3];
-              if (!yaff_row_vector(pA, pB, self, nab, S.rvecs, S.gvecs, 3, S.rmax, v, r)) continue;
-              const double e2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-              if (!(e2 < rc2)) continue;
-              if (n[0] < -511 || n[0] > 511 || n[1] < -511 || n[1] > 511 || n[2] < -511 || n[2] > 511) {
-                bad = true;
-                continue;
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int d0 = -S.range[0]; d0 <= S.range[0]; ++d0) {
+        int nb0 = c0 + d0, s0 = floor_div(nb0, S.grid[0]);
+        nb0 -= s0 * S.grid[0];
+        for (int d1 = -S.range[1]; d1 <= S.range[1]; ++d1) {
+          int nb1 = c1 + d1, s1 = floor_div(nb1, S.grid[1]);
+          nb1 -= s1 * S.grid[1];
+          for (int d2 = -S.range[2]; d2 <= S.range[2]; ++d2) {
+            int nb2 = c2 + d2, s2 = floor_div(nb2, S.grid[2]);
+            nb2 -= s2 * S.grid[2];
+            const int nbin = S.bin0 + (nb0 * S.grid[1] + nb1) * S.grid[2] + nb2;
+            const double sx = s0 * S.rvecs[0] + s1 * S.rvecs[3] + s2 * S.rvecs[6];
+            const double sy = s0 * S.rvecs[1] + s1 * S.rvecs[4] + s2 * S.rvecs[7];
+            const double sz = s0 * S.rvecs[2] + s1 * S.rvecs[5] + s2 * S.rvecs[8];
+            const double bx = sx - xi.x, by = sy - xi.y, bz = sz - xi.z;
+            const int js = bin_start[nbin], je = bin_end[nbin];
+            for (int jb = js; jb < je; jb += TILE) {
+              __syncthreads();
+              if (jb + tid < je) {
+                const double4 x = sxw[jb + tid];
+                l_x[tid] = x.x;
+                l_y[tid] = x.y;
+                l_z[tid] = x.z;
+                l_w[tid] = sw[jb + tid];
               }
-              store_entry(nl, npad, gi, cnt, gj, pack_img(n[0], n[1], n[2]));
-              ++cnt;
+              __syncthreads();
+              if (!active) continue;
+              const int nj = min(TILE, je - jb);
+              for (int q = s; q < nj; q += G) {
+                const double dx = l_x[q] + bx, dy = l_y[q] + by, dz = l_z[q] + bz;
+                const double dd2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                if (!(dd2 < rc2_hi)) continue;
+                const int4 wj = l_w[q];
+                const int gj = wj.w;
+                const int n[3] = {s0 - wj.x + wi.x, s1 - wj.y + wi.y, s2 - wj.z + wi.z};
+                const bool self = (gj == gi);
+                if (self && n[0] == 0 && n[1] == 0 && n[2] == 0) continue;
+                const int lj = gj - S.atom0;
+                const int A = li > lj ? li : lj, B = li > lj ? lj : li;
+                if (!nlow_nhigh_ok(A, B, S.nlow, S.nhigh)) continue;
+                if (!(fast_ok && dd2 < rc2_lo)) {
+                  const bool i_is_a = (li == A);
+                  const int nab[3] = {i_is_a ? n[0] : -n[0], i_is_a ? n[1] : -n[1], i_is_a ? n[2] : -n[2]};
+                  const double pj[3] = {pos[3 * (int64_t)gj], pos[3 * (int64_t)gj + 1], pos[3 * (int64_t)gj + 2]};
+                  double v[3];
+                  int r[3];
+                  if (!yaff_row_vector(i_is_a ? pi : pj, i_is_a ? pj : pi, self, nab, S.rvecs, S.gvecs, 3, S.rmax, v,
+                                       r))
+                    continue;
+                  if (!(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] < rc2)) continue;
+                }
+                if (n[0] < -511 || n[0] > 511 || n[1] < -511 || n[1] > 511 || n[2] < -511 || n[2] > 511) {
+                  bad = true;
+                  continue;
+                }
+                if (pass == 0) {
+                  ++cnt;
+                } else {
+                  if (off < nl.kcap) {
+                    nl.j[gi * nl.kcap + off] = gj;
+                    nl.img[gi * nl.kcap + off] = pack_img(n[0], n[1], n[2]);
+                  }
+                  ++off;
+                }
+              }
             }
           }
         }
       }
+      if (pass == 0) {
+        off = group_exclusive_scan(cnt, s, &total);
+        if (active && s == 0) finish_count(nl, gi, total, false);
+      }
     }
-    if (active) finish_count(nl, gi, cnt, bad);
+    if (active && bad) atomicOr(nl.status, 2);
   }
 }
 
@@ -300,7 +342,7 @@ __global__ void k_rows_count(const ynb_sys *sys, int s, int64_t npad, ynb_nlist 
   const int n = min(nl.cnt[gi], nl.kcap);
   int c = 0;
   for (int k = 0; k < n; ++k)
-    c += canonical_entry(gi, nl.j[(int64_t)k * npad + gi], nl.img[(int64_t)k * npad + gi]);
+    c += canonical_entry(gi, nl.j[gi * nl.kcap + k], nl.img[gi * nl.kcap + k]);
   rowcnt[li] = c;
 }
 
@@ -317,8 +359,8 @@ __global__ void k_rows_emit(const ynb_sys *sys, int s, int64_t npad, const doubl
   const uint64_t nimg = (uint64_t)(2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
   const double pi[3] = {pos[3 * gi], pos[3 * gi + 1], pos[3 * gi + 2]};
   for (int k = 0; k < n; ++k) {
-    const int32_t gj = nl.j[(int64_t)k * npad + gi];
-    const int32_t img = nl.img[(int64_t)k * npad + gi];
+    const int32_t gj = nl.j[gi * nl.kcap + k];
+    const int32_t img = nl.img[gi * nl.kcap + k];
     if (!canonical_entry(gi, gj, img)) continue;
     int nn[3];
     unpack_img(img, nn);
@@ -477,6 +519,16 @@ int ynb_sys_plan(ynb_sys *s) {
     nbin = (int64_t)s->grid[0] * s->grid[1] * s->grid[2];
   }
   s->nbin = (int32_t)nbin;
+  // The image box |r_k| <= rmax_k never rejects a pair within rc when
+  // rc*|g_k| < rmax_k + 1/2 on every axis: r_k = g_k.v - g_k.delta0 with
+  // |g_k.v| < rc*|g_k| and |g_k.delta0| <= 1/2 after Yaff's wrap.  Then the
+  // cheap distance alone decides away from the rc^2 boundary.
+  s->box_safe = 1;
+  for (int k = 0; k < 3; ++k) {
+    const double *g = s->gvecs + 3 * k;
+    const double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    if (!(s->rc * gn < s->rmax[k] + 0.5 - 1e-6)) s->box_safe = 0;
+  }
   return YNB_OK;
 }
 
@@ -512,8 +564,9 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
     hipLaunchKernelGGL(k_binrange, dim3(nb), dim3(tb), 0, st, npad, w.key_sorted, w.val_sorted, w.xw, w.wr,
                        w.bin_start, w.bin_end, w.sxw, w.sw);
     YNB_LAUNCHED();
-    YNB_LAUNCH("nlist_build_binned", k_build_binned, dim3(b->nbin), dim3(64), 0, st, b->sys, b->bin_sys, npad, pos,
-               w.bin_start, w.bin_end, w.sxw, w.sw, *nl);
+    const int ychunks = b->max_bin_atoms > 0 ? (int)ceil_div(b->max_bin_atoms, BA) : 4;
+    YNB_LAUNCH("nlist_build_binned", k_build_binned, dim3(b->nbin, ychunks), dim3(256), 0, st, b->sys, b->bin_sys,
+               npad, pos, w.bin_start, w.bin_end, w.sxw, w.sw, *nl);
     YNB_LAUNCHED();
   }
   YNB_LAUNCH("nlist_build_allpairs", k_build_allpairs, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos,

@@ -21,60 +21,86 @@ namespace {
 
 constexpr double TWO_DIV_SQRT_PI = 1.1283791670955125739;  // yaff/pes/constants.h:31
 
-template <typename T>
+// Transcendentals of the two precisions.  FP64 mode calls the FP64 device
+// library; mixed mode keeps every argument reduction in FP64 and evaluates
+// only a well-conditioned remainder in FP32:
+//   exp(a)   = 2^k * exp2f(f),  k = rint(a log2 e), f = a log2 e - k in FP64;
+//   erfc(x)  = exp(-x^2) * erfcxf(x), erfcx = exp(x^2) erfc(x) being smooth,
+//              so rounding x to FP32 barely moves it.
+template <int PREC>
+struct Tr {
+  __device__ static double ex(double a) { return exp(a); }
+  __device__ static double erfc_(double x) { return erfc(x); }
+  __device__ static double erf_(double x) { return erf(x); }
+};
+template <>
+struct Tr<YNB_MIXED> {
+  __device__ static double ex(double a) {
+    const double t = a * 1.4426950408889634074;  // log2(e)
+    const double k = rint(t);
+    return ldexp((double)exp2f((float)(t - k)), (int)k);
+  }
+  __device__ static double erfc_(double x) { return ex(-x * x) * (double)erfcxf((float)x); }
+  __device__ static double erf_(double x) { return 1.0 - erfc_(x); }
+};
+
+template <int PREC>
 struct PotEval {
+  using R = Tr<PREC>;
   // Returns the energy; g receives dE/dd / d (Yaff's vg).
-  __device__ static T lj(T sa, T sb, T ea, T eb, T d, T &g) {  // pair_pot.c:277-301
-    const T sigma = T(0.5) * (sa + sb);
-    const T eps = sqrt(ea * eb);
-    T x = sigma / d;
+  __device__ static double lj(double sa, double sb, double ea, double eb, double d, double &g) {  // pair_pot.c:277-301
+    const double sigma = 0.5 * (sa + sb);
+    const double eps = sqrt(ea * eb);
+    double x = sigma / d;
     x *= x;
     x *= x * x;
-    g = T(24.0) * eps / d / d * x * (T(1.0) - T(2.0) * x);
-    return T(4.0) * eps * (x * (x - T(1.0)));
+    g = 24.0 * eps / d / d * x * (1.0 - 2.0 * x);
+    return 4.0 * eps * (x * (x - 1.0));
   }
-  __device__ static T mm3(T sa, T sb, T ea, T eb, int pauli, T d, T &g) {  // pair_pot.c:321-352
-    const T sigma = sa + sb;
-    const T eps = sqrt(ea * eb);
-    T x = sigma / d;
-    const T e = T(1.84e5) * exp(T(-12.0) / x);
+  __device__ static double mm3(double sa, double sb, double ea, double eb, int pauli, double d,
+                               double &g) {  // pair_pot.c:321-352
+    const double sigma = sa + sb;
+    const double eps = sqrt(ea * eb);
+    double x = sigma / d;
+    const double e = 1.84e5 * R::ex(-12.0 / x);
     if (pauli != 0) {
-      g = eps / d * (T(-12.0) / sigma * e);
+      g = eps / d * (-12.0 / sigma * e);
       return eps * e;
     }
     x *= x;
-    x *= T(2.25) * x * x;
-    g = eps / d * (T(-12.0) / sigma * e + T(6.0) / d * x);
+    x *= 2.25 * x * x;
+    g = eps / d * (-12.0 / sigma * e + 6.0 / d * x);
     return eps * (e - x);
   }
-  __device__ static T exprep(T amp, T b, T d, T &g) {  // pair_pot.c:473-489
-    if (amp == T(0) || b == T(0)) {
-      g = T(0);
-      return T(0);
+  __device__ static double exprep(double amp, double b, double d, double &g) {  // pair_pot.c:473-489
+    if (amp == 0.0 || b == 0.0) {
+      g = 0.0;
+      return 0.0;
     }
-    const T e = amp * exp(-b * d);
+    const double e = amp * R::ex(-b * d);
     g = -e * b / d;
     return e;
   }
-  __device__ static T ei(T qa, T qb, T ra, T rb, T alpha, T diel, T d, T &g) {  // pair_pot.c:711-751
-    const T qq = (qa * qb) / diel;
-    const T rab = sqrt(ra * ra + rb * rb);
-    T e;
-    if (alpha > T(0)) {
-      const T x = alpha * d;
-      if (rab > T(0)) {
-        const T y = d / rab;
-        e = qq / d * (erfc(x) - erfc(y));
-        g = (T(TWO_DIV_SQRT_PI) * (exp(-y * y) / rab - exp(-x * x) * alpha) * qq - e) / (d * d);
+  __device__ static double ei(double qa, double qb, double ra, double rb, double alpha, double diel, double d,
+                              double &g) {  // pair_pot.c:711-751
+    const double qq = (qa * qb) / diel;
+    const double rab = sqrt(ra * ra + rb * rb);
+    double e;
+    if (alpha > 0.0) {
+      const double x = alpha * d;
+      if (rab > 0.0) {
+        const double y = d / rab;
+        e = qq / d * (R::erfc_(x) - R::erfc_(y));
+        g = (TWO_DIV_SQRT_PI * (R::ex(-y * y) / rab - R::ex(-x * x) * alpha) * qq - e) / (d * d);
       } else {
-        e = qq * erfc(x) / d;
-        g = (-T(TWO_DIV_SQRT_PI) * alpha * exp(-x * x) * qq - e) / (d * d);
+        e = qq * R::erfc_(x) / d;
+        g = (-TWO_DIV_SQRT_PI * alpha * R::ex(-x * x) * qq - e) / (d * d);
       }
     } else {
-      if (rab > T(0)) {
-        const T y = d / rab;
-        e = qq / d * erf(y);
-        g = (T(TWO_DIV_SQRT_PI) / rab * exp(-y * y) * qq - e) / (d * d);
+      if (rab > 0.0) {
+        const double y = d / rab;
+        e = qq / d * R::erf_(y);
+        g = (TWO_DIV_SQRT_PI / rab * R::ex(-y * y) * qq - e) / (d * d);
       } else {
         e = qq / d;
         g = -e / (d * d);
@@ -82,29 +108,29 @@ struct PotEval {
     }
     return e;
   }
-  __device__ static T switch3(T d, T rcut, T w, T &hg) {  // truncation.c:59-76
+  __device__ static double switch3(double d, double rcut, double w, double &hg) {  // truncation.c:59-76
     if (d < rcut) {
-      T x = rcut - d;
+      double x = rcut - d;
       if (x > w) {
-        hg = T(0);
-        return T(1);
+        hg = 0.0;
+        return 1.0;
       }
       x /= w;
-      hg = T(-6) * x * (T(1) - x) / w;
-      return (T(3) - T(2) * x) * x * x;
+      hg = -6 * x * (1 - x) / w;
+      return (3 - 2 * x) * x * x;
     }
-    hg = T(0);
-    return T(0);
+    hg = 0.0;
+    return 0.0;
   }
-  __device__ static T hammer(T d, T rcut, T tau, T &hg) {  // truncation.c:30-41
+  __device__ static double hammer(double d, double rcut, double tau, double &hg) {  // truncation.c:30-41
     if (d < rcut) {
-      const T x = d - rcut;
-      const T h = exp(tau / x);
+      const double x = d - rcut;
+      const double h = R::ex(tau / x);
       hg = -h * tau / x / x;
       return h;
     }
-    hg = T(0);
-    return T(0);
+    hg = 0.0;
+    return 0.0;
   }
 };
 
@@ -123,22 +149,46 @@ __device__ inline double stab_scale(const ynb_scaling_row *st, int n, int64_t a,
   return 1.0;
 }
 
+constexpr int G = 16;  // lanes per atom
+
+template <typename T>
+__device__ inline T group_sum(T v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+
+// G lanes per atom i walk the atom's row (entries s, s+G, ...); both
+// directions of every pair are in the list, so each lane accumulates the
+// gradient of its own atom and energy/virial get a factor 1/2.
+//
+// Image vectors: u = pos_j - pos_i + n.R in FP64.  Where Yaff's bits matter
+// -- a distance within a relative 1e-12 of a cutoff, or the image index of a
+// scaled (bonded) pair -- the vector is rebuilt exactly as nlist_build_low
+// does (yaff_row_vector), so the cutoff tests d < rcut and the scaling
+// lookups agree with the reference.  FP64 mode evaluates the potentials in
+// FP64; mixed mode in FP32, accumulating gradient, energy and virial in FP64.
 template <int PREC, bool SLOTS>
 __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
                                               ynb_pairpar pp, const double *pos, ynb_nlist nl, uint32_t mask,
                                               double *gpos, int accumulate, double *gpos_slots,
                                               double *partial) {
-  using T = typename std::conditional<PREC == YNB_FP64, double, float>::type;
-  const int64_t gi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t wave = gi >> 6;
+  using T = double;
+  using PE = PotEval<PREC>;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t gi = t / G;
+  const int s = (int)(t % G);
+  const int64_t wave = t >> 6;
   const int lane = threadIdx.x & 63;
-  if (wave >= npad / 64) return;  // npad is a multiple of 64: whole waves leave
-  const int sid = uniform_int(wave_sys[wave]);
+  if (gi >= npad) return;  // whole waves: npad * G is a multiple of 64
+  const int sid = uniform_int(wave_sys[gi >> 6]);
   if (sid < 0) return;
   const ynb_sys &S = sys[sid];
   const int li = (int)(gi - S.atom0);
   const bool active = li < S.natom;
   const int npot = S.npot;
+  double rcmax = 0.0;
+  for (int p = 0; p < npot; ++p) rcmax = fmax(rcmax, S.pot[p].rcut);
 
   double acc_e[YNB_MAX_POT], acc_v[YNB_MAX_POT][6], acc_g[SLOTS ? YNB_MAX_POT : 1][3];
   for (int p = 0; p < YNB_MAX_POT; ++p) {
@@ -155,92 +205,85 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
       par_i[p][1] = (p < npot) ? pp.par[((int64_t)p * 3 + 1) * npad + gi] : 0.0;
     }
     const int n = min(nl.cnt[gi], nl.kcap);
-    for (int k = 0; k < n; ++k) {
-      const int64_t gj = nl.j[(int64_t)k * npad + gi];
+    const int32_t *rowj = nl.j + gi * nl.kcap;
+    const int32_t *rowi = nl.img + gi * nl.kcap;
+    for (int k = s; k < n; k += G) {
+      const int64_t gj = rowj[k];
       int nn[3];
-      unpack_img(nl.img[(int64_t)k * npad + gi], nn);
+      unpack_img(rowi[k], nn);
       const int lj = (int)(gj - S.atom0);
       const bool self = (gj == gi);
       const double pj[3] = {pos[3 * gj], pos[3 * gj + 1], pos[3 * gj + 2]};
-      double u[3];  // from atom i to the partner image
+      double u[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+      for (int a = 0; a < 3; ++a) {
+        const double na = (double)nn[a];
+        u[0] = fma(na, S.rvecs[3 * a], u[0]);
+        u[1] = fma(na, S.rvecs[3 * a + 1], u[1]);
+        u[2] = fma(na, S.rvecs[3 * a + 2], u[2]);
+      }
+      double dd = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      bool near_cut = false;
+      for (int p = 0; p < npot; ++p) near_cut |= fabs(dd - S.pot[p].rcut) <= 1e-12 * S.pot[p].rcut;
       bool r_zero = false;
-      int A = li > lj ? li : lj, B = li > lj ? lj : li;
-      if (PREC == YNB_FP64) {
+      const int A = li > lj ? li : lj, B = li > lj ? lj : li;
+      if (near_cut || (S.any_stab && !self)) {
         const bool i_is_a = (li == A);
         const int nab[3] = {i_is_a ? nn[0] : -nn[0], i_is_a ? nn[1] : -nn[1], i_is_a ? nn[2] : -nn[2]};
         double v[3];
         int r[3];
         yaff_row_vector(i_is_a ? pi : pj, i_is_a ? pj : pi, self, nab, S.rvecs, S.gvecs, S.nvec, S.rmax, v, r);
         r_zero = !self && r[0] == 0 && r[1] == 0 && r[2] == 0;
-        u[0] = i_is_a ? v[0] : -v[0];
-        u[1] = i_is_a ? v[1] : -v[1];
-        u[2] = i_is_a ? v[2] : -v[2];
-      } else {
-        if (S.any_stab && !self) {
-          // Scaling applies to r == 0 rows only: r = n_ab + mic shift.
-          const bool i_is_a = (li == A);
-          const int nab[3] = {i_is_a ? nn[0] : -nn[0], i_is_a ? nn[1] : -nn[1], i_is_a ? nn[2] : -nn[2]};
-          double v[3];
-          int r[3];
-          yaff_row_vector(i_is_a ? pi : pj, i_is_a ? pj : pi, self, nab, S.rvecs, S.gvecs, S.nvec, S.rmax, v, r);
-          r_zero = r[0] == 0 && r[1] == 0 && r[2] == 0;
-        }
-        u[0] = pj[0] - pi[0];
-        u[1] = pj[1] - pi[1];
-        u[2] = pj[2] - pi[2];
-        for (int a = 0; a < 3; ++a) {
-          const double na = (double)nn[a];
-          u[0] = fma(na, S.rvecs[3 * a], u[0]);
-          u[1] = fma(na, S.rvecs[3 * a + 1], u[1]);
-          u[2] = fma(na, S.rvecs[3 * a + 2], u[2]);
+        if (near_cut) {
+          u[0] = i_is_a ? v[0] : -v[0];
+          u[1] = i_is_a ? v[1] : -v[1];
+          u[2] = i_is_a ? v[2] : -v[2];
+          dd = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         }
       }
-      const double d2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-      const double dd = sqrt(d2);
+      if (!(dd < rcmax)) continue;
       const T d = (T)dd;
       for (int p = 0; p < YNB_MAX_POT; ++p) {
         if (p >= npot || !((mask >> p) & 1u)) continue;
         const ynb_pot &P = S.pot[p];
         if (!(dd < P.rcut)) continue;
-        double s = 1.0;
-        if (P.nstab > 0 && r_zero) s = stab_scale(pp.stab + P.stab0, P.nstab, A, B);
-        if (s == 0.0) continue;
-        const int64_t gjj = gj;
+        double sc = 1.0;
+        if (P.nstab > 0 && r_zero) sc = stab_scale(pp.stab + P.stab0, P.nstab, A, B);
+        if (sc == 0.0) continue;
         T g = T(0), e;
         switch (P.kind) {
           case YNB_POT_LJ:
-            e = PotEval<T>::lj((T)par_i[p][0], (T)pp.par[((int64_t)p * 3) * npad + gjj], (T)par_i[p][1],
-                               (T)pp.par[((int64_t)p * 3 + 1) * npad + gjj], d, g);
+            e = PE::lj((T)par_i[p][0], (T)pp.par[((int64_t)p * 3) * npad + gj], (T)par_i[p][1],
+                               (T)pp.par[((int64_t)p * 3 + 1) * npad + gj], d, g);
             break;
           case YNB_POT_MM3: {
             const int pa = (int)pp.par[((int64_t)p * 3 + 2) * npad + gi];
-            const int pb = (int)pp.par[((int64_t)p * 3 + 2) * npad + gjj];
-            e = PotEval<T>::mm3((T)par_i[p][0], (T)pp.par[((int64_t)p * 3) * npad + gjj], (T)par_i[p][1],
-                                (T)pp.par[((int64_t)p * 3 + 1) * npad + gjj], pa + pb, d, g);
+            const int pb = (int)pp.par[((int64_t)p * 3 + 2) * npad + gj];
+            e = PE::mm3((T)par_i[p][0], (T)pp.par[((int64_t)p * 3) * npad + gj], (T)par_i[p][1],
+                                (T)pp.par[((int64_t)p * 3 + 1) * npad + gj], pa + pb, d, g);
             break;
           }
           case YNB_POT_EXPREP: {
             const int ta = (int)par_i[p][0];
-            const int tb = (int)pp.par[((int64_t)p * 3) * npad + gjj];
+            const int tb = (int)pp.par[((int64_t)p * 3) * npad + gj];
             const int nf = P.nffatype;
             const int64_t ci = P.cross0 + (int64_t)(ta * nf + tb);  // tables are symmetric
-            e = PotEval<T>::exprep((T)pp.cross[ci], (T)pp.cross[ci + (int64_t)nf * nf], d, g);
+            e = PE::exprep((T)pp.cross[ci], (T)pp.cross[ci + (int64_t)nf * nf], d, g);
             break;
           }
           default:
-            e = PotEval<T>::ei((T)par_i[p][0], (T)pp.par[((int64_t)p * 3) * npad + gjj], (T)par_i[p][1],
-                               (T)pp.par[((int64_t)p * 3 + 1) * npad + gjj], (T)P.alpha, (T)P.dielectric, d, g);
+            e = PE::ei((T)par_i[p][0], (T)pp.par[((int64_t)p * 3) * npad + gj], (T)par_i[p][1],
+                               (T)pp.par[((int64_t)p * 3 + 1) * npad + gj], (T)P.alpha, (T)P.dielectric, d, g);
             break;
         }
         if (P.trunc != YNB_TR_NONE && (e != T(0) || g != T(0))) {  // pair_pot.c:130-139
           T hg;
-          const T h = (P.trunc == YNB_TR_SWITCH3) ? PotEval<T>::switch3(d, (T)P.rcut, (T)P.trunc_par, hg)
-                                                   : PotEval<T>::hammer(d, (T)P.rcut, (T)P.trunc_par, hg);
+          const T h = (P.trunc == YNB_TR_SWITCH3) ? PE::switch3(d, (T)P.rcut, (T)P.trunc_par, hg)
+                                                   : PE::hammer(d, (T)P.rcut, (T)P.trunc_par, hg);
           g = g * h + e * hg / d;
           e *= h;
         }
-        const double vg = (double)g * s;
-        acc_e[p] += s * (double)e;
+        const double vg = (double)g * sc;
+        acc_e[p] += sc * (double)e;
         const int gs = SLOTS ? p : 0;
         acc_g[gs][0] -= u[0] * vg;
         acc_g[gs][1] -= u[1] * vg;
@@ -253,15 +296,21 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
         acc_v[p][5] += u[1] * (u[2] * vg);
       }
     }
+  }
+  // Gradient: sum over the atom's G lanes; lane 0 of the group stores.
+  double G3[3] = {0.0, 0.0, 0.0};
+  for (int p = 0; p < (SLOTS ? YNB_MAX_POT : 1); ++p)
+    for (int c = 0; c < 3; ++c) {
+      acc_g[p][c] = group_sum(acc_g[p][c]);
+      G3[c] += acc_g[p][c];
+    }
+  if (active && s == 0) {
     if (gpos) {
-      double G[3] = {0.0, 0.0, 0.0};
-      for (int p = 0; p < (SLOTS ? YNB_MAX_POT : 1); ++p)
-        for (int c = 0; c < 3; ++c) G[c] += acc_g[p][c];
       for (int c = 0; c < 3; ++c) {
         if (accumulate)
-          gpos[3 * gi + c] += G[c];
+          gpos[3 * gi + c] += G3[c];
         else
-          gpos[3 * gi + c] = G[c];
+          gpos[3 * gi + c] = G3[c];
       }
     }
     if (SLOTS && gpos_slots) {
@@ -296,7 +345,7 @@ __global__ void k_pair_reduce(const ynb_sys *sys, const double *partial, double 
     const int p = t / 8;
     double acc = 0.0;
     if (p < S.npot) {
-      const int64_t w0 = S.atom0 / 64, w1 = (S.atom0 + S.natom + 63) / 64;
+      const int64_t w0 = (int64_t)S.atom0 * G / 64, w1 = ((int64_t)(S.atom0 + S.natom) * G + 63) / 64;
       for (int64_t w = w0; w < w1; ++w) acc += partial[(w * YNB_MAX_POT) * 8 + t];
     }
     red[t] = 0.5 * acc;
@@ -311,7 +360,6 @@ __global__ void k_pair_reduce(const ynb_sys *sys, const double *partial, double 
     o[7] = r[5]; o[8] = r[6]; o[9] = r[3];
   }
 }
-
 
 // pair_pot_compute over explicit Yaff rows (yaff/pes/pair_pot.c:84-176):
 // one thread per row, rows in any order.  Gradients are scattered with FP64
@@ -331,28 +379,28 @@ __global__ __launch_bounds__(256) void k_pair_rows(const ynb_neigh_row *rows, in
         double g = 0.0, pe;
         const double d = w.d;
         switch (P.kind) {
-          case YNB_POT_LJ: pe = PotEval<double>::lj(p0[w.a], p0[w.b], p1[w.a], p1[w.b], d, g); break;
+          case YNB_POT_LJ: pe = PotEval<YNB_FP64>::lj(p0[w.a], p0[w.b], p1[w.a], p1[w.b], d, g); break;
           case YNB_POT_MM3:
-            pe = PotEval<double>::mm3(p0[w.a], p0[w.b], p1[w.a], p1[w.b], (int)p2[w.a] + (int)p2[w.b], d, g);
+            pe = PotEval<YNB_FP64>::mm3(p0[w.a], p0[w.b], p1[w.a], p1[w.b], (int)p2[w.a] + (int)p2[w.b], d, g);
             break;
           case YNB_POT_EXPREP: {
             const int nf = P.nffatype;
             const int64_t ci = P.cross0 + (int64_t)((int)p0[w.a] * nf + (int)p0[w.b]);
-            pe = PotEval<double>::exprep(cross[ci], cross[ci + (int64_t)nf * nf], d, g);
+            pe = PotEval<YNB_FP64>::exprep(cross[ci], cross[ci + (int64_t)nf * nf], d, g);
             break;
           }
-          default: pe = PotEval<double>::ei(p0[w.a], p0[w.b], p1[w.a], p1[w.b], P.alpha, P.dielectric, d, g); break;
+          default: pe = PotEval<YNB_FP64>::ei(p0[w.a], p0[w.b], p1[w.a], p1[w.b], P.alpha, P.dielectric, d, g); break;
         }
         if (gpos == nullptr) {
           if (P.trunc != YNB_TR_NONE && pe != 0.0) {
             double hg;
-            pe *= (P.trunc == YNB_TR_SWITCH3) ? PotEval<double>::switch3(d, P.rcut, P.trunc_par, hg)
-                                              : PotEval<double>::hammer(d, P.rcut, P.trunc_par, hg);
+            pe *= (P.trunc == YNB_TR_SWITCH3) ? PotEval<YNB_FP64>::switch3(d, P.rcut, P.trunc_par, hg)
+                                              : PotEval<YNB_FP64>::hammer(d, P.rcut, P.trunc_par, hg);
           }
         } else if (P.trunc != YNB_TR_NONE && (pe != 0.0 || g != 0.0)) {
           double hg;
-          const double h = (P.trunc == YNB_TR_SWITCH3) ? PotEval<double>::switch3(d, P.rcut, P.trunc_par, hg)
-                                                        : PotEval<double>::hammer(d, P.rcut, P.trunc_par, hg);
+          const double h = (P.trunc == YNB_TR_SWITCH3) ? PotEval<YNB_FP64>::switch3(d, P.rcut, P.trunc_par, hg)
+                                                        : PotEval<YNB_FP64>::hammer(d, P.rcut, P.trunc_par, hg);
           g = g * h + pe * hg / d;
           pe *= h;
         }
@@ -418,7 +466,7 @@ extern "C" int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const
     return YNB_EINVAL;
   }
   hipStream_t st = (hipStream_t)stream;
-  const dim3 grid((unsigned)ceil_div(b->npad, 256)), block(256);
+  const dim3 grid((unsigned)ceil_div(b->npad * G, 256)), block(256);
   const bool slots = gpos_slots != nullptr;
   if (precision == YNB_FP64) {
     if (slots)

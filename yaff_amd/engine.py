@@ -313,7 +313,7 @@ class Engine(object):
         self.wave_sys_d = self._t(self.wave_sys)
         self.bin_sys_d = self._t(self.bin_sys)
         self.batch = YnbBatch(self.sys_d.data_ptr(), self.wave_sys_d.data_ptr(), self.bin_sys_d.data_ptr(),
-                              self.nsys, self.nbin, self.npad, 0, 0)
+                              self.nsys, self.nbin, self.npad, self._max_bin_atoms(), 0)
         self.pos_d = torch.zeros((self.npad, 3), dtype=torch.float64, device=dev)
         self.gpos_d = torch.zeros((self.npad, 3), dtype=torch.float64, device=dev)
         self.par_d = self._t(self.par_h)
@@ -326,7 +326,7 @@ class Engine(object):
         self.cnt_d = torch.zeros(self.npad, dtype=torch.int32, device=dev)
         self.status_d = torch.zeros(4, dtype=torch.int32, device=dev)
         self._alloc_nlist(kcap if kcap is not None else self._estimate_kcap())
-        self.partial_d = torch.zeros((self.npad//64)*MAX_POT*8, dtype=torch.float64, device=dev)
+        self.partial_d = torch.zeros((self.npad//4)*MAX_POT*8, dtype=torch.float64, device=dev)
         self.pair_out_d = torch.zeros((self.nsys, MAX_POT, 10), dtype=torch.float64, device=dev)
         self.ewald_out_d = torch.zeros((self.nsys, 10), dtype=torch.float64, device=dev)
         self.total_d = torch.zeros((self.nsys, 10), dtype=torch.float64, device=dev)
@@ -363,6 +363,14 @@ class Engine(object):
         self.dt_d = torch.zeros(self.nsys, dtype=torch.float64, device=dev)
         self.ekin2_d = torch.zeros(self.nsys, dtype=torch.float64, device=dev)
         self.md_partial_d = torch.zeros(self.npad//64, dtype=torch.float64, device=dev)
+
+    def _max_bin_atoms(self):
+        """Sizes the build grid: twice the largest mean bin occupancy."""
+        m = 16
+        for i, s in enumerate(self.systems):
+            if self.hsys[i].nbin > 0:
+                m = max(m, int(2*s.natom/self.hsys[i].nbin) + 16)
+        return m
 
     def _estimate_kcap(self):
         est = 16
