@@ -99,6 +99,7 @@ typedef struct ynb_sys {
   int32_t seg0, nseg;     /* segments of <= seglen k vectors of one row */
   int32_t chunk0, nchunk; /* atom chunks of the structure-factor pass */
   int64_t spart0;         /* offset of this system's partial structure factors, in k entries */
+  int32_t nchunk_low, nchunk_high; /* chunks [0, low) below nlow, the last `high` above nhigh */
 } ynb_sys;
 
 /* Device tables shared by all launches over one batch. */
@@ -109,7 +110,7 @@ typedef struct ynb_batch {
   int32_t nsys;
   int32_t nbin;              /* total bins (0 when every system uses the all-pairs path) */
   int64_t npad;              /* padded atom count, multiple of 64 */
-  int32_t max_bin_atoms;     /* upper bound of atoms per bin (sizes the build grid; 0: default) */
+  int32_t max_bin_atoms;     /* capacity of a bin, 1..1024 (ynb_nlist_build flags overflow) */
   int32_t pad_;
 } ynb_batch;
 
@@ -121,7 +122,9 @@ typedef struct ynb_nlist {
   int32_t *j;
   int32_t *img;
   int32_t *cnt;        /* [npad] entries per atom (also counted past kcap) */
-  int32_t *status;     /* [4]: [0] overflow flag, [1] max count */
+  int32_t *status;     /* [4]: [0] flags: 1 row > kcap, 2 image shift > 511 cells, 4 bin > max_bin_atoms.
+                          On a flag the caller reads cnt (rows) or the bin counts
+                          (ynb_nlist_bin_counts) to size the retry. */
   int32_t kcap;
   int32_t pad_;
 } ynb_nlist;
@@ -132,6 +135,8 @@ typedef struct ynb_pairpar {
                           EI charge, radius; ExpRep ffatype id (as double) */
   const double *cross; /* ExpRep amp/b tables, row-major nffatype^2 each */
   const ynb_scaling_row *stab; /* scaling rows (a > b, local indices), sorted per system */
+  int32_t lanes;       /* lanes per atom in ynb_pair_compute: 16, 32 or 64 (0: 16) */
+  int32_t pad_;
 } ynb_pairpar;
 
 /* Ewald reciprocal-space tables and work buffers (all device). */
@@ -144,6 +149,7 @@ typedef struct ynb_ewald {
   const int32_t *chunk;   /* [nchunk_tot][4]: system, first local atom, atom count, group 0 mid/1 low/2 high */
   const int32_t *work;    /* [nwork][4]: system, first global segment, global chunk, segments (<= 64) */
   const int32_t *kblk_sys;/* [nk_tot/256]: system of each block of 256 k vectors */
+  const int32_t *kg;      /* [nk_tot][4]: g0, g1, g2 and 1 when the k vector starts a row */
   int32_t nwork;
   int32_t nk_tot;
   int32_t seglen;         /* 4, 8 or 16 */
@@ -175,8 +181,9 @@ int ynb_profile_read(int i, const char **name, float *ms);
  * (nbin = 0). */
 int ynb_sys_plan(ynb_sys *sys);
 
-/* Workspace of ynb_nlist_build in bytes. */
-size_t ynb_nlist_workspace_bytes(int64_t npad, int32_t nbin);
+/* Workspace of ynb_nlist_build in bytes, for bins of bin_cap atoms
+ * (ynb_batch.max_bin_atoms). */
+size_t ynb_nlist_workspace_bytes(int64_t npad, int32_t nbin, int32_t bin_cap);
 
 /* Workspace of ynb_nlist_rows for a system of natom atoms and nrows rows. */
 size_t ynb_rows_workspace_bytes(int64_t natom, int64_t nrows);
@@ -184,9 +191,16 @@ size_t ynb_rows_workspace_bytes(int64_t natom, int64_t nrows);
 /* Cell-binned (or all-pairs) neighbour list of every system: the full list of
  * partners j of each atom i (both directions, periodic self images included)
  * whose Yaff half row -- built exactly as nlist_build_low builds it -- has
- * d^2 < rc^2.  Sets status[0] when an atom has more than kcap entries. */
+ * d^2 < rc^2.  Overflows are flagged in nl->status (see ynb_nlist): the
+ * caller grows kcap or max_bin_atoms and builds again.  Uses no memset, no
+ * library sort and no synchronisation, so it can be captured in a graph. */
 int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t work_bytes,
                     const ynb_nlist *nl, ynb_stream stream);
+
+/* Copies the atom count of every bin of the last build (nbin int32) from the
+ * workspace to `counts` (device); used to size max_bin_atoms after a bin
+ * overflow. */
+int ynb_nlist_bin_counts(const ynb_batch *b, const void *work, int32_t *counts, ynb_stream stream);
 
 /* Materialises system `s`'s half list as Yaff rows (local atom indices) in
  * nlist_build_low's order.  Synchronises.  Writes at most `cap` rows to
@@ -203,7 +217,7 @@ int ynb_nlist_recompute(const ynb_sys *h_sys, const double *pos, const double *p
  * gpos (nullable): [npad][3] gradient summed over the slots, written or, with
  * accumulate != 0, added.  gpos_slots (nullable): [YNB_MAX_POT][npad][3].
  * out: [nsys][YNB_MAX_POT][10] energy followed by the 3x3 virial.
- * partial: work of npad/4 * YNB_MAX_POT * 8 doubles (16 lanes per atom). */
+ * partial: work of npad*lanes/256 * YNB_MAX_POT * 8 doubles. */
 int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const double *pos,
                      const ynb_nlist *nl, uint32_t slot_mask, int32_t precision,
                      double *gpos, int32_t accumulate, double *gpos_slots,

@@ -36,7 +36,8 @@ class YnbSys(C.Structure):
                 ('any_stab', C.c_int32), ('box_safe', C.c_int32), ('pot', YnbPot * MAX_POT),
                 ('alpha', C.c_double), ('dielectric', C.c_double), ('k0', C.c_int32), ('nk', C.c_int32),
                 ('row0', C.c_int32), ('nrow', C.c_int32), ('seg0', C.c_int32), ('nseg', C.c_int32),
-                ('chunk0', C.c_int32), ('nchunk', C.c_int32), ('spart0', C.c_int64)]
+                ('chunk0', C.c_int32), ('nchunk', C.c_int32), ('spart0', C.c_int64), ('nchunk_low', C.c_int32),
+                ('nchunk_high', C.c_int32)]
 
 
 class YnbBatch(C.Structure):
@@ -50,12 +51,14 @@ class YnbNlist(C.Structure):
 
 
 class YnbPairpar(C.Structure):
-    _fields_ = [('par', C.c_void_p), ('cross', C.c_void_p), ('stab', C.c_void_p)]
+    _fields_ = [('par', C.c_void_p), ('cross', C.c_void_p), ('stab', C.c_void_p), ('lanes', C.c_int32),
+                ('pad_', C.c_int32)]
 
 
 class YnbEwald(C.Structure):
     _fields_ = [('kvec', C.c_void_p), ('krow', C.c_void_p), ('krow_k', C.c_void_p), ('seg', C.c_void_p),
                 ('seg_len', C.c_void_p), ('chunk', C.c_void_p), ('work', C.c_void_p), ('kblk_sys', C.c_void_p),
+                ('kg', C.c_void_p),
                 ('nwork', C.c_int32), ('nk_tot', C.c_int32), ('seglen', C.c_int32), ('kchunks', C.c_int32),
                 ('groups', C.c_int32), ('pad_', C.c_int32), ('frac', C.c_void_p), ('step', C.c_void_p),
                 ('spart', C.c_void_p), ('coef', C.c_void_p), ('kpart', C.c_void_p), ('gpart', C.c_void_p)]
@@ -64,7 +67,7 @@ class YnbEwald(C.Structure):
 EXPORTS = ['ynb_version', 'ynb_last_error', 'ynb_sys_plan', 'ynb_nlist_workspace_bytes',
            'ynb_rows_workspace_bytes', 'ynb_nlist_build', 'ynb_nlist_rows', 'ynb_nlist_recompute',
            'ynb_pair_compute', 'ynb_pair_rows', 'ynb_ewald_reci', 'ynb_ewald_corr', 'ynb_verlet_kick_drift', 'ynb_verlet_kick',
-           'ynb_total', 'ynb_profile_enable', 'ynb_profile_count', 'ynb_profile_read']
+           'ynb_total', 'ynb_profile_enable', 'ynb_profile_count', 'ynb_profile_read', 'ynb_nlist_bin_counts']
 
 
 class YnbError(RuntimeError):
@@ -87,10 +90,11 @@ def load(path=LIB_PATH):
     L.ynb_last_error.restype = C.c_char_p
     L.ynb_sys_plan.argtypes = [P(YnbSys)]
     L.ynb_nlist_workspace_bytes.restype = sz
-    L.ynb_nlist_workspace_bytes.argtypes = [i64, i32]
+    L.ynb_nlist_workspace_bytes.argtypes = [i64, i32, i32]
     L.ynb_rows_workspace_bytes.restype = sz
     L.ynb_rows_workspace_bytes.argtypes = [i64, i64]
     L.ynb_nlist_build.argtypes = [P(YnbBatch), vp, vp, sz, P(YnbNlist), vp]
+    L.ynb_nlist_bin_counts.argtypes = [P(YnbBatch), vp, vp, vp]
     L.ynb_nlist_rows.argtypes = [P(YnbBatch), i32, vp, P(YnbNlist), vp, sz, vp, i64, P(i64), vp]
     L.ynb_nlist_recompute.argtypes = [P(YnbSys), vp, vp, vp, i64, vp]
     L.ynb_pair_compute.argtypes = [P(YnbBatch), P(YnbPairpar), vp, P(YnbNlist), C.c_uint32, i32, vp, i32, vp,

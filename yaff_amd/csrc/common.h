@@ -138,6 +138,19 @@ __device__ inline T wave_sum(T v) {
 
 __device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+__device__ inline int wave_max(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Status counters written by many threads: one atomic per wavefront (device
+// atomics on one address from every lane serialise across the chip).
+__device__ inline void wave_status_max(int32_t *addr, int v) {
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(addr, v);
+}
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace ynb

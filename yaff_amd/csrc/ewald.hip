@@ -136,62 +136,107 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *sys, ynb_ewald e
   }
 }
 
+// Block of 32 k vectors x 8 chunk parts: part p sums the chunks p, p+8, ...
+// of each group (4 independent accumulators), then part sums are added in a
+// fixed order; thread (k, part 0) forms the energy/virial terms of k and the
+// coefficients; a fixed LDS tree sums energy and virial over the block.
 __global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_ewald ew) {
-  const int blk = blockIdx.x;
   const int t = threadIdx.x;
-  const int k = blk * 256 + t;
-  const int sid = ew.kblk_sys[blk];
+  const int kl = t & 31, part = t >> 5;
+  const int k = blockIdx.x * 32 + kl;
+  const int sid = ew.kblk_sys[k >> 8];
   const ynb_sys &S = sys[sid];
   const int lk = k - S.k0;
   const double4 kv = *(const double4 *)(ew.kvec + 4 * (int64_t)k);
-  double e = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
-  double cf[6] = {0, 0, 0, 0, 0, 0};
-  if (kv.w != 0.0) {
-    double sc[3] = {0, 0, 0}, ss[3] = {0, 0, 0};
-    for (int c = 0; c < S.nchunk; ++c) {
-      const int grp = ew.chunk[4 * (S.chunk0 + c) + 3];
-      const double *p = ew.spart + 2 * (S.spart0 + (int64_t)c * S.nk + lk);
-      sc[grp] += p[0];
-      ss[grp] += p[1];
+  __shared__ double l_s[6][8][32];
+  // chunk groups: [0, lo) low (1), [lo, nchunk - hi) mid (0), [nchunk - hi, nchunk) high (2)
+  const int lo = S.nchunk_low, hi = S.nchunk_high, nmid_end = S.nchunk - hi;
+  const int gb[3] = {lo, 0, nmid_end}, ge[3] = {nmid_end, lo, S.nchunk};
+#pragma unroll
+  for (int grp = 0; grp < 3; ++grp) {
+    double ac[4] = {0, 0, 0, 0}, as[4] = {0, 0, 0, 0};
+    if (kv.w != 0.0) {
+      for (int c = gb[grp] + part; c < ge[grp]; c += 32) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int cu = c + 8 * u;
+          if (cu < ge[grp]) {
+            const double2 v = *(const double2 *)(ew.spart + 2 * (S.spart0 + (int64_t)cu * S.nk + lk));
+            ac[u] += v.x;
+            as[u] += v.y;
+          }
+        }
+      }
     }
-    // ewald.c:82-95: S0 (mid), S1 (low), S2 (high).
-    double s = sc[0] * (sc[0] + 2.0 * (sc[1] + sc[2])) + ss[0] * (ss[0] + 2.0 * (ss[1] + ss[2]));
-    s += 2.0 * (sc[1] * sc[2] + ss[1] * ss[2]);
-    e = kv.w * s;
-    const double x = 2.0 * kv.w;
-    cf[0] = x * sc[0]; cf[1] = x * ss[0];
-    cf[2] = x * sc[1]; cf[3] = x * ss[1];
-    cf[4] = x * sc[2]; cf[5] = x * ss[2];
-    // ewald.c:113-127
-    const double ksq = kv.x * kv.x + kv.y * kv.y + kv.z * kv.z;
-    const double fac2 = 0.25 / S.alpha / S.alpha;
-    const double c = kv.w * (2.0 * (1.0 / ksq + fac2) * s);
-    v[0] = c * kv.x * kv.x;
-    v[1] = c * kv.y * kv.y;
-    v[2] = c * kv.z * kv.z;
-    v[3] = c * kv.y * kv.x;
-    v[4] = c * kv.z * kv.x;
-    v[5] = c * kv.z * kv.y;
+    l_s[2 * grp][part][kl] = (ac[0] + ac[1]) + (ac[2] + ac[3]);
+    l_s[2 * grp + 1][part][kl] = (as[0] + as[1]) + (as[2] + as[3]);
   }
-  double *o = ew.coef + 6 * (int64_t)k;
-  for (int c = 0; c < 6; ++c) o[c] = cf[c];
-  // Block sum in a fixed tree.
-  __shared__ double red[7][256];
-  red[0][t] = e;
-  for (int c = 0; c < 6; ++c) red[1 + c][t] = v[c];
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (t < off)
-      for (int c = 0; c < 7; ++c) red[c][t] += red[c][t + off];
+  double e = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
+  if (part == 0) {
+    double sc[3], ss[3];
+#pragma unroll
+    for (int grp = 0; grp < 3; ++grp) {
+      double a = 0.0, b = 0.0;
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        a += l_s[2 * grp][p][kl];
+        b += l_s[2 * grp + 1][p][kl];
+      }
+      sc[grp] = a;
+      ss[grp] = b;
+    }
+    double cf[6] = {0, 0, 0, 0, 0, 0};
+    if (kv.w != 0.0) {
+      // ewald.c:82-95: S0 (mid), S1 (low), S2 (high).
+      double s = sc[0] * (sc[0] + 2.0 * (sc[1] + sc[2])) + ss[0] * (ss[0] + 2.0 * (ss[1] + ss[2]));
+      s += 2.0 * (sc[1] * sc[2] + ss[1] * ss[2]);
+      e = kv.w * s;
+      const double x = 2.0 * kv.w;
+      cf[0] = x * sc[0]; cf[1] = x * ss[0];
+      cf[2] = x * sc[1]; cf[3] = x * ss[1];
+      cf[4] = x * sc[2]; cf[5] = x * ss[2];
+      // ewald.c:113-127
+      const double ksq = kv.x * kv.x + kv.y * kv.y + kv.z * kv.z;
+      const double fac2 = 0.25 / S.alpha / S.alpha;
+      const double c = kv.w * (2.0 * (1.0 / ksq + fac2) * s);
+      v[0] = c * kv.x * kv.x;
+      v[1] = c * kv.y * kv.y;
+      v[2] = c * kv.z * kv.z;
+      v[3] = c * kv.y * kv.x;
+      v[4] = c * kv.z * kv.x;
+      v[5] = c * kv.z * kv.y;
+    }
+    double *o = ew.coef + 6 * (int64_t)k;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) o[c] = cf[c];
+  }
+  __syncthreads();
+  __shared__ double red[7][32];
+  if (part == 0) {
+    red[0][kl] = e;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) red[1 + c][kl] = v[c];
+  }
+  __syncthreads();
+  for (int off = 16; off > 0; off >>= 1) {
+    if (part == 0 && kl < off)
+      for (int c = 0; c < 7; ++c) red[c][kl] += red[c][kl + off];
     __syncthreads();
   }
-  if (t < 7) ew.kpart[8 * blk + t] = red[t][0];
+  if (t < 7) ew.kpart[8 * (int64_t)blockIdx.x + t] = red[t][0];
 }
 
+// One lane per atom; blockIdx.y selects a contiguous part of the system's
+// k rows.  The part's k data (k vector, coefficients, g and a row-start flag)
+// is staged through LDS 64 k at a time and read as broadcasts; at a row start
+// the lane seeds exp(i k.r) with one sincospi, then steps it along g0.
 template <int PREC, bool GROUPS>
 __global__ __launch_bounds__(64) void k_ewald_force(const ynb_sys *sys, const int32_t *wave_sys, ynb_ewald ew,
                                                     int64_t npad) {
-  using T = double;
+  __shared__ double l_k[3][64];
+  __shared__ double l_c[6][64];
+  __shared__ int4 l_g[64];
   const int64_t wave = blockIdx.x;
   const int kc = blockIdx.y;
   const int lane = threadIdx.x;
@@ -204,48 +249,69 @@ __global__ __launch_bounds__(64) void k_ewald_force(const ynb_sys *sys, const in
   const bool active = li < S.natom;
   const double f0 = ew.frac[g], f1 = ew.frac[npad + g], f2 = ew.frac[2 * npad + g];
   const double q = ew.frac[3 * npad + g];
-  const T cr = (T)ew.step[g], ci = (T)ew.step[npad + g];
-  T wl = T(0), wh = T(0);
+  const double cr = ew.step[g], ci = ew.step[npad + g];
+  double wl = 0.0, wh = 0.0;
   if (GROUPS) {
-    wh = (li < S.nhigh) ? T(1) : T(0);
-    wl = (li >= S.nlow) ? T(1) : T(0);
+    wh = (li < S.nhigh) ? 1.0 : 0.0;
+    wl = (li >= S.nlow) ? 1.0 : 0.0;
   }
   const int rb = S.row0 + (int)(((int64_t)S.nrow * kc) / ew.kchunks);
   const int re = S.row0 + (int)(((int64_t)S.nrow * (kc + 1)) / ew.kchunks);
-  double G[3] = {0.0, 0.0, 0.0};
-  for (int row = rb; row < re; ++row) {
-    const int4 kr = *(const int4 *)(ew.krow + 4 * row);
-    const int kf = ew.krow_k[row];
-    T s, c;
-    seed_sincospi<PREC>(2.0 * reduced_phase(kr.z, kr.x, kr.y, f0, f1, f2), &s, &c);
-    T zr = c, zi = s;
-    T R0 = T(0), R1 = T(0), R2 = T(0);
-    for (int m = 0; m < kr.w; ++m) {
-      const double4 kv = *(const double4 *)(ew.kvec + 4 * (int64_t)(kf + m));
-      const double *cf = ew.coef + 6 * (int64_t)(kf + m);
-      T ac = (T)cf[0], as = (T)cf[1];
-      if (GROUPS) {
-        ac += wh * (T)cf[4] + wl * (T)cf[2];
-        as += wh * (T)cf[5] + wl * (T)cf[3];
+  double G0 = 0.0, G1 = 0.0, G2 = 0.0;
+  if (re > rb) {
+    const int kbeg = ew.krow_k[rb];
+    const int kend = ew.krow_k[re - 1] + ew.krow[4 * (re - 1) + 3];
+    double zr = 1.0, zi = 0.0;
+    for (int k0 = kbeg; k0 < kend; k0 += 64) {
+      const int kk = k0 + lane;
+      __syncthreads();
+      if (kk < kend) {
+        const double4 kv = *(const double4 *)(ew.kvec + 4 * (int64_t)kk);
+        l_k[0][lane] = kv.x;
+        l_k[1][lane] = kv.y;
+        l_k[2][lane] = kv.z;
+        const double *cf = ew.coef + 6 * (int64_t)kk;
+        l_c[0][lane] = cf[0];
+        l_c[1][lane] = cf[1];
+        if (GROUPS) {
+          l_c[2][lane] = cf[2];
+          l_c[3][lane] = cf[3];
+          l_c[4][lane] = cf[4];
+          l_c[5][lane] = cf[5];
+        }
+        l_g[lane] = *(const int4 *)(ew.kg + 4 * (int64_t)kk);
       }
-      // ewald.c:104-111: x = 2c (S_c (-q sin) + S_s (q cos)); q applied below.
-      const T x = as * zr - ac * zi;
-      R0 += (T)kv.x * x;
-      R1 += (T)kv.y * x;
-      R2 += (T)kv.z * x;
-      const T nr = zr * cr - zi * ci;
-      zi = zr * ci + zi * cr;
-      zr = nr;
+      __syncthreads();
+      const int nt = min(64, kend - k0);
+      for (int t = 0; t < nt; ++t) {
+        const int4 gg = l_g[t];
+        if (uniform_int(gg.w)) {  // row start (same k for the whole wave): seed exp(2 pi i g.f)
+          double s, c;
+          seed_sincospi<PREC>(2.0 * reduced_phase(gg.x, gg.y, gg.z, f0, f1, f2), &s, &c);
+          zr = c;
+          zi = s;
+        }
+        double ac = l_c[0][t], as = l_c[1][t];
+        if (GROUPS) {
+          ac += wh * l_c[4][t] + wl * l_c[2][t];
+          as += wh * l_c[5][t] + wl * l_c[3][t];
+        }
+        // ewald.c:104-111: x = 2c (S_c (-q sin) + S_s (q cos)); q applied below.
+        const double x = as * zr - ac * zi;
+        G0 = fma(l_k[0][t], x, G0);
+        G1 = fma(l_k[1][t], x, G1);
+        G2 = fma(l_k[2][t], x, G2);
+        const double nr = zr * cr - zi * ci;
+        zi = zr * ci + zi * cr;
+        zr = nr;
+      }
     }
-    G[0] += (double)R0;
-    G[1] += (double)R1;
-    G[2] += (double)R2;
   }
   const double scale = active ? q / S.dielectric : 0.0;
   double *o = ew.gpart + ((int64_t)kc * npad + g) * 3;
-  o[0] = G[0] * scale;
-  o[1] = G[1] * scale;
-  o[2] = G[2] * scale;
+  o[0] = G0 * scale;
+  o[1] = G1 * scale;
+  o[2] = G2 * scale;
 }
 
 __global__ void k_ewald_gsum(const ynb_sys *sys, const int32_t *wave_sys, ynb_ewald ew, int64_t npad, double *gpos,
@@ -273,7 +339,7 @@ __global__ void k_ewald_sysreduce(const ynb_sys *sys, int nsys, ynb_ewald ew, do
   if (s >= nsys) return;
   const ynb_sys &S = sys[s];
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int b = S.k0 / 256; b < (S.k0 + S.nk) / 256; ++b)
+  for (int b = S.k0 / 32; b < (S.k0 + S.nk) / 32; ++b)
     for (int c = 0; c < 7; ++c) acc[c] += ew.kpart[8 * b + c];
   const double e = acc[0];
   const double f = (S.nk > 0) ? 1.0 / S.dielectric : 0.0;
@@ -369,7 +435,7 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
     YNB_LAUNCHED();
   }
   if (ew->nk_tot > 0) {
-    YNB_LAUNCH("ewald_kreduce", k_ewald_kreduce, dim3(ew->nk_tot / 256), dim3(256), 0, st, b->sys, *ew);
+    YNB_LAUNCH("ewald_kreduce", k_ewald_kreduce, dim3(ew->nk_tot / 32), dim3(256), 0, st, b->sys, *ew);
     YNB_LAUNCHED();
   }
   const dim3 fgrid((unsigned)(npad / 64), (unsigned)ew->kchunks);

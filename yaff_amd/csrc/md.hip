@@ -85,13 +85,21 @@ __global__ void k_kick(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad
   if ((threadIdx.x & 63) == 0) partial[g >> 6] = e2;
 }
 
-__global__ void k_wave_to_sys(const ynb_sys *sys, int nsys, const double *partial, double *out) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsys) return;
-  const ynb_sys &S = sys[s];
+// One block per system: fixed strided sums over the system's waves, then a
+// fixed LDS tree.
+__global__ __launch_bounds__(256) void k_wave_to_sys(const ynb_sys *sys, const double *partial, double *out) {
+  const ynb_sys &S = sys[blockIdx.x];
+  const int t = threadIdx.x;
+  __shared__ double red[256];
   double acc = 0.0;
-  for (int64_t w = S.atom0 / 64; w < (S.atom0 + S.natom + 63) / 64; ++w) acc += partial[w];
-  out[s] = acc;
+  for (int64_t w = S.atom0 / 64 + t; w < (S.atom0 + S.natom + 63) / 64; w += 256) acc += partial[w];
+  red[t] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off) red[t] += red[t + off];
+    __syncthreads();
+  }
+  if (t == 0) out[blockIdx.x] = red[0];
 }
 
 __global__ void k_total(const ynb_sys *sys, int nsys, const double *pair_out, const double *ewald_out,
@@ -164,8 +172,7 @@ int ynb_verlet_kick(const ynb_batch *b, double *vel, const double *gpos, const d
   YNB_LAUNCH("verlet_kick", k_kick, dim3((unsigned)ceil_div(b->npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys,
              b->npad, vel, gpos, minv, masses, dt, partial);
   YNB_LAUNCHED();
-  hipLaunchKernelGGL(k_wave_to_sys, dim3((unsigned)ceil_div(b->nsys, 64)), dim3(64), 0, st, b->sys, b->nsys, partial,
-                     ekin2);
+  hipLaunchKernelGGL(k_wave_to_sys, dim3((unsigned)b->nsys), dim3(256), 0, st, b->sys, partial, ekin2);
   YNB_LAUNCHED();
   return YNB_OK;
 }

@@ -18,25 +18,17 @@ namespace ynb {
 namespace {
 
 struct Work {
-  int32_t *key, *key_sorted, *val, *val_sorted;
-  double4 *xw, *sxw;
-  int4 *wr, *sw;
+  int32_t *slot_idx;   // [nbin][cb] atoms of each bin in arrival order
+  int32_t *bin_count;  // [nbin]
+  double4 *xw, *sxw;   // wrapped coordinates: per atom, and per bin slot
+  int4 *wr, *sw;       // wrap counts (+ atom index in .w)
   int32_t *bin_start, *bin_end;
-  void *cub;
-  size_t cub_bytes;
   size_t total;
 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-size_t cub_sort_bytes(int64_t n) {
-  size_t bytes = 0;
-  hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (int32_t *)nullptr, (int32_t *)nullptr,
-                                     (int32_t *)nullptr, (int32_t *)nullptr, (int)n);
-  return bytes;
-}
-
-Work carve(void *base, int64_t npad, int32_t nbin) {
+Work carve(void *base, int64_t npad, int32_t nbin, int32_t cb) {
   Work w;
   char *p = (char *)base;
   size_t off = 0;
@@ -45,18 +37,16 @@ Work carve(void *base, int64_t npad, int32_t nbin) {
     off += align256(bytes);
     return q;
   };
-  w.key = (int32_t *)take(4 * npad);
-  w.key_sorted = (int32_t *)take(4 * npad);
-  w.val = (int32_t *)take(4 * npad);
-  w.val_sorted = (int32_t *)take(4 * npad);
+  const size_t nb = (size_t)(nbin > 0 ? nbin : 1);
+  const size_t slots = nb * (size_t)(cb > 0 ? cb : 1);
+  w.slot_idx = (int32_t *)take(4 * slots);
+  w.bin_count = (int32_t *)take(4 * nb);
   w.xw = (double4 *)take(32 * npad);
-  w.sxw = (double4 *)take(32 * npad);
   w.wr = (int4 *)take(16 * npad);
-  w.sw = (int4 *)take(16 * npad);
-  w.bin_start = (int32_t *)take(4 * (size_t)(nbin > 0 ? nbin : 1));
-  w.bin_end = (int32_t *)take(4 * (size_t)(nbin > 0 ? nbin : 1));
-  w.cub_bytes = cub_sort_bytes(npad);
-  w.cub = take(w.cub_bytes);
+  w.sxw = (double4 *)take(32 * slots);
+  w.sw = (int4 *)take(16 * slots);
+  w.bin_start = (int32_t *)take(4 * nb);
+  w.bin_end = (int32_t *)take(4 * nb);
   w.total = off;
   return w;
 }
@@ -68,51 +58,67 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
   if (i < n) p[i] = v;
 }
 
-// Fractional wrap and bin of every atom of the binned systems.
-__global__ void k_bin(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, const double *pos,
-                      int32_t *key, int32_t *val, double4 *xw, int4 *wr) {
+// Fractional wrap and bin of every atom of the binned systems; each atom
+// takes a slot of its bin (arrival order, made deterministic by k_bin_order).
+__global__ void k_bin(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, const double *pos, int32_t cb,
+                      int32_t *bin_count, int32_t *slot_idx, double4 *xw, int4 *wr, int32_t *status) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= npad) return;
-  val[g] = (int32_t)g;
-  key[g] = INT32_MAX;
-  const int sid = wave_sys[g >> 6];
+  if (g >= npad) return;  // whole waves: npad is a multiple of 64
+  const int sid = uniform_int(wave_sys[g >> 6]);
   if (sid < 0) return;
   const ynb_sys &S = sys[sid];
-  if (g - S.atom0 >= S.natom || S.nbin == 0) return;
-  const double p0 = pos[3 * g], p1 = pos[3 * g + 1], p2 = pos[3 * g + 2];
-  int c[3], w[3];
-  double x = p0, y = p1, z = p2;
-  for (int k = 0; k < 3; ++k) {
-    const double f = fma(S.gvecs[3 * k + 2], p2, fma(S.gvecs[3 * k + 1], p1, S.gvecs[3 * k] * p0));
-    const double fl = floor(f);
-    int ci = (int)((f - fl) * S.grid[k]);
-    ci = ci < 0 ? 0 : (ci >= S.grid[k] ? S.grid[k] - 1 : ci);
-    c[k] = ci;
-    w[k] = (int)fl;
-    x = fma(-fl, S.rvecs[3 * k], x);
-    y = fma(-fl, S.rvecs[3 * k + 1], y);
-    z = fma(-fl, S.rvecs[3 * k + 2], z);
+  int fill = 0;
+  if (g - S.atom0 < S.natom && S.nbin != 0) {
+    const double p0 = pos[3 * g], p1 = pos[3 * g + 1], p2 = pos[3 * g + 2];
+    int c[3], w[3];
+    double x = p0, y = p1, z = p2;
+    for (int k = 0; k < 3; ++k) {
+      const double f = fma(S.gvecs[3 * k + 2], p2, fma(S.gvecs[3 * k + 1], p1, S.gvecs[3 * k] * p0));
+      const double fl = floor(f);
+      int ci = (int)((f - fl) * S.grid[k]);
+      ci = ci < 0 ? 0 : (ci >= S.grid[k] ? S.grid[k] - 1 : ci);
+      c[k] = ci;
+      w[k] = (int)fl;
+      x = fma(-fl, S.rvecs[3 * k], x);
+      y = fma(-fl, S.rvecs[3 * k + 1], y);
+      z = fma(-fl, S.rvecs[3 * k + 2], z);
+    }
+    const int key = S.bin0 + (c[0] * S.grid[1] + c[1]) * S.grid[2] + c[2];
+    xw[g] = make_double4(x, y, z, 0.0);
+    wr[g] = make_int4(w[0], w[1], w[2], (int)g);
+    const int slot = atomicAdd(bin_count + key, 1);
+    if (slot < cb)
+      slot_idx[(int64_t)key * cb + slot] = (int32_t)g;
+    else
+      atomicOr(status, 4);
+    fill = slot + 1;
   }
-  key[g] = S.bin0 + (c[0] * S.grid[1] + c[1]) * S.grid[2] + c[2];
-  xw[g] = make_double4(x, y, z, 0.0);
-  wr[g] = make_int4(w[0], w[1], w[2], 0);
+  (void)fill;
 }
 
-// Bin ranges in the sorted order and sorted copies of the wrapped atoms.
-__global__ void k_binrange(int64_t npad, const int32_t *key_sorted, const int32_t *val_sorted,
-                           const double4 *xw, const int4 *wr, int32_t *bin_start, int32_t *bin_end,
-                           double4 *sxw, int4 *sw) {
-  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= npad) return;
-  const int32_t k = key_sorted[p];
-  if (k == INT32_MAX) return;
-  if (p == 0 || key_sorted[p - 1] != k) bin_start[k] = (int32_t)p;
-  if (p == npad - 1 || key_sorted[p + 1] != k) bin_end[k] = (int32_t)(p + 1);
-  const int32_t g = val_sorted[p];
-  sxw[p] = xw[g];
-  int4 w = wr[g];
-  w.w = g;
-  sw[p] = w;
+// One block per bin: orders the bin's atoms by index (rank among the bin's
+// atoms) and writes their wrapped data to the bin's slots, so the layout
+// does not depend on the order of the atomics in k_bin.
+__global__ __launch_bounds__(256) void k_bin_order(int32_t cb, const int32_t *bin_count, const int32_t *slot_idx,
+                                                   const double4 *xw, const int4 *wr, double4 *sxw, int4 *sw,
+                                                   int32_t *bin_start, int32_t *bin_end) {
+  __shared__ int32_t l_idx[1024];
+  const int b = blockIdx.x;
+  const int n = min(bin_count[b], cb);
+  for (int e = threadIdx.x; e < n; e += blockDim.x) l_idx[e] = slot_idx[(int64_t)b * cb + e];
+  __syncthreads();
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int g = l_idx[e];
+    int rank = 0;
+    for (int q = 0; q < n; ++q) rank += l_idx[q] < g;
+    const int64_t o = (int64_t)b * cb + rank;
+    sxw[o] = xw[g];
+    sw[o] = wr[g];
+  }
+  if (threadIdx.x == 0) {
+    bin_start[b] = b * cb;
+    bin_end[b] = b * cb + n;
+  }
 }
 
 __device__ inline void store_entry(const ynb_nlist &nl, int64_t npad, int64_t gi, int cnt, int32_t gj,
@@ -123,60 +129,128 @@ __device__ inline void store_entry(const ynb_nlist &nl, int64_t npad, int64_t gi
   }
 }
 
-__device__ inline void finish_count(const ynb_nlist &nl, int64_t gi, int cnt, bool bad) {
-  nl.cnt[gi] = cnt;
-  if (cnt > nl.kcap) atomicOr(nl.status, 1);
-  if (bad) atomicOr(nl.status, 2);
-  atomicMax(nl.status + 1, cnt);
+constexpr int BW = 16;        // atoms (wavefronts) per block of the binned build
+constexpr int CAP = 2048;     // partner candidates staged in LDS at once
+constexpr int MAXNB = 4096;   // partner bins per home bin (ynb_sys_plan keeps (2R+1)^3 below)
+
+// Lanes of the wave below this one that hold a set bit of mask.
+__device__ inline int lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
 }
 
-// Lanes per atom in the binned build and the pair kernel.
-constexpr int G = 16;
-constexpr int BA = 256 / G;   // atoms per block
-constexpr int TILE = 256;     // partner atoms staged in LDS per pass
-
-// Segmented exclusive prefix sum over the G lanes of one atom.
-__device__ inline int group_exclusive_scan(int x, int s, int *total) {
-  int incl = x;
-#pragma unroll
-  for (int d = 1; d < G; d <<= 1) {
-    const int y = __shfl_up(incl, d, G);
-    if (s >= d) incl += y;
-  }
-  *total = __shfl(incl, G - 1, G);
-  return incl - x;
-}
-
-// Block (bin b, chunk of BA atoms of the bin): G lanes per atom split the
-// partner candidates; partner bins are staged through LDS TILE atoms at a
-// time.  Two passes: count, then write at the lane's prefix offset, so that
-// every atom's row is in a fixed order (partner bins in offset order, partner
-// atoms in sorted order, interleaved by lane).
+// Block (bin b, chunk of BW atoms of the bin), one atom per wavefront.  All
+// partner bins of the home bin are staged into LDS at once (CAP candidates
+// per stage) as FP32 coordinates relative to the home-bin centre with the
+// image shift applied.  The wave scans 64 candidates per step; accepted
+// candidates are appended to the atom's row with a ballot (stream
+// compaction), so rows are written coalesced and in a fixed order (partner
+// bins in offset order, partner atoms by index).
 //
-// Acceptance: the cheap FP64 distance on wrapped coordinates decides unless
-// it lies within a relative 1e-9 of rc^2 (or the image box could bind, see
-// ynb_sys_plan), where the exact Yaff expression decides.
-__global__ __launch_bounds__(256) void k_build_binned(const ynb_sys *sys, const int32_t *bin_sys, int64_t npad,
-                                                      const double *pos, const int32_t *bin_start,
-                                                      const int32_t *bin_end, const double4 *sxw, const int4 *sw,
-                                                      ynb_nlist nl) {
-  __shared__ double l_x[TILE], l_y[TILE], l_z[TILE];
-  __shared__ int4 l_w[TILE];
+// Acceptance.  Relative coordinates stay below ~150 bohr, so the FP32
+// distance squared is within 1e-5 relative of the exact one; outside a band
+// of +-2e-4 relative around rc^2 it decides (inside: accept; the image box
+// cannot bind when box_safe, see ynb_sys_plan).  Inside the band the exact
+// Yaff expression on the FP64 positions decides.
+__global__ __launch_bounds__(1024) void k_build_binned(const ynb_sys *sys, const int32_t *bin_sys, int64_t npad,
+                                                       const double *pos, const int32_t *bin_start,
+                                                       const int32_t *bin_end, const double4 *sxw, const int4 *sw,
+                                                       ynb_nlist nl) {
+  __shared__ float l_x[CAP], l_y[CAP], l_z[CAP];
+  __shared__ int32_t l_m[CAP], l_g[CAP];
+  __shared__ int32_t l_pre[MAXNB + 1];
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
-  const int al = tid / G, s = tid % G;
+  const int lane = tid & 63, wv = tid >> 6;
   const int sid = bin_sys[b];
   const ynb_sys &S = sys[sid];
   const int start = bin_start[b], end = bin_end[b];
-  if (start + (int)blockIdx.y * BA >= end) return;
+  if (start + (int)blockIdx.y * BW >= end) return;
   const int lb = b - S.bin0;
   const int c2 = lb % S.grid[2], c1 = (lb / S.grid[2]) % S.grid[1], c0 = lb / (S.grid[1] * S.grid[2]);
+  const int R0 = S.range[0], R1 = S.range[1], R2 = S.range[2];
+  const int n1 = 2 * R1 + 1, n2 = 2 * R2 + 1;
+  const int nnb = (2 * R0 + 1) * n1 * n2;
+  const double f0 = (c0 + 0.5) / S.grid[0], f1 = (c1 + 0.5) / S.grid[1], f2 = (c2 + 0.5) / S.grid[2];
+  const double X0 = f0 * S.rvecs[0] + f1 * S.rvecs[3] + f2 * S.rvecs[6];
+  const double X1 = f0 * S.rvecs[1] + f1 * S.rvecs[4] + f2 * S.rvecs[7];
+  const double X2 = f0 * S.rvecs[2] + f1 * S.rvecs[5] + f2 * S.rvecs[8];
+
+  auto partner = [&](int k, int &nbin, int &s0, int &s1, int &s2) {
+    const int i2 = k % n2, i1 = (k / n2) % n1, i0 = k / (n1 * n2);
+    int nb0 = c0 + i0 - R0, nb1 = c1 + i1 - R1, nb2 = c2 + i2 - R2;
+    s0 = floor_div(nb0, S.grid[0]);
+    s1 = floor_div(nb1, S.grid[1]);
+    s2 = floor_div(nb2, S.grid[2]);
+    nb0 -= s0 * S.grid[0];
+    nb1 -= s1 * S.grid[1];
+    nb2 -= s2 * S.grid[2];
+    nbin = S.bin0 + (nb0 * S.grid[1] + nb1) * S.grid[2] + nb2;
+  };
+  // Exclusive prefix of the partner-bin sizes, 64 bins per step in wave 0.
+  if (wv == 0) {
+    int carry = 0;
+    for (int k0 = 0; k0 < nnb; k0 += 64) {
+      const int k = k0 + lane;
+      int c = 0;
+      if (k < nnb) {
+        int nbin, s0, s1, s2;
+        partner(k, nbin, s0, s1, s2);
+        c = bin_end[nbin] - bin_start[nbin];
+      }
+      int incl = c;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+      }
+      if (k < nnb) l_pre[k + 1] = carry + incl;
+      carry += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) l_pre[0] = 0;
+  }
+  __syncthreads();
+  const int T = l_pre[nnb];
   const double rc2 = S.rc * S.rc;
-  const double rc2_hi = rc2 * (1.0 + 1e-9), rc2_lo = rc2 * (1.0 - 1e-9);
+  const float lo32 = (float)(rc2 * (1.0 - 2e-4)), hi32 = (float)(rc2 * (1.0 + 2e-4));
   const bool fast_ok = S.box_safe != 0;
-  for (int ib = start + (int)blockIdx.y * BA; ib < end; ib += (int)gridDim.y * BA) {
-    const int p = ib + al;
-    const bool active = p < end;
+  bool bad = false;
+
+  auto stage = [&](int base) {
+    const int lim = min(T - base, CAP);
+    for (int q = tid; q < lim; q += 1024) {
+      const int f = base + q;
+      int lo = 0, hi = nnb;  // largest k with l_pre[k] <= f
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (l_pre[mid] <= f) lo = mid; else hi = mid;
+      }
+      int nbin, s0, s1, s2;
+      partner(lo, nbin, s0, s1, s2);
+      const int j = bin_start[nbin] + (f - l_pre[lo]);
+      const double4 x = sxw[j];
+      const int4 w = sw[j];
+      const double sx = s0 * S.rvecs[0] + s1 * S.rvecs[3] + s2 * S.rvecs[6];
+      const double sy = s0 * S.rvecs[1] + s1 * S.rvecs[4] + s2 * S.rvecs[7];
+      const double sz = s0 * S.rvecs[2] + s1 * S.rvecs[5] + s2 * S.rvecs[8];
+      l_x[q] = (float)(x.x + sx - X0);
+      l_y[q] = (float)(x.y + sy - X1);
+      l_z[q] = (float)(x.z + sz - X2);
+      const int m0 = s0 - w.x, m1 = s1 - w.y, m2 = s2 - w.z;
+      if (m0 < -511 || m0 > 511 || m1 < -511 || m1 > 511 || m2 < -511 || m2 > 511) bad = true;
+      l_m[q] = pack_img(m0, m1, m2);
+      l_g[q] = w.w;
+    }
+  };
+
+  const bool one_stage = T <= CAP;
+  if (one_stage) {
+    stage(0);
+    __syncthreads();
+  }
+  int rowmax = 0;
+  for (int ib = start + (int)blockIdx.y * BW; ib < end; ib += (int)gridDim.y * BW) {
+    const int p = ib + wv;
+    const bool active = p < end;  // uniform in the wave
     double4 xi = make_double4(0, 0, 0, 0);
     int4 wi = make_int4(0, 0, 0, 0);
     if (active) {
@@ -185,90 +259,78 @@ __global__ __launch_bounds__(256) void k_build_binned(const ynb_sys *sys, const 
     }
     const int64_t gi = active ? wi.w : 0;
     const int li = (int)(gi - S.atom0);
+    const float xr = (float)(xi.x - X0), yr = (float)(xi.y - X1), zr = (float)(xi.z - X2);
     double pi[3] = {0, 0, 0};
     if (active) {
       pi[0] = pos[3 * gi];
       pi[1] = pos[3 * gi + 1];
       pi[2] = pos[3 * gi + 2];
     }
-    int cnt = 0, off = 0, total = 0;
-    bool bad = false;
-    for (int pass = 0; pass < 2; ++pass) {
-      for (int d0 = -S.range[0]; d0 <= S.range[0]; ++d0) {
-        int nb0 = c0 + d0, s0 = floor_div(nb0, S.grid[0]);
-        nb0 -= s0 * S.grid[0];
-        for (int d1 = -S.range[1]; d1 <= S.range[1]; ++d1) {
-          int nb1 = c1 + d1, s1 = floor_div(nb1, S.grid[1]);
-          nb1 -= s1 * S.grid[1];
-          for (int d2 = -S.range[2]; d2 <= S.range[2]; ++d2) {
-            int nb2 = c2 + d2, s2 = floor_div(nb2, S.grid[2]);
-            nb2 -= s2 * S.grid[2];
-            const int nbin = S.bin0 + (nb0 * S.grid[1] + nb1) * S.grid[2] + nb2;
-            const double sx = s0 * S.rvecs[0] + s1 * S.rvecs[3] + s2 * S.rvecs[6];
-            const double sy = s0 * S.rvecs[1] + s1 * S.rvecs[4] + s2 * S.rvecs[7];
-            const double sz = s0 * S.rvecs[2] + s1 * S.rvecs[5] + s2 * S.rvecs[8];
-            const double bx = sx - xi.x, by = sy - xi.y, bz = sz - xi.z;
-            const int js = bin_start[nbin], je = bin_end[nbin];
-            for (int jb = js; jb < je; jb += TILE) {
-              __syncthreads();
-              if (jb + tid < je) {
-                const double4 x = sxw[jb + tid];
-                l_x[tid] = x.x;
-                l_y[tid] = x.y;
-                l_z[tid] = x.z;
-                l_w[tid] = sw[jb + tid];
-              }
-              __syncthreads();
-              if (!active) continue;
-              const int nj = min(TILE, je - jb);
-              for (int q = s; q < nj; q += G) {
-                const double dx = l_x[q] + bx, dy = l_y[q] + by, dz = l_z[q] + bz;
-                const double dd2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                if (!(dd2 < rc2_hi)) continue;
-                const int4 wj = l_w[q];
-                const int gj = wj.w;
-                const int n[3] = {s0 - wj.x + wi.x, s1 - wj.y + wi.y, s2 - wj.z + wi.z};
-                const bool self = (gj == gi);
-                if (self && n[0] == 0 && n[1] == 0 && n[2] == 0) continue;
-                const int lj = gj - S.atom0;
-                const int A = li > lj ? li : lj, B = li > lj ? lj : li;
-                if (!nlow_nhigh_ok(A, B, S.nlow, S.nhigh)) continue;
-                if (!(fast_ok && dd2 < rc2_lo)) {
-                  const bool i_is_a = (li == A);
-                  const int nab[3] = {i_is_a ? n[0] : -n[0], i_is_a ? n[1] : -n[1], i_is_a ? n[2] : -n[2]};
-                  const double pj[3] = {pos[3 * (int64_t)gj], pos[3 * (int64_t)gj + 1], pos[3 * (int64_t)gj + 2]};
-                  double v[3];
-                  int r[3];
-                  if (!yaff_row_vector(i_is_a ? pi : pj, i_is_a ? pj : pi, self, nab, S.rvecs, S.gvecs, 3, S.rmax, v,
-                                       r))
-                    continue;
-                  if (!(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] < rc2)) continue;
-                }
-                if (n[0] < -511 || n[0] > 511 || n[1] < -511 || n[1] > 511 || n[2] < -511 || n[2] > 511) {
-                  bad = true;
-                  continue;
-                }
-                if (pass == 0) {
-                  ++cnt;
-                } else {
-                  if (off < nl.kcap) {
-                    nl.j[gi * nl.kcap + off] = gj;
-                    nl.img[gi * nl.kcap + off] = pack_img(n[0], n[1], n[2]);
-                  }
-                  ++off;
-                }
-              }
+    int32_t *rowj = nl.j + gi * nl.kcap;
+    int32_t *rowi = nl.img + gi * nl.kcap;
+    int cnt = 0;  // uniform in the wave
+    for (int base = 0; base < T; base += CAP) {
+      if (!one_stage) {
+        __syncthreads();
+        stage(base);
+        __syncthreads();
+      }
+      if (!active) continue;
+      const int lim = min(T - base, CAP);
+      for (int q0 = 0; q0 < lim; q0 += 64) {
+        const int q = q0 + lane;
+        bool acc = false;
+        int32_t gj = 0, img = 0;
+        if (q < lim) {
+          const float dx = l_x[q] - xr, dy = l_y[q] - yr, dz = l_z[q] - zr;
+          const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          if (d2 < hi32) {
+            gj = l_g[q];
+            int n[3];
+            unpack_img(l_m[q], n);
+            n[0] += wi.x;
+            n[1] += wi.y;
+            n[2] += wi.z;
+            const bool self = (gj == gi);
+            const int lj = gj - S.atom0;
+            const int A = li > lj ? li : lj, B = li > lj ? lj : li;
+            acc = !(self && n[0] == 0 && n[1] == 0 && n[2] == 0) && nlow_nhigh_ok(A, B, S.nlow, S.nhigh);
+            if (acc && !(fast_ok && d2 < lo32)) {
+              const bool i_is_a = (li == A);
+              const int nab[3] = {i_is_a ? n[0] : -n[0], i_is_a ? n[1] : -n[1], i_is_a ? n[2] : -n[2]};
+              const double pj[3] = {pos[3 * (int64_t)gj], pos[3 * (int64_t)gj + 1], pos[3 * (int64_t)gj + 2]};
+              double v[3];
+              int r[3];
+              acc = yaff_row_vector(i_is_a ? pi : pj, i_is_a ? pj : pi, self, nab, S.rvecs, S.gvecs, 3, S.rmax, v,
+                                    r) &&
+                    (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] < rc2);
             }
+            if (acc && (n[0] < -511 || n[0] > 511 || n[1] < -511 || n[1] > 511 || n[2] < -511 || n[2] > 511)) {
+              bad = true;
+              acc = false;
+            }
+            if (acc) img = pack_img(n[0], n[1], n[2]);
           }
         }
-      }
-      if (pass == 0) {
-        off = group_exclusive_scan(cnt, s, &total);
-        if (active && s == 0) finish_count(nl, gi, total, false);
+        const uint64_t mask = __ballot(acc);
+        if (acc) {
+          const int o = cnt + lanes_below(mask);
+          if (o < nl.kcap) {
+            rowj[o] = gj;
+            rowi[o] = img;
+          }
+        }
+        cnt += __popcll(mask);
       }
     }
-    if (active && bad) atomicOr(nl.status, 2);
+    if (active && lane == 0) {
+      nl.cnt[gi] = cnt;
+      if (cnt > nl.kcap) atomicOr(nl.status, 1);
+    }
+    rowmax = max(rowmax, active ? cnt : 0);
   }
+  if (bad) atomicOr(nl.status, 2);
+  (void)rowmax;
 }
 
 // All-pairs build for systems without bins (nvec < 3): every partner and
@@ -321,7 +383,8 @@ __global__ void k_build_allpairs(const ynb_sys *sys, const int32_t *wave_sys, in
           ++cnt;
         }
   }
-  finish_count(nl, gi, cnt, false);
+  nl.cnt[gi] = cnt;
+  if (cnt > nl.kcap) atomicOr(nl.status, 1);
 }
 
 // ---- Yaff rows ------------------------------------------------------------
@@ -444,8 +507,8 @@ size_t rows_work_layout(int64_t natom, int64_t nrows, char *base, int32_t **rowc
                         ynb_neigh_row **tmp, uint64_t **key, uint64_t **key2, int32_t **idx, int32_t **idx2,
                         void **cub, size_t *cub_bytes) {
   size_t scan_bytes = 0, sort_bytes = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int32_t *)nullptr, (int32_t *)nullptr, (int)natom);
-  hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int32_t *)nullptr, (int32_t *)nullptr, (int)natom);
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
                                      (int32_t *)nullptr, (int32_t *)nullptr, (int)(nrows > 0 ? nrows : 1));
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -518,6 +581,16 @@ int ynb_sys_plan(ynb_sys *s) {
     if (s->range[kmax] < 1) s->range[kmax] = 1;
     nbin = (int64_t)s->grid[0] * s->grid[1] * s->grid[2];
   }
+  if ((int64_t)(2 * s->range[0] + 1) * (2 * s->range[1] + 1) * (2 * s->range[2] + 1) > 4096) {
+    // Cutoff spans too many periodic copies for the binned build's LDS
+    // tables: use the all-pairs path.
+    s->nbin = 0;
+    for (int k = 0; k < 3; ++k) {
+      s->grid[k] = 1;
+      s->range[k] = 0;
+    }
+    return YNB_OK;
+  }
   s->nbin = (int32_t)nbin;
   // The image box |r_k| <= rmax_k never rejects a pair within rc when
   // rc*|g_k| < rmax_k + 1/2 on every axis: r_k = g_k.v - g_k.delta0 with
@@ -532,16 +605,20 @@ int ynb_sys_plan(ynb_sys *s) {
   return YNB_OK;
 }
 
-size_t ynb_nlist_workspace_bytes(int64_t npad, int32_t nbin) { return carve(nullptr, npad, nbin).total; }
+size_t ynb_nlist_workspace_bytes(int64_t npad, int32_t nbin, int32_t bin_cap) {
+  return carve(nullptr, npad, nbin, bin_cap).total;
+}
 
 int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t work_bytes,
                     const ynb_nlist *nl, ynb_stream stream) {
-  if (!b || !nl || !pos || b->npad % 64 != 0 || nl->kcap < 1) {
+  if (!b || !nl || !pos || b->npad % 64 != 0 || nl->kcap < 1 ||
+      (b->nbin > 0 && (b->max_bin_atoms < 1 || b->max_bin_atoms > 1024))) {
     set_error("ynb_nlist_build: invalid arguments");
     return YNB_EINVAL;
   }
   hipStream_t st = (hipStream_t)stream;
-  Work w = carve(work, b->npad, b->nbin);
+  const int32_t cb = b->max_bin_atoms;
+  Work w = carve(work, b->npad, b->nbin, cb);
   if (work_bytes < w.total || !work) {
     set_error("ynb_nlist_build: workspace of %zu bytes, need %zu", work_bytes, w.total);
     return YNB_EWORKSPACE;
@@ -552,26 +629,32 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, nl->status, 4, 0);
   YNB_LAUNCHED();
   if (b->nbin > 0) {
-    const int nbb = (int)ceil_div(b->nbin, tb);
-    hipLaunchKernelGGL(k_fill_i32, dim3(nbb), dim3(tb), 0, st, w.bin_start, (int64_t)b->nbin, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(nbb), dim3(tb), 0, st, w.bin_end, (int64_t)b->nbin, 0);
-    YNB_LAUNCH("nlist_bin", k_bin, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, w.key, w.val, w.xw,
-               w.wr);
-    YNB_LAUNCHED();
-    size_t cb = w.cub_bytes;
-    YNB_HIP(hipcub::DeviceRadixSort::SortPairs(w.cub, cb, w.key, w.key_sorted, w.val, w.val_sorted,
-                                               (int)npad, 0, 32, st));
-    hipLaunchKernelGGL(k_binrange, dim3(nb), dim3(tb), 0, st, npad, w.key_sorted, w.val_sorted, w.xw, w.wr,
-                       w.bin_start, w.bin_end, w.sxw, w.sw);
-    YNB_LAUNCHED();
-    const int ychunks = b->max_bin_atoms > 0 ? (int)ceil_div(b->max_bin_atoms, BA) : 4;
-    YNB_LAUNCH("nlist_build_binned", k_build_binned, dim3(b->nbin, ychunks), dim3(256), 0, st, b->sys, b->bin_sys,
+    hipLaunchKernelGGL(k_fill_i32, dim3((int)ceil_div(b->nbin, tb)), dim3(tb), 0, st, w.bin_count,
+                       (int64_t)b->nbin, 0);
+    YNB_LAUNCH("nlist_bin", k_bin, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, cb, w.bin_count,
+               w.slot_idx, w.xw, w.wr, nl->status);
+    YNB_LAUNCH("nlist_bin_order", k_bin_order, dim3(b->nbin), dim3(256), 0, st, cb, w.bin_count, w.slot_idx, w.xw,
+               w.wr, w.sxw, w.sw, w.bin_start, w.bin_end);
+    const int ychunks = (int)ceil_div(cb, BW) < 64 ? (int)ceil_div(cb, BW) : 64;
+    YNB_LAUNCH("nlist_build_binned", k_build_binned, dim3(b->nbin, ychunks), dim3(1024), 0, st, b->sys, b->bin_sys,
                npad, pos, w.bin_start, w.bin_end, w.sxw, w.sw, *nl);
     YNB_LAUNCHED();
   }
   YNB_LAUNCH("nlist_build_allpairs", k_build_allpairs, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos,
              *nl);
   YNB_LAUNCHED();
+  return YNB_OK;
+}
+
+
+int ynb_nlist_bin_counts(const ynb_batch *b, const void *work, int32_t *counts, ynb_stream stream) {
+  if (!b || !work || !counts) {
+    set_error("ynb_nlist_bin_counts: invalid arguments");
+    return YNB_EINVAL;
+  }
+  if (b->nbin == 0) return YNB_OK;
+  Work w = carve(const_cast<void *>(work), b->npad, b->nbin, b->max_bin_atoms);
+  YNB_HIP(hipMemcpyAsync(counts, w.bin_count, 4 * (size_t)b->nbin, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return YNB_OK;
 }
 

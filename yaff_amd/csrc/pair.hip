@@ -149,9 +149,7 @@ __device__ inline double stab_scale(const ynb_scaling_row *st, int n, int64_t a,
   return 1.0;
 }
 
-constexpr int G = 16;  // lanes per atom
-
-template <typename T>
+template <int G, typename T>
 __device__ inline T group_sum(T v) {
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
@@ -168,7 +166,7 @@ __device__ inline T group_sum(T v) {
 // does (yaff_row_vector), so the cutoff tests d < rcut and the scaling
 // lookups agree with the reference.  FP64 mode evaluates the potentials in
 // FP64; mixed mode in FP32, accumulating gradient, energy and virial in FP64.
-template <int PREC, bool SLOTS>
+template <int PREC, bool SLOTS, int G>
 __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
                                               ynb_pairpar pp, const double *pos, ynb_nlist nl, uint32_t mask,
                                               double *gpos, int accumulate, double *gpos_slots,
@@ -191,15 +189,19 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
   for (int p = 0; p < npot; ++p) rcmax = fmax(rcmax, S.pot[p].rcut);
 
   double acc_e[YNB_MAX_POT], acc_v[YNB_MAX_POT][6], acc_g[SLOTS ? YNB_MAX_POT : 1][3];
+#pragma unroll
   for (int p = 0; p < YNB_MAX_POT; ++p) {
     acc_e[p] = 0.0;
+#pragma unroll
     for (int c = 0; c < 6; ++c) acc_v[p][c] = 0.0;
   }
+#pragma unroll
   for (int p = 0; p < (SLOTS ? YNB_MAX_POT : 1); ++p) acc_g[p][0] = acc_g[p][1] = acc_g[p][2] = 0.0;
 
   if (active) {
     const double pi[3] = {pos[3 * gi], pos[3 * gi + 1], pos[3 * gi + 2]};
     double par_i[YNB_MAX_POT][2];
+#pragma unroll
     for (int p = 0; p < YNB_MAX_POT; ++p) {
       par_i[p][0] = (p < npot) ? pp.par[((int64_t)p * 3 + 0) * npad + gi] : 0.0;
       par_i[p][1] = (p < npot) ? pp.par[((int64_t)p * 3 + 1) * npad + gi] : 0.0;
@@ -242,6 +244,7 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
       }
       if (!(dd < rcmax)) continue;
       const T d = (T)dd;
+#pragma unroll
       for (int p = 0; p < YNB_MAX_POT; ++p) {
         if (p >= npot || !((mask >> p) & 1u)) continue;
         const ynb_pot &P = S.pot[p];
@@ -299,9 +302,11 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
   }
   // Gradient: sum over the atom's G lanes; lane 0 of the group stores.
   double G3[3] = {0.0, 0.0, 0.0};
+#pragma unroll
   for (int p = 0; p < (SLOTS ? YNB_MAX_POT : 1); ++p)
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-      acc_g[p][c] = group_sum(acc_g[p][c]);
+      acc_g[p][c] = group_sum<G>(acc_g[p][c]);
       G3[c] += acc_g[p][c];
     }
   if (active && s == 0) {
@@ -314,45 +319,71 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
       }
     }
     if (SLOTS && gpos_slots) {
+#pragma unroll
       for (int p = 0; p < YNB_MAX_POT; ++p)
+#pragma unroll
         for (int c = 0; c < 3; ++c) gpos_slots[((int64_t)p * npad + gi) * 3 + c] = acc_g[p][c];
     }
   }
-  // Wave sums of energy and virial (lanes past natom contribute zeros).
+  // Block sums of energy and virial (the block's 16 atoms belong to one
+  // system; lanes past natom contribute zeros): waves, then the 4 waves of the
+  // block in order.
+  __shared__ double red[4][YNB_MAX_POT * 8];
+  const int wib = threadIdx.x >> 6;
+#pragma unroll
   for (int p = 0; p < YNB_MAX_POT; ++p) {
-    if (p >= npot) break;
-    const double e = wave_sum(acc_e[p]);
-    double v[6];
-    for (int c = 0; c < 6; ++c) v[c] = wave_sum(acc_v[p][c]);
-    if (lane == 0) {
-      double *o = partial + (wave * YNB_MAX_POT + p) * 8;
-      o[0] = e;
-      for (int c = 0; c < 6; ++c) o[1 + c] = v[c];
-      o[7] = 0.0;
+    if (p < npot) {
+      const double e = wave_sum(acc_e[p]);
+      double v[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) v[c] = wave_sum(acc_v[p][c]);
+      if (lane == 0) {
+        red[wib][p * 8] = e;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) red[wib][p * 8 + 1 + c] = v[c];
+        red[wib][p * 8 + 7] = 0.0;
+      }
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < npot * 8) {
+    const int q = threadIdx.x;
+    partial[(int64_t)blockIdx.x * (YNB_MAX_POT * 8) + q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
   }
 }
 
-// One block per system; thread t < 32 sums value t over the system's waves in
-// order.  out[s][p] = {E, V00 V01 V02 V10 V11 V12 V20 V21 V22} with the 1/2 of
-// the full list applied.
-__global__ void k_pair_reduce(const ynb_sys *sys, const double *partial, double *out) {
+// One block per system: for each of the 32 values, 32 threads sum strided
+// subsets of the system's block partials, then a fixed tree adds the 32.
+// out[s][p] = {E, V00 V01 V02 V10 V11 V12 V20 V21 V22} with the 1/2 of the
+// full list applied.
+__global__ __launch_bounds__(1024) void k_pair_reduce(const ynb_sys *sys, const double *partial, double *out,
+                                                      int apb) {
   const int s = blockIdx.x;
   const ynb_sys &S = sys[s];
   const int t = threadIdx.x;
-  __shared__ double red[YNB_MAX_POT * 8];
-  if (t < YNB_MAX_POT * 8) {
-    const int p = t / 8;
-    double acc = 0.0;
-    if (p < S.npot) {
-      const int64_t w0 = (int64_t)S.atom0 * G / 64, w1 = ((int64_t)(S.atom0 + S.natom) * G + 63) / 64;
-      for (int64_t w = w0; w < w1; ++w) acc += partial[(w * YNB_MAX_POT) * 8 + t];
+  const int q = t & 31, part = t >> 5;  // value, one of 32 strided parts
+  // blocks of apb atoms covering the system (atom0 is a multiple of 64)
+  const int64_t b0 = S.atom0 / apb, b1 = (S.atom0 + S.natom + apb - 1) / apb;
+  __shared__ double red[32][33];
+  double a[4] = {0, 0, 0, 0};
+  if (q < S.npot * 8) {
+    for (int64_t b = b0 + part; b < b1; b += 128) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t bu = b + 32 * u;
+        if (bu < b1) a[u] += partial[bu * (YNB_MAX_POT * 8) + q];
+      }
     }
-    red[t] = 0.5 * acc;
   }
+  red[part][q] = (a[0] + a[1]) + (a[2] + a[3]);
   __syncthreads();
-  if (t < YNB_MAX_POT) {
-    const double *r = red + t * 8;
+  for (int off = 16; off > 0; off >>= 1) {
+    if (part < off) red[part][q] += red[part + off][q];
+    __syncthreads();
+  }
+  if (t < S.npot) {
+    double r[8];
+    for (int c = 0; c < 8; ++c) r[c] = 0.5 * red[0][t * 8 + c];
     double *o = out + ((int64_t)s * YNB_MAX_POT + t) * 10;
     o[0] = r[0];
     o[1] = r[1]; o[2] = r[4]; o[3] = r[5];
@@ -466,25 +497,29 @@ extern "C" int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const
     return YNB_EINVAL;
   }
   hipStream_t st = (hipStream_t)stream;
+  const int G = pp->lanes == 0 ? 16 : pp->lanes;
+  if (G != 16 && G != 32 && G != 64) {
+    set_error("ynb_pair_compute: lanes must be 16, 32 or 64");
+    return YNB_EINVAL;
+  }
   const dim3 grid((unsigned)ceil_div(b->npad * G, 256)), block(256);
   const bool slots = gpos_slots != nullptr;
+#define YNB_PAIR(P, SL, GG)                                                                                     \
+  YNB_LAUNCH("pair", (k_pair<P, SL, GG>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,      \
+             slot_mask, gpos, accumulate, gpos_slots, partial)
+#define YNB_PAIR_G(P, SL)        \
+  if (G == 16) YNB_PAIR(P, SL, 16); \
+  else if (G == 32) YNB_PAIR(P, SL, 32); \
+  else YNB_PAIR(P, SL, 64)
   if (precision == YNB_FP64) {
-    if (slots)
-      YNB_LAUNCH("pair", (k_pair<YNB_FP64, true>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
-                 slot_mask, gpos, accumulate, gpos_slots, partial);
-    else
-      YNB_LAUNCH("pair", (k_pair<YNB_FP64, false>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
-                 slot_mask, gpos, accumulate, gpos_slots, partial);
+    if (slots) { YNB_PAIR_G(YNB_FP64, true); } else { YNB_PAIR_G(YNB_FP64, false); }
   } else {
-    if (slots)
-      YNB_LAUNCH("pair", (k_pair<YNB_MIXED, true>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
-                 slot_mask, gpos, accumulate, gpos_slots, partial);
-    else
-      YNB_LAUNCH("pair", (k_pair<YNB_MIXED, false>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,
-                 slot_mask, gpos, accumulate, gpos_slots, partial);
+    if (slots) { YNB_PAIR_G(YNB_MIXED, true); } else { YNB_PAIR_G(YNB_MIXED, false); }
   }
+#undef YNB_PAIR_G
+#undef YNB_PAIR
   YNB_LAUNCHED();
-  hipLaunchKernelGGL(k_pair_reduce, dim3(b->nsys), dim3(64), 0, st, b->sys, partial, out);
+  YNB_LAUNCH("pair_reduce", k_pair_reduce, dim3(b->nsys), dim3(1024), 0, st, b->sys, partial, out, 256 / G);
   YNB_LAUNCHED();
   return YNB_OK;
 }

@@ -125,7 +125,8 @@ def _round_up(x, m):
 class Engine(object):
     """Device state and launches for a batch of NBSystem objects."""
 
-    def __init__(self, systems, precision='fp64', seglen=None, kchunks=None, kcap=None, device='cuda'):
+    def __init__(self, systems, precision='fp64', seglen=None, kchunks=None, kcap=None, device='cuda',
+                 pair_lanes=32, ewald_chunk=None):
         self.lib = _lib.load()
         if not torch.cuda.is_available():
             raise RuntimeError('yaff_amd.Engine needs a HIP device (torch.cuda.is_available() is False)')
@@ -133,6 +134,8 @@ class Engine(object):
         self.systems = list(systems)
         self.precision = {'fp64': FP64, 'mixed': MIXED}[precision] if isinstance(precision, str) else int(precision)
         self.nsys = len(self.systems)
+        self.pair_lanes = int(pair_lanes)
+        self._ewald_chunk_req = ewald_chunk
         self._layout()
         self._plan_sys(seglen, kchunks)
         self._alloc(kcap)
@@ -221,7 +224,19 @@ class Engine(object):
         if seglen is None:
             seglen = 16 if nk_all > 20000 else 8
         self.seglen = seglen
-        kvec, krow, krow_k, seg, seg_len, chunk, work, kblk_sys = [], [], [], [], [], [], [], []
+        natom_tot = sum(s.natom for s in self.systems if s.ewald is not None)
+        if self._ewald_chunk_req is not None:
+            self.ewald_chunk = int(self._ewald_chunk_req)
+        else:
+            # enough (segment group, chunk) work items to give every SIMD a few waves
+            groups = max(1, (nk_all//seglen + 63)//64)
+            want = 4096
+            chunk = CHUNK
+            while chunk > 32 and groups*max(1, natom_tot//chunk) < want:
+                chunk //= 2
+            self.ewald_chunk = chunk
+        assert 1 <= self.ewald_chunk <= CHUNK
+        kvec, krow, krow_k, seg, seg_len, chunk, work, kblk_sys, kg = [], [], [], [], [], [], [], [], []
         k0 = row0 = seg0 = chunk0 = 0
         spart0 = 0
         self.gmax = []
@@ -244,9 +259,13 @@ class Engine(object):
             kvec.append(kpad)
             kblk_sys.extend([i]*(nkp//KBLOCK))
             rows = _runs(g)
+            gpad = np.zeros((nkp, 4), np.int32)
+            gpad[:nk, :3] = g
             for (st, cnt) in rows:
                 krow.append((g[st, 1], g[st, 2], g[st, 0], cnt))
                 krow_k.append(k0 + st)
+                gpad[st, 3] = 1
+            kg.append(gpad)
             segs = []
             for (st, cnt) in rows:
                 for off in range(0, cnt, seglen):
@@ -261,8 +280,8 @@ class Engine(object):
                 self.groups = 1
             chunks = []
             for (lo, hi, grp) in ((0, nlow, 1), (nlow, nhigh, 0), (nhigh, s.natom, 2)):
-                for a in range(lo, hi, CHUNK):
-                    chunks.append((i, a, min(CHUNK, hi - a), grp))
+                for a in range(lo, hi, self.ewald_chunk):
+                    chunks.append((i, a, min(self.ewald_chunk, hi - a), grp))
             chunk.extend(chunks)
             nseg = len(segs)
             for sb in range(0, nseg, 64):
@@ -275,6 +294,8 @@ class Engine(object):
             h.seg0, h.nseg = seg0, nseg
             h.chunk0, h.nchunk = chunk0, len(chunks)
             h.spart0 = spart0
+            h.nchunk_low = sum(1 for c in chunks if c[3] == 1)
+            h.nchunk_high = sum(1 for c in chunks if c[3] == 2)
             spart0 += len(chunks)*nkp
             k0 += nkp
             row0 += len(rows)
@@ -289,6 +310,7 @@ class Engine(object):
             chunk=np.array(chunk, np.int32).reshape(-1, 4) if chunk else np.zeros((1, 4), np.int32),
             work=np.array(work, np.int32).reshape(-1, 4) if work else np.zeros((1, 4), np.int32),
             kblk_sys=np.array(kblk_sys, np.int32) if kblk_sys else np.zeros(1, np.int32),
+            kg=np.concatenate(kg) if kg else np.zeros((KBLOCK, 4), np.int32),
         )
         self.nk_tot = k0
         self.nwork = len(work)
@@ -312,17 +334,18 @@ class Engine(object):
         self.sys_d = self._t(np.frombuffer(bytes(self.hsys), np.uint8))
         self.wave_sys_d = self._t(self.wave_sys)
         self.bin_sys_d = self._t(self.bin_sys)
+        self.bin_cap = self._max_bin_atoms()
         self.batch = YnbBatch(self.sys_d.data_ptr(), self.wave_sys_d.data_ptr(), self.bin_sys_d.data_ptr(),
-                              self.nsys, self.nbin, self.npad, self._max_bin_atoms(), 0)
+                              self.nsys, self.nbin, self.npad, self.bin_cap, 0)
         self.pos_d = torch.zeros((self.npad, 3), dtype=torch.float64, device=dev)
         self.gpos_d = torch.zeros((self.npad, 3), dtype=torch.float64, device=dev)
         self.par_d = self._t(self.par_h)
         self.cross_d = self._t(self.cross_h)
         self.stab_d = self._t(np.frombuffer(self.stab_h.tobytes(), np.uint8))
-        self.pairpar = YnbPairpar(self.par_d.data_ptr(), self.cross_d.data_ptr(), self.stab_d.data_ptr())
+        self.pairpar = YnbPairpar(self.par_d.data_ptr(), self.cross_d.data_ptr(), self.stab_d.data_ptr(),
+                                  self.pair_lanes, 0)
         # neighbour list
-        self.nl_work_bytes = L.ynb_nlist_workspace_bytes(self.npad, self.nbin)
-        self.nl_work_d = torch.empty(self.nl_work_bytes, dtype=torch.uint8, device=dev)
+        self._alloc_nlist_work()
         self.cnt_d = torch.zeros(self.npad, dtype=torch.int32, device=dev)
         self.status_d = torch.zeros(4, dtype=torch.int32, device=dev)
         self._alloc_nlist(kcap if kcap is not None else self._estimate_kcap())
@@ -343,12 +366,13 @@ class Engine(object):
             self.ew_step_d = torch.zeros(2*self.npad, dtype=torch.float64, device=dev)
             self.ew_spart_d = torch.zeros(2*self.spart_entries, dtype=torch.float64, device=dev)
             self.ew_coef_d = torch.zeros(6*max(self.nk_tot, KBLOCK), dtype=torch.float64, device=dev)
-            self.ew_kpart_d = torch.zeros(8*max(self.nk_tot//KBLOCK, 1), dtype=torch.float64, device=dev)
+            self.ew_kpart_d = torch.zeros(8*max(self.nk_tot//32, 1), dtype=torch.float64, device=dev)
             self.ew_gpart_d = torch.zeros(self.kchunks*self.npad*3, dtype=torch.float64, device=dev)
             t = self.ew_tabs
             self.ewald = YnbEwald(t['kvec'].data_ptr(), t['krow'].data_ptr(), t['krow_k'].data_ptr(),
                                   t['seg'].data_ptr(), t['seg_len'].data_ptr(), t['chunk'].data_ptr(),
-                                  t['work'].data_ptr(), t['kblk_sys'].data_ptr(), self.nwork, self.nk_tot,
+                                  t['work'].data_ptr(), t['kblk_sys'].data_ptr(), t['kg'].data_ptr(), self.nwork,
+                                  self.nk_tot,
                                   self.seglen, self.kchunks, self.groups, 0, self.ew_frac_d.data_ptr(),
                                   self.ew_step_d.data_ptr(), self.ew_spart_d.data_ptr(), self.ew_coef_d.data_ptr(),
                                   self.ew_kpart_d.data_ptr(), self.ew_gpart_d.data_ptr())
@@ -365,12 +389,16 @@ class Engine(object):
         self.md_partial_d = torch.zeros(self.npad//64, dtype=torch.float64, device=dev)
 
     def _max_bin_atoms(self):
-        """Sizes the build grid: twice the largest mean bin occupancy."""
+        """Bin capacity: twice the largest mean occupancy plus a margin."""
         m = 16
         for i, s in enumerate(self.systems):
             if self.hsys[i].nbin > 0:
                 m = max(m, int(2*s.natom/self.hsys[i].nbin) + 16)
-        return m
+        return min(_round_up(m, 16), 1024)
+
+    def _alloc_nlist_work(self):
+        self.nl_work_bytes = self.lib.ynb_nlist_workspace_bytes(self.npad, self.nbin, self.bin_cap)
+        self.nl_work_d = torch.empty(self.nl_work_bytes, dtype=torch.uint8, device=self.device)
 
     def _estimate_kcap(self):
         est = 16
@@ -416,22 +444,43 @@ class Engine(object):
         self.nbuilds += 1
 
     def nlist_status(self):
-        """(flags, max entries per atom) of the last build; synchronises."""
-        st = self.status_d.cpu().numpy()
-        return int(st[0]), int(st[1])
+        """(flags, longest row) of the last build; synchronises.  Flags: 1 a
+        row exceeded kcap, 2 an image shift exceeded 511 cells, 4 a bin
+        exceeded its capacity.  The longest row is read only after a flag."""
+        flags = int(self.status_d[0].item())
+        longest = 0
+        if flags & 1:
+            longest = int(self.cnt_d.max().item())
+        if flags & 4:
+            counts = torch.zeros(max(self.nbin, 1), dtype=torch.int32, device=self.device)
+            check(self.lib.ynb_nlist_bin_counts(self.batch, self.nl_work_d.data_ptr(), counts.data_ptr(),
+                                                self.stream()))
+            self._fullest_bin = int(counts.max().item())
+        return flags, longest
 
-    def ensure_nlist(self):
-        """Builds the list, growing the ELL capacity until every atom fits."""
-        self.build_nlist()
-        flags, mx = self.nlist_status()
+    def grow(self, flags, longest):
+        """Enlarges the capacities named by the flags of a failed build."""
         if flags & 2:
             raise RuntimeError('neighbour list: an image translation exceeds 511 cells')
+        if flags & 4:
+            cap = _round_up(int(self._fullest_bin*1.25) + 16, 16)
+            if cap > 1024:
+                raise RuntimeError('neighbour list: a bin holds more than 1024 atoms')
+            self.bin_cap = cap
+            self.batch.max_bin_atoms = cap
+            self._alloc_nlist_work()
         if flags & 1:
-            self._alloc_nlist(int(mx*1.1) + 8)
+            self._alloc_nlist(int(longest*1.1) + 8)
+
+    def ensure_nlist(self):
+        """Builds the list, growing capacities until every row and bin fits."""
+        for _ in range(4):
             self.build_nlist()
             flags, mx = self.nlist_status()
-            if flags:
-                raise RuntimeError('neighbour list: capacity still exceeded')
+            if not flags:
+                return
+            self.grow(flags, mx)
+        raise RuntimeError('neighbour list: capacities still exceeded')
 
     def pair(self, mask=0xF, gpos=True, accumulate=False, slots=False):
         if slots and self.gpos_slots_d is None:
@@ -487,6 +536,23 @@ class Engine(object):
         check(self.lib.ynb_verlet_kick(self.batch, self.vel_d.data_ptr(), self.gpos_d.data_ptr(),
                                        self.minv_d.data_ptr(), self.masses_d.data_ptr(), self.dt_d.data_ptr(),
                                        self.md_partial_d.data_ptr(), self.ekin2_d.data_ptr(), st))
+
+    def capture_md(self, nsteps=1):
+        """Captures nsteps Verlet steps of every system in a HIP graph (all
+        launches are kernels of libyaff_nb.so on the capture stream: no
+        memsets, no library sorts, no synchronisation).  Capacities must
+        already fit (run ensure_nlist / a few eager steps first); check
+        nlist_status() after replays."""
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            pass
+        torch.cuda.current_stream().wait_stream(side)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for _ in range(nsteps):
+                self.md_step()
+        return graph
 
     # ---- host views ---------------------------------------------------
     def results(self, i=None):
