@@ -51,6 +51,12 @@ def parse():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-mixed', action='store_true')
     ap.add_argument('--cpu-evals', type=int, default=2)
+    ap.add_argument('--no-batched', action='store_true')
+    ap.add_argument('--batch-steps', type=int, default=200)
+    ap.add_argument('--batch-systems', type=int, default=64)
+    ap.add_argument('--no-served', action='store_true')
+    ap.add_argument('--serve-requests', type=int, default=150)
+    ap.add_argument('--serve-rates', type=str, default='25,100,400')
     return ap.parse_args()
 
 
@@ -180,6 +186,104 @@ def cpu_baseline(s, evals):
     return n*evals/t, t, res
 
 
+def progress(msg):
+    print('bench: %s' % msg, file=sys.stderr, flush=True)
+
+
+def batched(nsys, steps, warmup):
+    """64 independent systems (200-6000 atoms, even index LJ fluid, odd index
+    rocksalt, triclinic angles 80-100 degrees, seed 0) in one set of launches:
+    autotuned launch parameters, MD steps replayed from a HIP graph, mixed
+    precision.  Compared with the same systems evaluated one at a time
+    through the FP64 drop-in yaff.pes interface (host arrays in and out, the
+    list rebuilt every call), with the FP64 batch, and with the mixed-vs-FP64
+    force error of every system."""
+    import torch
+    from yaff_amd import workloads, autotune
+    from yaff_amd.batch import BatchMD, forcefield, nbsystem
+    from yaff_amd.engine import Engine
+    systems = workloads.batch_of_systems(nsys, (200, 6000), (80.0, 100.0), seed=0)
+    natom = sum(s.natom for s in systems)
+    out = {'workload': '%d systems, %d atoms (LJ fluid / rocksalt alternating, 200-6000 atoms, angles 80-100 deg)'
+                       % (nsys, natom), 'natom': natom, 'steps': steps}
+
+    # FP64 drop-in, one system per ForceField.compute call
+    ffs = [forcefield(s) for s in systems]
+    for ff, s in zip(ffs, systems):
+        ff.compute(np.zeros((s.natom, 3)), np.zeros((3, 3)))
+    reps = 3
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for ff, s in zip(ffs, systems):
+            ff.update_pos(s.pos)
+            ff.compute(np.zeros((s.natom, 3)), np.zeros((3, 3)))
+    t_drop = time.perf_counter() - t0
+    out['dropin_fp64'] = {'value': natom*reps/t_drop, 'ms_per_pass': t_drop/reps*1e3,
+                          'note': 'ForceField.compute(gpos, vtens) of each system in turn, FP64, host buffers'}
+    del ffs
+
+    # mixed vs FP64 forces of the same systems, batched
+    nbs = [nbsystem(s) for s in systems]
+    res = {}
+    for prec in ('fp64', 'mixed'):
+        e = Engine(nbs, precision=prec)
+        e.ensure_nlist()
+        e.compute(build=False)
+        res[prec] = e.results()
+        del e
+    rms, emax, vmax = [], [], []
+    for (e64, g64, v64), (em, gm, vm) in zip(res['fp64'], res['mixed']):
+        rms.append(np.sqrt(((gm - g64)**2).sum(axis=1).mean()/(g64**2).sum(axis=1).mean()))
+        emax.append(abs(em - e64)/abs(e64))
+        vmax.append(np.abs(vm - v64).max()/np.abs(v64).max())
+    out['mixed_error'] = {'force_rms_rel_err_max': float(max(rms)), 'energy_rel_err_max': float(max(emax)),
+                          'virial_rel_err_max': float(max(vmax))}
+
+    for prec in ('mixed', 'fp64'):
+        md = BatchMD(systems, precision=prec)
+        t0 = time.perf_counter()
+        tun = autotune.tune(md.engine, cache=None)
+        t_tune = time.perf_counter() - t0
+        md.warm(warmup)
+        md.capture(1)
+        nst = steps if prec == 'mixed' else max(steps//4, 1)
+        dt = md.run(nst)
+        progress('batched %s %.3g atom-evals/s' % (prec, natom*nst/dt))
+        out['batched_' + prec] = {'value': natom*nst/dt, 'ms_per_step': dt/nst*1e3, 'steps': nst,
+                                  'tuning': tun, 'tune_s': t_tune}
+        del md
+        torch.cuda.empty_cache()
+    out['mixed_batched_vs_dropin_fp64'] = out['batched_mixed']['value']/out['dropin_fp64']['value']
+    out['mixed_batched_vs_fp64_batched'] = out['batched_mixed']['value']/out['batched_fp64']['value']
+    return out
+
+
+def served(nreq, rates):
+    """Poisson request streams (single evaluations and 20-step MD segments,
+    half and half; systems log-uniform in 100-50000 atoms, LJ fluid /
+    rocksalt alternating, triclinic angles 80-100 degrees) served by the
+    continuous-batching server, FP64 and mixed, at each arrival rate:
+    p50/p99 latency (arrival to results on the host) and throughput."""
+    from yaff_amd import serve
+    out = {'workload': '%d requests per rate, log-uniform 100-50000 atoms, 50%% eval / 50%% 20-step MD' % nreq}
+    warm = serve.poisson_requests(6, 1000.0, natom_range=(100, 3000), nsteps=2, seed=99)
+    serve.Server(precision='mixed').run(warm, realtime=False)
+    for prec in ('mixed', 'fp64'):
+        res = []
+        for rate in rates:
+            reqs = serve.poisson_requests(nreq, rate, seed=11)
+            srv = serve.Server(precision=prec)
+            done = srv.run(reqs)
+            summ = serve.latency_summary(done)
+            summ.update({'rate_per_s': rate, 'batch_forms': srv.stats['forms'], 'iterations': srv.stats['iterations'],
+                         'form_s': srv.stats['form_s'], 'peak_est_bytes': srv.peak_bytes})
+            res.append(summ)
+            progress('served %s rate %g: p50 %.1f ms p99 %.1f ms' % (prec, rate, summ['p50_ms'], summ['p99_ms']))
+        out[prec] = res
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -263,6 +367,15 @@ def main():
                         'force_max_abs_err': float(np.abs(gm - g64).max()),
                         'virial_rel_err': float(np.abs(vm - v64).max()/np.abs(v64).max()),
                         'kernels': kernel_table(profm, K)}
+
+    progress('single-system FP64/mixed done')
+    if not args.no_batched:
+        out['batched'] = batched(args.batch_systems, args.batch_steps, 2)
+        progress('batched done')
+
+    if not args.no_served:
+        out['served'] = served(args.serve_requests, [float(x) for x in args.serve_rates.split(',')])
+        progress('served done')
 
     if not args.no_cpu_baseline:
         try:

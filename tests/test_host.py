@@ -1,0 +1,160 @@
+"""Host-side logic without a GPU: the Ewald k-vector set and its launch
+tables, scaling tables, the drop-in classes' bookkeeping, workload
+generators and the serving loop's admission."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+
+def _brute_kvectors(rvecs, alpha, gcut):
+    """The k set of compute_ewald_reci (ewald.c:41-58) by plain loops: g2 >= 0,
+    half space, |k| <= 2 pi gcut, visited g2, g1, g0 ascending."""
+    gvecs = np.linalg.inv(rvecs).T
+    gsp = 1.0/np.linalg.norm(rvecs, axis=1)   # cell.c: gspacing_i = 1/|r_i|
+    gmax = np.ceil(gcut/gsp - 0.5).astype(int)
+    out = []
+    vol = abs(np.linalg.det(rvecs))
+    for g2 in range(0, gmax[2] + 1):
+        for g1 in range(-gmax[1], gmax[1] + 1):
+            for g0 in range(-gmax[0], gmax[0] + 1):
+                if g2 == 0 and (g1 < 0 or (g1 == 0 and g0 <= 0)):
+                    continue
+                k = 2*math.pi*(g0*gvecs[0] + g1*gvecs[1] + g2*gvecs[2])
+                ksq = (k**2).sum()
+                if ksq > (2*math.pi*gcut)**2:
+                    continue
+                c = 4*math.pi/vol*math.exp(-ksq/(4*alpha*alpha))/ksq
+                out.append((g0, g1, g2, k[0], k[1], k[2], c))
+    return gmax, np.array(out)
+
+
+@pytest.mark.parametrize('angles', [(90.0, 90.0, 90.0), (84.0, 97.0, 101.0)])
+def test_kvectors_brute_force(angles):
+    from yaff_amd import workloads
+    from yaff_amd.engine import ewald_kvectors
+    s = workloads.rocksalt((2, 3, 2), angles=angles)
+    sp = s.spec
+    gmax, g, kv = ewald_kvectors(s.cell, sp.alpha, sp.gcut)
+    gmax_b, ref = _brute_kvectors(s.cell.rvecs, sp.alpha, sp.gcut)
+    assert (gmax == gmax_b).all()
+    assert (g == ref[:, :3].astype(int)).all()
+    scale = np.abs(ref[:, 3:6]).max()
+    assert np.abs(kv[:, :3] - ref[:, 3:6]).max() <= 1e-13*scale
+    assert np.allclose(kv[:, 3], ref[:, 6], rtol=1e-12, atol=0)
+
+
+def test_ewald_local_plan_covers_every_k_and_atom():
+    from yaff_amd import workloads
+    from yaff_amd.batch import nbsystem
+    from yaff_amd.engine import _ewald_local_plan
+    s = workloads.rocksalt((3, 2, 2), angles=(88.0, 93.0, 95.0))
+    nb = nbsystem(s)
+    nb.ewald.nlow, nb.ewald.nhigh = 5, s.natom - 9
+    for seglen, chunk in ((4, 32), (8, 64), (16, 256)):
+        p = _ewald_local_plan(nb, seglen, chunk)
+        nk = p['nk']
+        # segments tile the k vectors: consecutive g0 within one row, at most seglen long
+        covered = np.zeros(nk, int)
+        for (g0, g1, g2, k), ln in zip(p['seg'], p['seg_len']):
+            assert 1 <= ln <= seglen
+            covered[k:k + ln] += 1
+            g = p['gpad'][k:k + ln, :3]
+            assert (g[:, 0] == g0 + np.arange(ln)).all() and (g[:, 1] == g1).all() and (g[:, 2] == g2).all()
+        assert (covered == 1).all()
+        # rows: starts flagged in the k table
+        assert p['gpad'][:, 3].sum() == p['nrow'] == len(p['krow'])
+        # chunks tile the atoms, with the nlow/nhigh groups
+        cov = np.zeros(s.natom, int)
+        for (_, a, ln, grp) in p['chunks']:
+            assert 1 <= ln <= chunk
+            cov[a:a + ln] += 1
+            want = 1 if a < 5 else (2 if a >= s.natom - 9 else 0)
+            assert grp == want and (a + ln <= 5 or a >= 5) and (a + ln <= s.natom - 9 or a >= s.natom - 9)
+        assert (cov == 1).all()
+        # work items: every (block of 64 segments, chunk) pair once
+        pairs = set((int(w[1]), int(w[2])) for w in p['work'])
+        nseg = len(p['seg'])
+        assert pairs == set(itertools.product(range(0, nseg, 64), range(len(p['chunks']))))
+        assert all(w[3] == min(64, nseg - w[1]) for w in p['work'])
+
+
+def test_scalings_match_oracle_table():
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), 'golden'))
+    import cases
+    from oracle.oracle import scaling_table
+    import yaff_amd as ya
+    f = cases.fixture_arrays('water32')
+    system = ya.System(f['numbers'], f['pos'], bonds=f['bonds'], rvecs=f['rvecs'])
+    for sc in [(0.0, 0.0, 0.5), (0.2, 0.4, 0.8, 0.9), (1.0, 1.0, 1.0)]:
+        mine = ya.Scalings(system, *sc).stab
+        ref = scaling_table(system.natom, f['bonds'], *sc)
+        assert mine.tobytes() == np.asarray(ref, dtype=mine.dtype).tobytes()
+    with pytest.raises(ValueError):
+        ya.Scalings(system, 1.5)
+
+
+def test_neighborlist_rmax_and_arguments():
+    import yaff_amd as ya
+    from yaff_amd import workloads
+    s = workloads.lj_fluid((2, 2, 2), angles=(80.0, 95.0, 100.0))
+    nl = ya.NeighborList(s, skin=1.0)
+    nl.request_rcut(10.0)
+    nl.request_rcut(7.0)
+    assert nl.rcut == 10.0
+    assert (nl.rmax == np.ceil(11.0/s.cell.rspacings - 0.5).astype(int)).all()
+    with pytest.raises(ValueError):
+        ya.NeighborList(s, skin=-1)
+    with pytest.raises(ValueError):
+        ya.NeighborList(s, nlow=-1)
+    with pytest.raises(ValueError):
+        ya.NeighborList(s, nlow=10, nhigh=5)
+
+
+def test_workloads():
+    from yaff_amd import workloads
+    from yaff_amd.units import angstrom
+    s = workloads.rocksalt((12, 12, 6))
+    assert s.natom == 6912 and abs(s.charges.sum()) == 0
+    l = workloads.lj_fluid()
+    assert l.natom == 864 and abs(l.cell.volume - (34.7*angstrom)**3) < 1e-6*l.cell.volume
+    for n, per in ((200, 4), (6000, 8), (50000, 8), (100, 4)):
+        r = workloads.natom_to_reps(n, per)
+        assert abs(np.prod(r)*per - n) <= 0.25*n
+    b = workloads.batch_of_systems(8, (200, 6000), (80.0, 100.0), seed=3)
+    assert [x.spec.kind for x in b] == ['lj', 'ionic']*4
+    for x in b:
+        ang = np.degrees(np.arccos([np.dot(x.cell.rvecs[1], x.cell.rvecs[2]),
+                                    np.dot(x.cell.rvecs[0], x.cell.rvecs[2]),
+                                    np.dot(x.cell.rvecs[0], x.cell.rvecs[1])]
+                                   / np.array([np.linalg.norm(x.cell.rvecs[i])*np.linalg.norm(x.cell.rvecs[j])
+                                               for i, j in ((1, 2), (0, 2), (0, 1))])))
+        assert ((ang > 79.999) & (ang < 100.001)).all()
+
+
+def test_poisson_stream_and_admission():
+    from yaff_amd import serve
+    rs = serve.poisson_requests(400, 50.0, natom_range=(100, 800), seed=2)
+    gaps = np.diff([0.0] + [r.arrival for r in rs])
+    assert abs(gaps.mean()*50.0 - 1.0) < 0.15
+    assert 0.35 < np.mean([r.kind == 'md' for r in rs]) < 0.65
+    assert all(r.iters_total == (21 if r.kind == 'md' else 1) for r in rs)
+    # admission: FIFO while the estimated batch footprint fits the budget
+    few = rs[:6]
+    one = max(serve.estimate_bytes([r.nb]) for r in few)
+    srv = serve.Server(precision='fp64', hbm_budget=int(2.2*one), max_systems=64)
+    srv.waiting.extend(few)
+    srv._admit(0.0)
+    assert 1 <= len(srv.active) < len(few)
+    assert serve.estimate_bytes([r.nb for r in srv.active]) <= srv.hbm_budget
+    assert [r.rid for r in srv.active] == [r.rid for r in few[:len(srv.active)]]
+    # the estimate grows with every member
+    sizes = [serve.estimate_bytes([r.nb for r in few[:k]]) for k in range(1, 7)]
+    assert all(b > a for a, b in zip(sizes, sizes[1:]))
+    # a request that can never fit is refused loudly
+    srv = serve.Server(precision='fp64', hbm_budget=1000)
+    srv.waiting.append(rs[0])
+    with pytest.raises(RuntimeError):
+        srv._admit(0.0)

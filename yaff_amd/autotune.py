@@ -1,0 +1,98 @@
+"""Autotuner of the launch parameters of a batch.
+
+Parameters: lanes per atom of the pair kernel (16/32/64), k vectors per
+segment of the structure-factor pass (4/8/16), atoms per structure-factor
+chunk (32..256) and the number of k-row parts of the force pass.  Each
+parameter affects a known set of kernels; coordinate descent times only
+those kernels (HIP events on the launch stream, median of a few runs) with
+the other parameters fixed.  Results are cached in a JSON file keyed by
+architecture, precision, potential mix and size bucket.
+"""
+import json
+import math
+import os
+
+import torch
+
+from . import _lib
+
+__all__ = ['tune', 'cache_key']
+
+CANDIDATES = [
+    ('pair_lanes', (16, 32, 64), ('pair',)),
+    ('seglen', (4, 8, 16), ('ewald_sk', 'ewald_kreduce')),
+    ('ewald_chunk', (32, 64, 128, 256), ('ewald_sk', 'ewald_kreduce')),
+    ('kchunks', None, ('ewald_force',)),
+]
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'yaff_amd', 'autotune.json')
+
+
+def cache_key(engine):
+    try:
+        arch = torch.cuda.get_device_properties(0).gcnArchName.split(':')[0]
+    except Exception:
+        arch = 'gfx950'
+    kinds = sorted(set(tuple(sorted(p.kind for p in s.pots)) + (('ewald',) if s.ewald else ())
+                       for s in engine.systems))
+    natom = sum(s.natom for s in engine.systems)
+    return '%s|%s|%s|n%d|s%d' % (arch, 'fp64' if engine.precision == _lib.FP64 else 'mixed', kinds,
+                                 int(round(math.log2(max(natom, 1)))), int(round(math.log2(max(engine.nsys, 1)))))
+
+
+def _time_kernels(engine, names, reps=3):
+    """Median over reps of the summed duration of the named kernels in one
+    force evaluation (the list is built once, outside the timing)."""
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        _lib.profile_enable(True)
+        engine.pair(gpos=True, accumulate=False)
+        engine.ewald_reci(gpos=True, accumulate=True)
+        prof = _lib.profile_read()
+        _lib.profile_enable(False)
+        times.append(sum(ms for n, ms in prof if n in names))
+    times.sort()
+    return times[len(times)//2]
+
+
+def tune(engine, cache=DEFAULT_CACHE, verbose=False):
+    """Coordinate descent over CANDIDATES; applies and returns the best
+    parameters (loaded from the cache when the key is known)."""
+    key = cache_key(engine)
+    table = {}
+    if cache and os.path.exists(cache):
+        try:
+            with open(cache) as f:
+                table = json.load(f)
+        except (OSError, ValueError):
+            table = {}
+    if key in table:
+        engine.set_tuning(**table[key])
+        return dict(table[key])
+    engine.ensure_nlist()
+    best = {}
+    for name, values, kernels in CANDIDATES:
+        if name != 'pair_lanes' and not engine.has_ewald:
+            continue
+        if values is None:  # kchunks: around the current choice
+            k0 = engine.kchunks
+            values = sorted(set(max(1, int(k0*f)) for f in (0.25, 0.5, 1.0, 2.0)))
+        results = []
+        for v in values:
+            engine.set_tuning(**{name: v})
+            results.append((_time_kernels(engine, kernels), v))
+            if verbose:
+                print('autotune %s=%s: %.1f us' % (name, v, results[-1][0]*1e3), flush=True)
+        t, v = min(results)
+        engine.set_tuning(**{name: v})
+        best[name] = v
+    if cache:
+        table[key] = best
+        try:
+            os.makedirs(os.path.dirname(cache), exist_ok=True)
+            with open(cache, 'w') as f:
+                json.dump(table, f, indent=1, sort_keys=True)
+        except OSError:
+            pass
+    return best

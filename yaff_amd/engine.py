@@ -85,11 +85,10 @@ def ewald_kvectors(cell, alpha, gcut, gmax=None):
     if gmax is None:
         gmax = np.ceil(gcut/cell.gspacings - 0.5).astype(np.int64)
     kv = TWO_PI*cell.gvecs_full
-    g0, g1, g2 = np.meshgrid(np.arange(-gmax[0], gmax[0] + 1), np.arange(-gmax[1], gmax[1] + 1),
-                             np.arange(0, gmax[2] + 1), indexing='ij')
+    # g2 outermost, g0 innermost: already the row order
+    g2, g1, g0 = np.meshgrid(np.arange(0, gmax[2] + 1), np.arange(-gmax[1], gmax[1] + 1),
+                             np.arange(-gmax[0], gmax[0] + 1), indexing='ij')
     g = np.stack([g0.ravel(), g1.ravel(), g2.ravel()], axis=1)
-    order = np.lexsort((g[:, 0], g[:, 1], g[:, 2]))
-    g = g[order]
     half = (g[:, 2] > 0) | ((g[:, 2] == 0) & ((g[:, 1] > 0) | ((g[:, 1] == 0) & (g[:, 0] > 0))))
     g = g[half]
     gf = g.astype(np.float64)
@@ -105,6 +104,69 @@ def ewald_kvectors(cell, alpha, gcut, gmax=None):
     fac2 = 0.25/alpha/alpha
     c = fac1*np.exp(-ksq*fac2)/ksq
     return gmax, g, np.stack([kx, ky, kz, c], axis=1)
+
+
+def ewald_kvectors_cached(s):
+    """ewald_kvectors of an NBSystem, cached on it."""
+    ew = s.ewald
+    cache = s.__dict__.setdefault('_plan_cache', {})
+    kkey = ('kv', s.cell.rvecs_full.tobytes(), ew.alpha, ew.gcut, None if ew.gmax is None else tuple(ew.gmax))
+    if kkey not in cache:
+        cache.clear()
+        cache[kkey] = ewald_kvectors(s.cell, ew.alpha, ew.gcut, ew.gmax)
+    return cache[kkey]
+
+
+def _ewald_local_plan(s, seglen, chunk_atoms):
+    """The Ewald tables of one system with system-local offsets, cached on
+    the NBSystem (keyed by the cell, the Ewald settings and the launch
+    parameters) so that batches can be re-formed cheaply."""
+    ew = s.ewald
+    gmax, g, kv = ewald_kvectors_cached(s)
+    cache = s._plan_cache
+    kkey = next(k for k in cache if k[0] == 'kv')
+    pkey = ('plan', kkey, seglen, chunk_atoms)
+    if pkey in cache:
+        return cache[pkey]
+    nk = len(g)
+    nkp = max(_round_up(nk, KBLOCK), KBLOCK)
+    kpad = np.zeros((nkp, 4))
+    kpad[:nk] = kv
+    rows = _runs(g)
+    st, cnt = (rows[:, 0], rows[:, 1]) if len(rows) else (np.zeros(0, np.int64), np.zeros(0, np.int64))
+    gpad = np.zeros((nkp, 4), np.int32)
+    gpad[:nk, :3] = g
+    gpad[st, 3] = 1
+    krow = np.stack([g[st, 1], g[st, 2], g[st, 0], cnt], axis=1).astype(np.int32) if len(st) else \
+        np.zeros((0, 4), np.int32)
+    # segments of at most seglen consecutive k along each row
+    nsr = (cnt + seglen - 1)//seglen
+    srow = np.repeat(np.arange(len(st)), nsr)
+    first = np.repeat(np.cumsum(nsr) - nsr, nsr)
+    off = (np.arange(len(srow)) - first)*seglen
+    kk = st[srow] + off
+    seg = np.stack([g[kk, 0], g[st[srow], 1], g[st[srow], 2], kk], axis=1).astype(np.int32) if len(kk) else \
+        np.zeros((0, 4), np.int32)
+    seg_len = np.minimum(seglen, cnt[srow] - off).astype(np.int32)
+    # atom chunks: below nlow (group 1), [nlow, nhigh) (group 0), above nhigh (group 2)
+    nhigh = s.natom if ew.nhigh == -1 else ew.nhigh
+    nlow = ew.nlow
+    parts = []
+    for (lo, hi, grp) in ((0, nlow, 1), (nlow, nhigh, 0), (nhigh, s.natom, 2)):
+        a = np.arange(lo, hi, chunk_atoms)
+        parts.append(np.stack([np.zeros_like(a), a, np.minimum(chunk_atoms, hi - a), np.full_like(a, grp)], axis=1))
+    chunks = np.concatenate(parts).astype(np.int32)
+    nseg = len(seg)
+    sb = np.arange(0, nseg, 64)
+    wsb, wc = np.meshgrid(sb, np.arange(len(chunks)), indexing='ij')
+    work = np.stack([np.zeros(wsb.size, np.int64), wsb.ravel(), wc.ravel(), np.minimum(64, nseg - wsb.ravel())],
+                    axis=1).astype(np.int32)
+    plan = dict(gmax=gmax, nk=nk, nkp=nkp, kpad=kpad, gpad=gpad, krow=krow, krow_k=st.astype(np.int32),
+                seg=seg, seg_len=seg_len, chunks=chunks, work=work, nrow=len(st),
+                grouped=bool(nlow > 0 or nhigh < s.natom),
+                nchunk_low=int((chunks[:, 3] == 1).sum()), nchunk_high=int((chunks[:, 3] == 2).sum()))
+    cache[pkey] = plan
+    return plan
 
 
 def _runs(g):
@@ -126,7 +188,7 @@ class Engine(object):
     """Device state and launches for a batch of NBSystem objects."""
 
     def __init__(self, systems, precision='fp64', seglen=None, kchunks=None, kcap=None, device='cuda',
-                 pair_lanes=32, ewald_chunk=None):
+                 pair_lanes=32, ewald_chunk=None, upload=True):
         self.lib = _lib.load()
         if not torch.cuda.is_available():
             raise RuntimeError('yaff_amd.Engine needs a HIP device (torch.cuda.is_available() is False)')
@@ -139,7 +201,8 @@ class Engine(object):
         self._layout()
         self._plan_sys(seglen, kchunks)
         self._alloc(kcap)
-        self.upload_positions()
+        if upload:
+            self.upload_positions()
         self.nbuilds = 0
 
     # ------------------------------------------------------------------
@@ -215,20 +278,22 @@ class Engine(object):
             self._plan_ewald(seglen, kchunks)
 
     def _plan_ewald(self, seglen, kchunks):
-        """k rows, segments, atom chunks and the work list of k_ewald_sk."""
-        nk_all = 0
-        for s in self.systems:
-            if s.ewald is not None:
-                _, g, _ = ewald_kvectors(s.cell, s.ewald.alpha, s.ewald.gcut, s.ewald.gmax)
-                nk_all = max(nk_all, len(g))
+        """k rows, segments, atom chunks and the work list of k_ewald_sk: the
+        cached per-system tables joined with their batch offsets."""
+        ew_sys = [i for i, s in enumerate(self.systems) if s.ewald is not None and s.natom > 0]
+        for i in ew_sys:
+            if self.systems[i].cell.nvec != 3:
+                raise ValueError('Ewald reciprocal needs a 3D periodic cell')
         if seglen is None:
+            nk_all = max([len(ewald_kvectors_cached(self.systems[i])[1]) for i in ew_sys] + [0])
             seglen = 16 if nk_all > 20000 else 8
         self.seglen = seglen
-        natom_tot = sum(s.natom for s in self.systems if s.ewald is not None)
+        natom_tot = sum(self.systems[i].natom for i in ew_sys)
         if self._ewald_chunk_req is not None:
             self.ewald_chunk = int(self._ewald_chunk_req)
         else:
             # enough (segment group, chunk) work items to give every SIMD a few waves
+            nk_all = max([len(ewald_kvectors_cached(self.systems[i])[1]) for i in ew_sys] + [0])
             groups = max(1, (nk_all//seglen + 63)//64)
             want = 4096
             chunk = CHUNK
@@ -239,81 +304,69 @@ class Engine(object):
         kvec, krow, krow_k, seg, seg_len, chunk, work, kblk_sys, kg = [], [], [], [], [], [], [], [], []
         k0 = row0 = seg0 = chunk0 = 0
         spart0 = 0
-        self.gmax = []
+        self.gmax = [None]*self.nsys
+        self.groups = 0
         for i, s in enumerate(self.systems):
             h = self.hsys[i]
             if s.ewald is None or s.natom == 0:
                 h.nk = 0
                 h.k0 = k0
-                self.gmax.append(None)
                 continue
             ew = s.ewald
-            if s.cell.nvec != 3:
-                raise ValueError('Ewald reciprocal needs a 3D periodic cell')
-            gmax, g, kv = ewald_kvectors(s.cell, ew.alpha, ew.gcut, ew.gmax)
-            self.gmax.append(gmax)
-            nk = len(g)
-            nkp = max(_round_up(nk, KBLOCK), KBLOCK)
-            kpad = np.zeros((nkp, 4))
-            kpad[:nk] = kv
-            kvec.append(kpad)
-            kblk_sys.extend([i]*(nkp//KBLOCK))
-            rows = _runs(g)
-            gpad = np.zeros((nkp, 4), np.int32)
-            gpad[:nk, :3] = g
-            for (st, cnt) in rows:
-                krow.append((g[st, 1], g[st, 2], g[st, 0], cnt))
-                krow_k.append(k0 + st)
-                gpad[st, 3] = 1
-            kg.append(gpad)
-            segs = []
-            for (st, cnt) in rows:
-                for off in range(0, cnt, seglen):
-                    ln = min(seglen, cnt - off)
-                    segs.append((g[st + off, 0], g[st, 1], g[st, 2], k0 + st + off, ln))
-            for x in segs:
-                seg.append(x[:4])
-                seg_len.append(x[4])
-            nhigh = s.natom if ew.nhigh == -1 else ew.nhigh
-            nlow = ew.nlow
-            if nlow > 0 or nhigh < s.natom:
+            p = _ewald_local_plan(s, self.seglen, self.ewald_chunk)
+            self.gmax[i] = p['gmax']
+            nkp = p['nkp']
+            kvec.append(p['kpad'])
+            kg.append(p['gpad'])
+            kblk_sys.append(np.full(nkp//KBLOCK, i, np.int32))
+            krow.append(p['krow'])
+            krow_k.append(p['krow_k'] + k0)
+            sg = p['seg'].copy()
+            sg[:, 3] += k0
+            seg.append(sg)
+            seg_len.append(p['seg_len'])
+            ch = p['chunks'].copy()
+            ch[:, 0] = i
+            chunk.append(ch)
+            wk = p['work'].copy()
+            wk[:, 0] = i
+            wk[:, 1] += seg0
+            wk[:, 2] += chunk0
+            work.append(wk)
+            if p['grouped']:
                 self.groups = 1
-            chunks = []
-            for (lo, hi, grp) in ((0, nlow, 1), (nlow, nhigh, 0), (nhigh, s.natom, 2)):
-                for a in range(lo, hi, self.ewald_chunk):
-                    chunks.append((i, a, min(self.ewald_chunk, hi - a), grp))
-            chunk.extend(chunks)
-            nseg = len(segs)
-            for sb in range(0, nseg, 64):
-                for c in range(len(chunks)):
-                    work.append((i, seg0 + sb, chunk0 + c, min(64, nseg - sb)))
+            nseg = len(sg)
+            nch = len(ch)
             h.alpha, h.dielectric = ew.alpha, ew.dielectric
-            h.nlow, h.nhigh = (h.nlow, h.nhigh)
             h.k0, h.nk = k0, nkp
-            h.row0, h.nrow = row0, len(rows)
+            h.row0, h.nrow = row0, p['nrow']
             h.seg0, h.nseg = seg0, nseg
-            h.chunk0, h.nchunk = chunk0, len(chunks)
+            h.chunk0, h.nchunk = chunk0, nch
             h.spart0 = spart0
-            h.nchunk_low = sum(1 for c in chunks if c[3] == 1)
-            h.nchunk_high = sum(1 for c in chunks if c[3] == 2)
-            spart0 += len(chunks)*nkp
+            h.nchunk_low = p['nchunk_low']
+            h.nchunk_high = p['nchunk_high']
+            spart0 += nch*nkp
             k0 += nkp
-            row0 += len(rows)
+            row0 += p['nrow']
             seg0 += nseg
-            chunk0 += len(chunks)
+            chunk0 += nch
+
+        def cat(parts, empty):
+            parts = [x for x in parts if len(x)]
+            return np.ascontiguousarray(np.concatenate(parts)) if parts else empty
         self.ewald_h = dict(
-            kvec=np.concatenate(kvec) if kvec else np.zeros((KBLOCK, 4)),
-            krow=np.array(krow, np.int32).reshape(-1, 4) if krow else np.zeros((1, 4), np.int32),
-            krow_k=np.array(krow_k, np.int32) if krow_k else np.zeros(1, np.int32),
-            seg=np.array(seg, np.int32).reshape(-1, 4) if seg else np.zeros((1, 4), np.int32),
-            seg_len=np.array(seg_len, np.int32) if seg_len else np.zeros(1, np.int32),
-            chunk=np.array(chunk, np.int32).reshape(-1, 4) if chunk else np.zeros((1, 4), np.int32),
-            work=np.array(work, np.int32).reshape(-1, 4) if work else np.zeros((1, 4), np.int32),
-            kblk_sys=np.array(kblk_sys, np.int32) if kblk_sys else np.zeros(1, np.int32),
-            kg=np.concatenate(kg) if kg else np.zeros((KBLOCK, 4), np.int32),
+            kvec=cat(kvec, np.zeros((KBLOCK, 4))),
+            krow=cat(krow, np.zeros((1, 4), np.int32)),
+            krow_k=cat(krow_k, np.zeros(1, np.int32)),
+            seg=cat(seg, np.zeros((1, 4), np.int32)),
+            seg_len=cat(seg_len, np.zeros(1, np.int32)),
+            chunk=cat(chunk, np.zeros((1, 4), np.int32)),
+            work=cat(work, np.zeros((1, 4), np.int32)),
+            kblk_sys=cat(kblk_sys, np.zeros(1, np.int32)),
+            kg=cat(kg, np.zeros((KBLOCK, 4), np.int32)),
         )
         self.nk_tot = k0
-        self.nwork = len(work)
+        self.nwork = int(sum(len(w) for w in work))
         self.spart_entries = max(spart0, 1)
         if kchunks is None:
             nwaves = self.npad//64
@@ -360,22 +413,7 @@ class Engine(object):
                 charges[self.atom0[i]:self.atom0[i] + s.natom] = s.charges
         self.charges_d = self._t(charges)
         if self.has_ewald:
-            e = self.ewald_h
-            self.ew_tabs = {k: self._t(v) for k, v in e.items()}
-            self.ew_frac_d = torch.zeros(4*self.npad, dtype=torch.float64, device=dev)
-            self.ew_step_d = torch.zeros(2*self.npad, dtype=torch.float64, device=dev)
-            self.ew_spart_d = torch.zeros(2*self.spart_entries, dtype=torch.float64, device=dev)
-            self.ew_coef_d = torch.zeros(6*max(self.nk_tot, KBLOCK), dtype=torch.float64, device=dev)
-            self.ew_kpart_d = torch.zeros(8*max(self.nk_tot//32, 1), dtype=torch.float64, device=dev)
-            self.ew_gpart_d = torch.zeros(self.kchunks*self.npad*3, dtype=torch.float64, device=dev)
-            t = self.ew_tabs
-            self.ewald = YnbEwald(t['kvec'].data_ptr(), t['krow'].data_ptr(), t['krow_k'].data_ptr(),
-                                  t['seg'].data_ptr(), t['seg_len'].data_ptr(), t['chunk'].data_ptr(),
-                                  t['work'].data_ptr(), t['kblk_sys'].data_ptr(), t['kg'].data_ptr(), self.nwork,
-                                  self.nk_tot,
-                                  self.seglen, self.kchunks, self.groups, 0, self.ew_frac_d.data_ptr(),
-                                  self.ew_step_d.data_ptr(), self.ew_spart_d.data_ptr(), self.ew_coef_d.data_ptr(),
-                                  self.ew_kpart_d.data_ptr(), self.ew_gpart_d.data_ptr())
+            self._alloc_ewald()
         # MD state
         masses = np.ones(self.npad)
         for i, s in enumerate(self.systems):
@@ -399,6 +437,49 @@ class Engine(object):
     def _alloc_nlist_work(self):
         self.nl_work_bytes = self.lib.ynb_nlist_workspace_bytes(self.npad, self.nbin, self.bin_cap)
         self.nl_work_d = torch.empty(self.nl_work_bytes, dtype=torch.uint8, device=self.device)
+
+    def _alloc_ewald(self):
+        dev = self.device
+        e = self.ewald_h
+        self.ew_tabs = {k: self._t(v) for k, v in e.items()}
+        self.ew_frac_d = torch.zeros(4*self.npad, dtype=torch.float64, device=dev)
+        self.ew_step_d = torch.zeros(2*self.npad, dtype=torch.float64, device=dev)
+        self.ew_spart_d = torch.zeros(2*self.spart_entries, dtype=torch.float64, device=dev)
+        self.ew_coef_d = torch.zeros(6*max(self.nk_tot, KBLOCK), dtype=torch.float64, device=dev)
+        self.ew_kpart_d = torch.zeros(8*max(self.nk_tot//32, 1), dtype=torch.float64, device=dev)
+        self.ew_gpart_d = torch.zeros(self.kchunks*self.npad*3, dtype=torch.float64, device=dev)
+        t = self.ew_tabs
+        self.ewald = YnbEwald(t['kvec'].data_ptr(), t['krow'].data_ptr(), t['krow_k'].data_ptr(),
+                              t['seg'].data_ptr(), t['seg_len'].data_ptr(), t['chunk'].data_ptr(),
+                              t['work'].data_ptr(), t['kblk_sys'].data_ptr(), t['kg'].data_ptr(), self.nwork,
+                              self.nk_tot, self.seglen, self.kchunks, self.groups, 0, self.ew_frac_d.data_ptr(),
+                              self.ew_step_d.data_ptr(), self.ew_spart_d.data_ptr(), self.ew_coef_d.data_ptr(),
+                              self.ew_kpart_d.data_ptr(), self.ew_gpart_d.data_ptr())
+
+    def tuning(self):
+        """The launch parameters an autotuner may change."""
+        return {'pair_lanes': self.pair_lanes, 'seglen': self.seglen if self.has_ewald else None,
+                'ewald_chunk': getattr(self, 'ewald_chunk', None), 'kchunks': getattr(self, 'kchunks', None)}
+
+    def set_tuning(self, pair_lanes=None, seglen=None, ewald_chunk=None, kchunks=None):
+        """Changes launch parameters in place (results are unchanged up to
+        summation order).  Ewald tables are re-planned when their parameters
+        change."""
+        if pair_lanes is not None:
+            if pair_lanes not in (16, 32, 64):
+                raise ValueError('pair_lanes must be 16, 32 or 64')
+            self.pair_lanes = int(pair_lanes)
+            self.pairpar.lanes = self.pair_lanes
+        if self.has_ewald and (seglen is not None or ewald_chunk is not None or kchunks is not None):
+            if ewald_chunk is not None:
+                self._ewald_chunk_req = int(ewald_chunk)
+            else:
+                self._ewald_chunk_req = self.ewald_chunk
+            self._plan_ewald(seglen if seglen is not None else self.seglen,
+                             kchunks if kchunks is not None else self.kchunks)
+            self.sys_d = self._t(np.frombuffer(bytes(self.hsys), np.uint8))
+            self.batch.sys = self.sys_d.data_ptr()
+            self._alloc_ewald()
 
     def _estimate_kcap(self):
         est = 16
@@ -522,8 +603,8 @@ class Engine(object):
         self.vel_d.copy_(torch.from_numpy(host))
 
     def set_timestep(self, dt):
-        dt = np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.nsys,))
-        self.dt_d.copy_(torch.from_numpy(np.ascontiguousarray(dt)))
+        dt = np.array(np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.nsys,)))
+        self.dt_d.copy_(torch.from_numpy(dt))
 
     def md_step(self):
         """kick, drift, neighbour list, forces, kick: one Verlet step of every
