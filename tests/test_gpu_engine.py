@@ -141,3 +141,27 @@ def test_mixed_precision_error(name):
     assert abs(e - e_ref) <= MIXED_REL_TOL*abs(e_ref)
     assert rms <= MIXED_FORCE_RMS_TOL
     assert np.abs(v - v_ref).max() <= MIXED_REL_TOL*np.abs(v_ref).max()
+
+
+@pytest.mark.parametrize('name', sorted(CASES))
+def test_total_fp64(name):
+    """compute(): all potentials summed per pair (the MD and batch path) plus
+    the Ewald reciprocal term, against the sum of the reference parts."""
+    print('case', name, flush=True)
+    c = CASES[name]
+    n = len(c['pos'])
+    eng = _engine(c)
+    eng.ensure_nlist()
+    eng.compute(build=False)
+    e, g, v = eng.results(0)
+    e_ref = sum(float(REF['%s/pot%d/e' % (name, ip)]) for ip in range(len(c['pots'])))
+    g_ref = sum(REF['%s/pot%d/gpos' % (name, ip)] for ip in range(len(c['pots'])))
+    v_ref = sum(REF['%s/pot%d/vtens' % (name, ip)] for ip in range(len(c['pots'])))
+    if c['ewald'] is not None:
+        e_ref += float(REF[name + '/reci/e'])
+        g_ref = g_ref + REF[name + '/reci/gpos']
+        v_ref = v_ref + REF[name + '/reci/vtens']
+    scale_e = sum(abs(float(REF['%s/pot%d/e' % (name, ip)])) for ip in range(len(c['pots'])))
+    assert abs(e - e_ref) <= TOL*scale_e, (e, e_ref)
+    assert rel(g, g_ref) <= TOL
+    assert rel(v, v_ref) <= TOL

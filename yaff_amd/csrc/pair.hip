@@ -15,23 +15,48 @@
 // pos_j - pos_i + n.R in FP64 and evaluates the potentials in FP32,
 // accumulating gradient, energy and virial in FP64.
 #include "common.h"
+#include "erfcx_table.h"
 
 namespace ynb {
 namespace {
 
 constexpr double TWO_DIV_SQRT_PI = 1.1283791670955125739;  // yaff/pes/constants.h:31
 
-// Transcendentals of the two precisions.  FP64 mode calls the FP64 device
-// library; mixed mode keeps every argument reduction in FP64 and evaluates
-// only a well-conditioned remainder in FP32:
+// Transcendentals of the two precisions.
+//
+// FP64: exp from the device library; erfc(x) = exp(-x^2) erfcx(x) with erfcx
+// a Chebyshev series (25 terms, relative error < 3e-14 on [0, 27], table
+// generated by gen_erfcx_table.py) in t = 2/(2 + x), so that erfc and the EI
+// gradient share one exp(-x^2) and the evaluation needs few registers.
+//
+// Mixed: every argument reduction in FP64, only a well-conditioned
+// remainder in FP32:
 //   exp(a)   = 2^k * exp2f(f),  k = rint(a log2 e), f = a log2 e - k in FP64;
 //   erfc(x)  = exp(-x^2) * erfcxf(x), erfcx = exp(x^2) erfc(x) being smooth,
 //              so rounding x to FP32 barely moves it.
+__device__ inline double erfcx_cheb(double x) {  // x >= 0
+  const double t = 2.0 / (2.0 + x);
+  const double y = 2.0 * t - 1.0;
+  const double y2 = 2.0 * y;
+  double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+  for (int k = ERFCX_NCOEF - 1; k >= 1; --k) {
+    const double b0 = fma(y2, b1, ERFCX_COEF[k] - b2);
+    b2 = b1;
+    b1 = b0;
+  }
+  return fma(y, b1, ERFCX_COEF[0] - b2);
+}
+
 template <int PREC>
 struct Tr {
   __device__ static double ex(double a) { return exp(a); }
-  __device__ static double erfc_(double x) { return erfc(x); }
-  __device__ static double erf_(double x) { return erf(x); }
+  // erfc(x) given e2 = exp(-x^2)
+  __device__ static double erfc_e2(double x, double e2) {
+    return x >= 0.0 ? e2 * erfcx_cheb(x) : 2.0 - e2 * erfcx_cheb(-x);
+  }
+  __device__ static double erfc_(double x) { return erfc_e2(x, exp(-x * x)); }
+  __device__ static double erf_(double x) { return 1.0 - erfc_(x); }
 };
 template <>
 struct Tr<YNB_MIXED> {
@@ -42,6 +67,7 @@ struct Tr<YNB_MIXED> {
   }
   __device__ static double erfc_(double x) { return ex(-x * x) * (double)erfcxf((float)x); }
   __device__ static double erf_(double x) { return 1.0 - erfc_(x); }
+  __device__ static double erfc_e2(double x, double e2) { return e2 * (double)erfcxf((float)x); }
 };
 
 template <int PREC>
@@ -134,6 +160,42 @@ struct PotEval {
   }
 };
 
+// The same potentials for the neighbour-list kernel, written around one
+// reciprocal distance inv = 1/d per pair (the reference divides by d in every
+// term).  LJ takes sqrt(epsilon) per atom (precomputed on the host), so
+// eps_ab = se_a * se_b instead of sqrt(eps_a * eps_b); EI takes 1/dielectric
+// and shares exp(-alpha^2 d^2) between erfc and the gradient.  These change
+// roundings only (relative 1e-16 per term).
+template <int PREC>
+struct PotFast {
+  using R = Tr<PREC>;
+  __device__ static double lj(double sa, double sb, double eps, double inv, double inv2, double &g) {
+    const double sigma = 0.5 * (sa + sb);
+    double x = sigma * inv;
+    x *= x;
+    x *= x * x;
+    g = 24.0 * eps * inv2 * x * (1.0 - 2.0 * x);
+    return 4.0 * eps * (x * (x - 1.0));
+  }
+  __device__ static double exprep(double amp, double b, double d, double inv, double &g) {
+    if (amp == 0.0 || b == 0.0) {
+      g = 0.0;
+      return 0.0;
+    }
+    const double e = amp * R::ex(-b * d);
+    g = -e * b * inv;
+    return e;
+  }
+  // alpha > 0, point charges (the common case); qq = qa qb / dielectric
+  __device__ static double ei_point(double qq, double alpha, double d, double inv, double inv2, double &g) {
+    const double x = alpha * d;
+    const double e2 = R::ex(-x * x);
+    const double e = qq * R::erfc_e2(x, e2) * inv;
+    g = (-TWO_DIV_SQRT_PI * alpha * e2 * qq - e) * inv2;
+    return e;
+  }
+};
+
 // pair_pot.c:65-81 as a binary search over the system's sorted rows.
 __device__ inline double stab_scale(const ynb_scaling_row *st, int n, int64_t a, int64_t b) {
   int lo = 0, hi = n;
@@ -165,18 +227,24 @@ __device__ inline T group_sum(T v) {
 // scaled (bonded) pair -- the vector is rebuilt exactly as nlist_build_low
 // does (yaff_row_vector), so the cutoff tests d < rcut and the scaling
 // lookups agree with the reference.  FP64 mode evaluates the potentials in
-// FP64; mixed mode in FP32, accumulating gradient, energy and virial in FP64.
+// FP64; mixed mode takes exp/erfc in FP32 on FP64-reduced arguments,
+// accumulating gradient, energy and virial in FP64.
+//
+// SLOTS: per-potential energy, virial and gradient (the drop-in parts need
+// them): one pass over the row per potential.  Otherwise one pass in which
+// the potentials of a pair are summed first, so that only totals are
+// accumulated (one gradient/virial update per pair); the totals go to
+// potential slot 0 and the other slots get zeros.
 template <int PREC, bool SLOTS, int G>
 __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
                                               ynb_pairpar pp, const double *pos, ynb_nlist nl, uint32_t mask,
                                               double *gpos, int accumulate, double *gpos_slots,
                                               double *partial) {
-  using T = double;
   using PE = PotEval<PREC>;
+  using PF = PotFast<PREC>;
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t gi = t / G;
   const int s = (int)(t % G);
-  const int64_t wave = t >> 6;
   const int lane = threadIdx.x & 63;
   if (gi >= npad) return;  // whole waves: npad * G is a multiple of 64
   const int sid = uniform_int(wave_sys[gi >> 6]);
@@ -187,28 +255,19 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
   const int npot = S.npot;
   double rcmax = 0.0;
   for (int p = 0; p < npot; ++p) rcmax = fmax(rcmax, S.pot[p].rcut);
+  const int ngroup = SLOTS ? npot : 1;
 
-  double acc_e[YNB_MAX_POT], acc_v[YNB_MAX_POT][6], acc_g[SLOTS ? YNB_MAX_POT : 1][3];
-#pragma unroll
-  for (int p = 0; p < YNB_MAX_POT; ++p) {
-    acc_e[p] = 0.0;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) acc_v[p][c] = 0.0;
-  }
-#pragma unroll
-  for (int p = 0; p < (SLOTS ? YNB_MAX_POT : 1); ++p) acc_g[p][0] = acc_g[p][1] = acc_g[p][2] = 0.0;
+  __shared__ double red[4][YNB_MAX_POT * 8];
+  const int wib = threadIdx.x >> 6;
+  double G3[3] = {0.0, 0.0, 0.0};
+  const double pi[3] = {active ? pos[3 * gi] : 0.0, active ? pos[3 * gi + 1] : 0.0, active ? pos[3 * gi + 2] : 0.0};
+  const int n = active ? min(nl.cnt[gi], nl.kcap) : 0;
+  const int32_t *rowj = nl.j + gi * nl.kcap;
+  const int32_t *rowi = nl.img + gi * nl.kcap;
 
-  if (active) {
-    const double pi[3] = {pos[3 * gi], pos[3 * gi + 1], pos[3 * gi + 2]};
-    double par_i[YNB_MAX_POT][2];
-#pragma unroll
-    for (int p = 0; p < YNB_MAX_POT; ++p) {
-      par_i[p][0] = (p < npot) ? pp.par[((int64_t)p * 3 + 0) * npad + gi] : 0.0;
-      par_i[p][1] = (p < npot) ? pp.par[((int64_t)p * 3 + 1) * npad + gi] : 0.0;
-    }
-    const int n = min(nl.cnt[gi], nl.kcap);
-    const int32_t *rowj = nl.j + gi * nl.kcap;
-    const int32_t *rowi = nl.img + gi * nl.kcap;
+  for (int grp = 0; grp < ngroup; ++grp) {
+    const int p_lo = SLOTS ? grp : 0, p_hi = SLOTS ? grp + 1 : npot;
+    double acc_e = 0.0, acc_v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, acc_g[3] = {0.0, 0.0, 0.0};
     for (int k = s; k < n; k += G) {
       const int64_t gj = rowj[k];
       int nn[3];
@@ -243,112 +302,103 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
         }
       }
       if (!(dd < rcmax)) continue;
-      const T d = (T)dd;
-#pragma unroll
-      for (int p = 0; p < YNB_MAX_POT; ++p) {
-        if (p >= npot || !((mask >> p) & 1u)) continue;
+      const double d = dd;
+      const double inv = 1.0 / d;
+      const double inv2 = inv * inv;
+      double vg_sum = 0.0, e_sum = 0.0;
+#pragma unroll 1
+      for (int p = p_lo; p < p_hi; ++p) {
+        if (!((mask >> p) & 1u)) continue;
         const ynb_pot &P = S.pot[p];
-        if (!(dd < P.rcut)) continue;
+        if (!(d < P.rcut)) continue;
         double sc = 1.0;
         if (P.nstab > 0 && r_zero) sc = stab_scale(pp.stab + P.stab0, P.nstab, A, B);
         if (sc == 0.0) continue;
-        T g = T(0), e;
+        const double *par0 = pp.par + (int64_t)p * 3 * npad;
+        const double *par1 = par0 + npad;
+        const double *par2 = par1 + npad;
+        double g = 0.0, e;
         switch (P.kind) {
-          case YNB_POT_LJ:
-            e = PE::lj((T)par_i[p][0], (T)pp.par[((int64_t)p * 3) * npad + gj], (T)par_i[p][1],
-                               (T)pp.par[((int64_t)p * 3 + 1) * npad + gj], d, g);
+          case YNB_POT_LJ:  // par2 = sqrt(epsilon)
+            e = PF::lj(par0[gi], par0[gj], par2[gi] * par2[gj], inv, inv2, g);
             break;
-          case YNB_POT_MM3: {
-            const int pa = (int)pp.par[((int64_t)p * 3 + 2) * npad + gi];
-            const int pb = (int)pp.par[((int64_t)p * 3 + 2) * npad + gj];
-            e = PE::mm3((T)par_i[p][0], (T)pp.par[((int64_t)p * 3) * npad + gj], (T)par_i[p][1],
-                                (T)pp.par[((int64_t)p * 3 + 1) * npad + gj], pa + pb, d, g);
+          case YNB_POT_MM3:
+            e = PE::mm3(par0[gi], par0[gj], par1[gi], par1[gj], (int)par2[gi] + (int)par2[gj], d, g);
             break;
-          }
           case YNB_POT_EXPREP: {
-            const int ta = (int)par_i[p][0];
-            const int tb = (int)pp.par[((int64_t)p * 3) * npad + gj];
             const int nf = P.nffatype;
-            const int64_t ci = P.cross0 + (int64_t)(ta * nf + tb);  // tables are symmetric
-            e = PE::exprep((T)pp.cross[ci], (T)pp.cross[ci + (int64_t)nf * nf], d, g);
+            const int64_t ci = P.cross0 + (int64_t)((int)par0[gi] * nf + (int)par0[gj]);  // symmetric tables
+            e = PF::exprep(pp.cross[ci], pp.cross[ci + (int64_t)nf * nf], d, inv, g);
             break;
           }
-          default:
-            e = PE::ei((T)par_i[p][0], (T)pp.par[((int64_t)p * 3) * npad + gj], (T)par_i[p][1],
-                               (T)pp.par[((int64_t)p * 3 + 1) * npad + gj], (T)P.alpha, (T)P.dielectric, d, g);
+          default: {
+            const double ra = par1[gi], rb = par1[gj];
+            if (P.alpha > 0.0 && ra == 0.0 && rb == 0.0) {
+              e = PF::ei_point((par0[gi] * par0[gj]) / P.dielectric, P.alpha, d, inv, inv2, g);
+            } else {
+              // Gaussian charges or alpha = 0: the reference expression
+              e = PE::ei(par0[gi], par0[gj], ra, rb, P.alpha, P.dielectric, d, g);
+            }
             break;
+          }
         }
-        if (P.trunc != YNB_TR_NONE && (e != T(0) || g != T(0))) {  // pair_pot.c:130-139
-          T hg;
-          const T h = (P.trunc == YNB_TR_SWITCH3) ? PE::switch3(d, (T)P.rcut, (T)P.trunc_par, hg)
-                                                   : PE::hammer(d, (T)P.rcut, (T)P.trunc_par, hg);
-          g = g * h + e * hg / d;
+        if (P.trunc != YNB_TR_NONE && (e != 0.0 || g != 0.0)) {  // pair_pot.c:130-139
+          double hg;
+          const double h = (P.trunc == YNB_TR_SWITCH3) ? PE::switch3(d, P.rcut, P.trunc_par, hg)
+                                                        : PE::hammer(d, P.rcut, P.trunc_par, hg);
+          g = g * h + e * hg * inv;
           e *= h;
         }
-        const double vg = (double)g * sc;
-        acc_e[p] += sc * (double)e;
-        const int gs = SLOTS ? p : 0;
-        acc_g[gs][0] -= u[0] * vg;
-        acc_g[gs][1] -= u[1] * vg;
-        acc_g[gs][2] -= u[2] * vg;
-        acc_v[p][0] += u[0] * (u[0] * vg);
-        acc_v[p][1] += u[1] * (u[1] * vg);
-        acc_v[p][2] += u[2] * (u[2] * vg);
-        acc_v[p][3] += u[0] * (u[1] * vg);
-        acc_v[p][4] += u[0] * (u[2] * vg);
-        acc_v[p][5] += u[1] * (u[2] * vg);
+        vg_sum += g * sc;
+        e_sum += sc * e;
       }
+      acc_e += e_sum;
+      acc_g[0] -= u[0] * vg_sum;
+      acc_g[1] -= u[1] * vg_sum;
+      acc_g[2] -= u[2] * vg_sum;
+      acc_v[0] += u[0] * (u[0] * vg_sum);
+      acc_v[1] += u[1] * (u[1] * vg_sum);
+      acc_v[2] += u[2] * (u[2] * vg_sum);
+      acc_v[3] += u[0] * (u[1] * vg_sum);
+      acc_v[4] += u[0] * (u[2] * vg_sum);
+      acc_v[5] += u[1] * (u[2] * vg_sum);
     }
-  }
-  // Gradient: sum over the atom's G lanes; lane 0 of the group stores.
-  double G3[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-  for (int p = 0; p < (SLOTS ? YNB_MAX_POT : 1); ++p)
+    // gradient: sum over the atom's G lanes; lane 0 of the group stores
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      acc_g[p][c] = group_sum<G>(acc_g[p][c]);
-      G3[c] += acc_g[p][c];
+      acc_g[c] = group_sum<G>(acc_g[c]);
+      G3[c] += acc_g[c];
     }
-  if (active && s == 0) {
-    if (gpos) {
-      for (int c = 0; c < 3; ++c) {
-        if (accumulate)
-          gpos[3 * gi + c] += G3[c];
-        else
-          gpos[3 * gi + c] = G3[c];
-      }
-    }
-    if (SLOTS && gpos_slots) {
+    if (SLOTS && gpos_slots && active && s == 0)
 #pragma unroll
-      for (int p = 0; p < YNB_MAX_POT; ++p)
+      for (int c = 0; c < 3; ++c) gpos_slots[((int64_t)grp * npad + gi) * 3 + c] = acc_g[c];
+    // wave sums of energy and virial of this slot
+    const double e = wave_sum(acc_e);
+    double v[6];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) gpos_slots[((int64_t)p * npad + gi) * 3 + c] = acc_g[p][c];
-    }
-  }
-  // Block sums of energy and virial (the block's 16 atoms belong to one
-  // system; lanes past natom contribute zeros): waves, then the 4 waves of the
-  // block in order.
-  __shared__ double red[4][YNB_MAX_POT * 8];
-  const int wib = threadIdx.x >> 6;
+    for (int c = 0; c < 6; ++c) v[c] = wave_sum(acc_v[c]);
+    if (lane == 0) {
+      red[wib][grp * 8] = e;
 #pragma unroll
-  for (int p = 0; p < YNB_MAX_POT; ++p) {
-    if (p < npot) {
-      const double e = wave_sum(acc_e[p]);
-      double v[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) v[c] = wave_sum(acc_v[p][c]);
-      if (lane == 0) {
-        red[wib][p * 8] = e;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) red[wib][p * 8 + 1 + c] = v[c];
-        red[wib][p * 8 + 7] = 0.0;
-      }
+      for (int c = 0; c < 6; ++c) red[wib][grp * 8 + 1 + c] = v[c];
+      red[wib][grp * 8 + 7] = 0.0;
     }
   }
+  if (active && s == 0 && gpos) {
+    for (int c = 0; c < 3; ++c) {
+      if (accumulate)
+        gpos[3 * gi + c] += G3[c];
+      else
+        gpos[3 * gi + c] = G3[c];
+    }
+  }
+  // Block sums (the block's atoms belong to one system; lanes past natom
+  // contribute zeros): the 4 waves of the block in order.
   __syncthreads();
   if (threadIdx.x < npot * 8) {
     const int q = threadIdx.x;
-    partial[(int64_t)blockIdx.x * (YNB_MAX_POT * 8) + q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+    partial[(int64_t)blockIdx.x * (YNB_MAX_POT * 8) + q] =
+        (q < ngroup * 8) ? ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q] : 0.0;
   }
 }
 

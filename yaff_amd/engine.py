@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (YnbSys, YnbBatch, YnbNlist, YnbPairpar, YnbEwald, MAX_POT, FP64, MIXED, TR_NONE,
+from ._lib import (YnbSys, YnbBatch, YnbNlist, YnbPairpar, YnbEwald, MAX_POT, FP64, MIXED, TR_NONE, POT_LJ,
                    ROW_DTYPE, SCALING_DTYPE, check)
 
 __all__ = ['PotSpec', 'EwaldSpec', 'NBSystem', 'Engine', 'ewald_kvectors']
@@ -28,7 +28,8 @@ KBLOCK = 256         # k vectors per reduction block
 
 class PotSpec(object):
     """One pair potential of a system.  p0/p1/p2 are per-atom parameters:
-    LJ/MM3 sigma, epsilon (MM3: p2 = onlypauli); EI charge, radius; ExpRep
+    LJ/MM3 sigma, epsilon (MM3: p2 = onlypauli; LJ: p2 is filled with
+    sqrt(epsilon) by the engine); EI charge, radius; ExpRep
     ffatype id with amp_cross/b_cross tables.  ``stab`` is this potential's
     scaling table (default: the system's)."""
 
@@ -261,6 +262,9 @@ class Engine(object):
                 for q, arr in enumerate((pot.p0, pot.p1, pot.p2)):
                     if arr is not None:
                         par[p, q, a0:a0 + s.natom] = np.asarray(arr, dtype=np.float64)
+                if pot.kind == POT_LJ and pot.p1 is not None:
+                    # sqrt(epsilon) per atom: the pair kernel forms eps_ab = se_a se_b
+                    par[p, 2, a0:a0 + s.natom] = np.sqrt(np.asarray(pot.p1, dtype=np.float64))
                 if pot.amp_cross is not None:
                     nff = pot.amp_cross.shape[0]
                     hp.nffatype = nff
