@@ -249,9 +249,18 @@ def batched(nsys, steps, warmup):
         md.capture(1)
         nst = steps if prec == 'mixed' else max(steps//4, 1)
         dt = md.run(nst)
+        # per-kernel durations of a few eager steps of the same batch
+        from yaff_amd import _lib
+        torch.cuda.synchronize()
+        _lib.profile_enable(True)
+        for _ in range(3):
+            md.engine.md_step()
+        kt = kernel_table(_lib.profile_read(), 3)
+        _lib.profile_enable(False)
         progress('batched %s %.3g atom-evals/s' % (prec, natom*nst/dt))
         out['batched_' + prec] = {'value': natom*nst/dt, 'ms_per_step': dt/nst*1e3, 'steps': nst,
-                                  'tuning': tun, 'tune_s': t_tune}
+                                  'tuning': tun, 'tune_s': t_tune,
+                                  'kernels_ms_per_step': {k: v['ms_per_step'] for k, v in kt.items()}}
         del md
         torch.cuda.empty_cache()
     out['mixed_batched_vs_dropin_fp64'] = out['batched_mixed']['value']/out['dropin_fp64']['value']

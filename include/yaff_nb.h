@@ -111,8 +111,12 @@ typedef struct ynb_batch {
   int32_t nbin;              /* total bins (0 when every system uses the all-pairs path) */
   int64_t npad;              /* padded atom count, multiple of 64 */
   int32_t max_bin_atoms;     /* capacity of a bin, 1..1024 (ynb_nlist_build flags overflow) */
-  int32_t pad_;
+  int32_t flags;             /* YNB_BATCH_*: which build paths the batch needs */
 } ynb_batch;
+
+/* ynb_batch.flags */
+#define YNB_BATCH_ALLPAIRS 1    /* some system has natom > 0 and no bins (nvec < 3 or a huge search box) */
+#define YNB_BATCH_EXACT_BUILD 2 /* some binned system is not box_safe: exact re-test inside the build */
 
 /* Full neighbour list, one row of kcap entries per atom: entry k of atom i is
  * at i*kcap + k.

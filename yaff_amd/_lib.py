@@ -15,6 +15,7 @@ MAX_POT = 4
 POT_LJ, POT_MM3, POT_EXPREP, POT_EI = 0, 1, 2, 3
 TR_NONE, TR_SWITCH3, TR_HAMMER = 0, 1, 2
 FP64, MIXED = 0, 1
+BATCH_ALLPAIRS, BATCH_EXACT_BUILD = 1, 2
 
 ROW_DTYPE = np.dtype([('a', np.int64), ('b', np.int64), ('d', np.float64), ('dx', np.float64),
                       ('dy', np.float64), ('dz', np.float64), ('r0', np.int64), ('r1', np.int64),
@@ -42,7 +43,7 @@ class YnbSys(C.Structure):
 
 class YnbBatch(C.Structure):
     _fields_ = [('sys', C.c_void_p), ('wave_sys', C.c_void_p), ('bin_sys', C.c_void_p), ('nsys', C.c_int32),
-                ('nbin', C.c_int32), ('npad', C.c_int64), ('max_bin_atoms', C.c_int32), ('pad_', C.c_int32)]
+                ('nbin', C.c_int32), ('npad', C.c_int64), ('max_bin_atoms', C.c_int32), ('flags', C.c_int32)]
 
 
 class YnbNlist(C.Structure):

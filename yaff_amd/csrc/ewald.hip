@@ -10,7 +10,7 @@
 // every phase unchanged modulo 2 pi.
 //
 // Pass 1 (k_ewald_sk): one lane per segment of <= L k vectors, atoms of a
-//   chunk broadcast from LDS; partial S(k) per (chunk, k).
+//   chunk read with scalar loads; partial S(k) per (chunk, k).
 // Pass 2 (k_ewald_kreduce): one thread per k sums its chunks in a fixed
 //   order per group (atoms below nlow, between, above nhigh), forms the
 //   energy and virial terms of ewald.c:82-127 and the coefficients 2 c S(k).
@@ -73,63 +73,58 @@ __device__ inline double reduced_phase(int g0, int g1, int g2, double f0, double
   return t - rint(t);
 }
 
+// One lane per segment of <= L consecutive k of a row; the atoms of the work
+// item's chunk are walked in order and their data are the same for every
+// lane, so they come through scalar loads (no LDS staging).
 template <int PREC, int L>
-__global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *sys, ynb_ewald ew, int64_t npad) {
-  using T = double;
-  __shared__ double l_f[3][CHUNK];
-  __shared__ T l_q[CHUNK];
-  __shared__ T l_st[2][CHUNK];
+__global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys, const int32_t *__restrict__ work,
+                                                 const int32_t *__restrict__ chunk, const int32_t *__restrict__ seg,
+                                                 const int32_t *__restrict__ seg_len,
+                                                 const double *__restrict__ frac, const double *__restrict__ step,
+                                                 double *__restrict__ spart, int nwork, int64_t npad) {
+#pragma clang fp contract(fast)
   const int lane = threadIdx.x;
-  for (int w = blockIdx.x; w < ew.nwork; w += gridDim.x) {
-    const int4 wk = *(const int4 *)(ew.work + 4 * w);
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int4 wk = *(const int4 *)(work + 4 * w);
     const ynb_sys &S = sys[wk.x];
-    const int4 ch = *(const int4 *)(ew.chunk + 4 * wk.z);
+    const int4 ch = *(const int4 *)(chunk + 4 * wk.z);
     const int64_t a0 = S.atom0 + ch.y;
     const int na = ch.z;
-    __syncthreads();
-    for (int t = lane; t < na; t += 64) {
-      const int64_t g = a0 + t;
-      l_f[0][t] = ew.frac[g];
-      l_f[1][t] = ew.frac[npad + g];
-      l_f[2][t] = ew.frac[2 * npad + g];
-      l_q[t] = (T)ew.frac[3 * npad + g];
-      l_st[0][t] = (T)ew.step[g];
-      l_st[1][t] = (T)ew.step[npad + g];
-    }
-    __syncthreads();
     const bool valid = lane < wk.w;
     int4 sg = make_int4(0, 0, 0, 0);
     int len = 0;
     if (valid) {
-      sg = *(const int4 *)(ew.seg + 4 * (wk.y + lane));
-      len = ew.seg_len[wk.y + lane];
+      sg = *(const int4 *)(seg + 4 * (wk.y + lane));
+      len = seg_len[wk.y + lane];
     }
-    T ar[L], ai[L];
+    double ar[L], ai[L];
 #pragma unroll
-    for (int m = 0; m < L; ++m) ar[m] = ai[m] = T(0);
-    if (valid) {
-      for (int t = 0; t < na; ++t) {
-        const double ph = reduced_phase(sg.x, sg.y, sg.z, l_f[0][t], l_f[1][t], l_f[2][t]);
-        T s, c;
-        seed_sincospi<PREC>(2.0 * ph, &s, &c);
-        const T q = l_q[t];
-        T zr = q * c, zi = q * s;
-        const T cr = l_st[0][t], ci = l_st[1][t];
+    for (int m = 0; m < L; ++m) ar[m] = ai[m] = 0.0;
+#pragma unroll 2
+    for (int t = 0; t < na; ++t) {
+      const int64_t g = a0 + t;
+      const double ph = reduced_phase(sg.x, sg.y, sg.z, frac[g], frac[npad + g], frac[2 * npad + g]);
+      double sn, cs;
+      seed_sincospi<PREC>(2.0 * ph, &sn, &cs);
+      const double q = frac[3 * npad + g];
+      double zr = q * cs, zi = q * sn;
+      const double cr = step[g], ci = step[npad + g];
 #pragma unroll
-        for (int m = 0; m < L; ++m) {
-          ar[m] += zr;
-          ai[m] += zi;
-          const T nr = zr * cr - zi * ci;
-          zi = zr * ci + zi * cr;
-          zr = nr;
-        }
+      for (int m = 0; m < L; ++m) {
+        ar[m] += zr;
+        ai[m] += zi;
+        const double nr = zr * cr - zi * ci;
+        zi = zr * ci + zi * cr;
+        zr = nr;
       }
-      double *out = ew.spart + 2 * (S.spart0 + (int64_t)(wk.z - S.chunk0) * S.nk + (sg.w - S.k0));
+    }
+    if (valid) {
+      double *out = spart + 2 * (S.spart0 + (int64_t)(wk.z - S.chunk0) * S.nk + (sg.w - S.k0));
 #pragma unroll
       for (int m = 0; m < L; ++m) {
         if (m < len) {
-          out[2 * m] = (double)ar[m];
-          out[2 * m + 1] = (double)ai[m];
+          out[2 * m] = ar[m];
+          out[2 * m + 1] = ai[m];
         }
       }
     }
@@ -234,6 +229,7 @@ __global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_e
 template <int PREC, bool GROUPS>
 __global__ __launch_bounds__(64) void k_ewald_force(const ynb_sys *sys, const int32_t *wave_sys, ynb_ewald ew,
                                                     int64_t npad) {
+#pragma clang fp contract(fast)
   __shared__ double l_k[3][64];
   __shared__ double l_c[6][64];
   __shared__ int4 l_g[64];
@@ -421,7 +417,9 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
   YNB_LAUNCHED();
   if (ew->nwork > 0) {
     const dim3 grid((unsigned)(ew->nwork < 65535 ? ew->nwork : 65535));
-#define YNB_SK(P, LL) YNB_LAUNCH("ewald_sk", (k_ewald_sk<P, LL>), grid, dim3(64), 0, st, b->sys, *ew, npad)
+#define YNB_SK(P, LL)                                                                                          \
+  YNB_LAUNCH("ewald_sk", (k_ewald_sk<P, LL>), grid, dim3(64), 0, st, b->sys, ew->work, ew->chunk, ew->seg,       \
+             ew->seg_len, ew->frac, ew->step, ew->spart, ew->nwork, npad)
     if (precision == YNB_FP64) {
       if (ew->seglen == 4) YNB_SK(YNB_FP64, 4);
       else if (ew->seglen == 8) YNB_SK(YNB_FP64, 8);

@@ -129,9 +129,9 @@ __device__ inline void store_entry(const ynb_nlist &nl, int64_t npad, int64_t gi
   }
 }
 
-constexpr int BW = 16;        // atoms (wavefronts) per block of the binned build
-constexpr int CAP = 2048;     // partner candidates staged in LDS at once
-constexpr int MAXNB = 4096;   // partner bins per home bin (ynb_sys_plan keeps (2R+1)^3 below)
+constexpr int BW = 8;         // atoms (wavefronts) per block of the binned build
+constexpr int CAP = 1024;     // partner candidates staged in LDS at once
+constexpr int MAXNB = 512;    // partner bins per home bin (ynb_sys_plan keeps (2R+1)^3 below)
 
 // Lanes of the wave below this one that hold a set bit of mask.
 __device__ inline int lanes_below(uint64_t mask) {
@@ -147,17 +147,25 @@ __device__ inline int lanes_below(uint64_t mask) {
 // bins in offset order, partner atoms by index).
 //
 // Acceptance.  Relative coordinates stay below ~150 bohr, so the FP32
-// distance squared is within 1e-5 relative of the exact one; outside a band
-// of +-2e-4 relative around rc^2 it decides (inside: accept; the image box
-// cannot bind when box_safe, see ynb_sys_plan).  Inside the band the exact
-// Yaff expression on the FP64 positions decides.
-__global__ __launch_bounds__(1024) void k_build_binned(const ynb_sys *sys, const int32_t *bin_sys, int64_t npad,
-                                                       const double *pos, const int32_t *bin_start,
-                                                       const int32_t *bin_end, const double4 *sxw, const int4 *sw,
-                                                       ynb_nlist nl) {
+// distance squared is within 1e-5 relative of the exact one; below a band of
+// -2e-4 relative around rc^2 it accepts, above +2e-4 it rejects.  Inside the
+// band: with EXACT (some system of the batch is not box_safe), the exact
+// Yaff expression on the FP64 positions decides here; without it the
+// candidate is kept tentatively (the image box cannot bind when box_safe, see
+// ynb_sys_plan) and the exact test is made by whoever needs it: the pair
+// kernel re-tests every entry within 1e-12 of a cutoff exactly and skips
+// d >= rc, and the row materialisation drops entries with exact d >= rc.
+template <bool EXACT>
+__global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *sys, const int32_t *bin_sys, int64_t npad,
+                                                          const double *pos, const int32_t *bin_start,
+                                                          const int32_t *bin_end, const double4 *sxw,
+                                                          const int4 *sw, ynb_nlist nl) {
   __shared__ float l_x[CAP], l_y[CAP], l_z[CAP];
   __shared__ int32_t l_m[CAP], l_g[CAP];
   __shared__ int32_t l_pre[MAXNB + 1];
+  __shared__ int32_t l_bs[MAXNB];       // slot start of each partner bin
+  __shared__ double l_sh[3][MAXNB];     // image shift of each partner bin minus the home-bin centre
+  __shared__ int32_t l_s[MAXNB];        // packed shift (s0, s1, s2)
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
@@ -175,27 +183,29 @@ __global__ __launch_bounds__(1024) void k_build_binned(const ynb_sys *sys, const
   const double X1 = f0 * S.rvecs[1] + f1 * S.rvecs[4] + f2 * S.rvecs[7];
   const double X2 = f0 * S.rvecs[2] + f1 * S.rvecs[5] + f2 * S.rvecs[8];
 
-  auto partner = [&](int k, int &nbin, int &s0, int &s1, int &s2) {
-    const int i2 = k % n2, i1 = (k / n2) % n1, i0 = k / (n1 * n2);
-    int nb0 = c0 + i0 - R0, nb1 = c1 + i1 - R1, nb2 = c2 + i2 - R2;
-    s0 = floor_div(nb0, S.grid[0]);
-    s1 = floor_div(nb1, S.grid[1]);
-    s2 = floor_div(nb2, S.grid[2]);
-    nb0 -= s0 * S.grid[0];
-    nb1 -= s1 * S.grid[1];
-    nb2 -= s2 * S.grid[2];
-    nbin = S.bin0 + (nb0 * S.grid[1] + nb1) * S.grid[2] + nb2;
-  };
-  // Exclusive prefix of the partner-bin sizes, 64 bins per step in wave 0.
+  // Partner-bin table and the exclusive prefix of the partner-bin sizes
+  // (64 bins per step in wave 0).
   if (wv == 0) {
     int carry = 0;
     for (int k0 = 0; k0 < nnb; k0 += 64) {
       const int k = k0 + lane;
       int c = 0;
       if (k < nnb) {
-        int nbin, s0, s1, s2;
-        partner(k, nbin, s0, s1, s2);
-        c = bin_end[nbin] - bin_start[nbin];
+        const int i2 = k % n2, i1 = (k / n2) % n1, i0 = k / (n1 * n2);
+        int nb0 = c0 + i0 - R0, nb1 = c1 + i1 - R1, nb2 = c2 + i2 - R2;
+        const int s0 = floor_div(nb0, S.grid[0]), s1 = floor_div(nb1, S.grid[1]), s2 = floor_div(nb2, S.grid[2]);
+        nb0 -= s0 * S.grid[0];
+        nb1 -= s1 * S.grid[1];
+        nb2 -= s2 * S.grid[2];
+        const int nbin = S.bin0 + (nb0 * S.grid[1] + nb1) * S.grid[2] + nb2;
+        const int bs = bin_start[nbin];
+        c = bin_end[nbin] - bs;
+        l_bs[k] = bs;
+        l_sh[0][k] = s0 * S.rvecs[0] + s1 * S.rvecs[3] + s2 * S.rvecs[6] - X0;
+        l_sh[1][k] = s0 * S.rvecs[1] + s1 * S.rvecs[4] + s2 * S.rvecs[7] - X1;
+        l_sh[2][k] = s0 * S.rvecs[2] + s1 * S.rvecs[5] + s2 * S.rvecs[8] - X2;
+        const bool far = s0 < -511 || s0 > 511 || s1 < -511 || s1 > 511 || s2 < -511 || s2 > 511;
+        l_s[k] = far ? INT32_MIN : pack_img(s0, s1, s2);
       }
       int incl = c;
 #pragma unroll
@@ -212,31 +222,28 @@ __global__ __launch_bounds__(1024) void k_build_binned(const ynb_sys *sys, const
   const int T = l_pre[nnb];
   const double rc2 = S.rc * S.rc;
   const float lo32 = (float)(rc2 * (1.0 - 2e-4)), hi32 = (float)(rc2 * (1.0 + 2e-4));
-  const bool fast_ok = S.box_safe != 0;
   bool bad = false;
 
   auto stage = [&](int base) {
     const int lim = min(T - base, CAP);
-    for (int q = tid; q < lim; q += 1024) {
+    for (int q = tid; q < lim; q += BW * 64) {
       const int f = base + q;
       int lo = 0, hi = nnb;  // largest k with l_pre[k] <= f
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (l_pre[mid] <= f) lo = mid; else hi = mid;
       }
-      int nbin, s0, s1, s2;
-      partner(lo, nbin, s0, s1, s2);
-      const int j = bin_start[nbin] + (f - l_pre[lo]);
+      const int j = l_bs[lo] + (f - l_pre[lo]);
       const double4 x = sxw[j];
       const int4 w = sw[j];
-      const double sx = s0 * S.rvecs[0] + s1 * S.rvecs[3] + s2 * S.rvecs[6];
-      const double sy = s0 * S.rvecs[1] + s1 * S.rvecs[4] + s2 * S.rvecs[7];
-      const double sz = s0 * S.rvecs[2] + s1 * S.rvecs[5] + s2 * S.rvecs[8];
-      l_x[q] = (float)(x.x + sx - X0);
-      l_y[q] = (float)(x.y + sy - X1);
-      l_z[q] = (float)(x.z + sz - X2);
-      const int m0 = s0 - w.x, m1 = s1 - w.y, m2 = s2 - w.z;
-      if (m0 < -511 || m0 > 511 || m1 < -511 || m1 > 511 || m2 < -511 || m2 > 511) bad = true;
+      l_x[q] = (float)(x.x + l_sh[0][lo]);
+      l_y[q] = (float)(x.y + l_sh[1][lo]);
+      l_z[q] = (float)(x.z + l_sh[2][lo]);
+      const int sp = l_s[lo];
+      int s3[3];
+      unpack_img(sp, s3);
+      const int m0 = s3[0] - w.x, m1 = s3[1] - w.y, m2 = s3[2] - w.z;
+      if (sp == INT32_MIN || m0 < -511 || m0 > 511 || m1 < -511 || m1 > 511 || m2 < -511 || m2 > 511) bad = true;
       l_m[q] = pack_img(m0, m1, m2);
       l_g[q] = w.w;
     }
@@ -247,7 +254,6 @@ __global__ __launch_bounds__(1024) void k_build_binned(const ynb_sys *sys, const
     stage(0);
     __syncthreads();
   }
-  int rowmax = 0;
   for (int ib = start + (int)blockIdx.y * BW; ib < end; ib += (int)gridDim.y * BW) {
     const int p = ib + wv;
     const bool active = p < end;  // uniform in the wave
@@ -261,11 +267,12 @@ __global__ __launch_bounds__(1024) void k_build_binned(const ynb_sys *sys, const
     const int li = (int)(gi - S.atom0);
     const float xr = (float)(xi.x - X0), yr = (float)(xi.y - X1), zr = (float)(xi.z - X2);
     double pi[3] = {0, 0, 0};
-    if (active) {
+    if (EXACT && active) {
       pi[0] = pos[3 * gi];
       pi[1] = pos[3 * gi + 1];
       pi[2] = pos[3 * gi + 2];
     }
+    const bool fast_ok = !EXACT || S.box_safe != 0;
     int32_t *rowj = nl.j + gi * nl.kcap;
     int32_t *rowi = nl.img + gi * nl.kcap;
     int cnt = 0;  // uniform in the wave
@@ -295,7 +302,7 @@ __global__ __launch_bounds__(1024) void k_build_binned(const ynb_sys *sys, const
             const int lj = gj - S.atom0;
             const int A = li > lj ? li : lj, B = li > lj ? lj : li;
             acc = !(self && n[0] == 0 && n[1] == 0 && n[2] == 0) && nlow_nhigh_ok(A, B, S.nlow, S.nhigh);
-            if (acc && !(fast_ok && d2 < lo32)) {
+            if (EXACT && acc && !(fast_ok && d2 < lo32)) {
               const bool i_is_a = (li == A);
               const int nab[3] = {i_is_a ? n[0] : -n[0], i_is_a ? n[1] : -n[1], i_is_a ? n[2] : -n[2]};
               const double pj[3] = {pos[3 * (int64_t)gj], pos[3 * (int64_t)gj + 1], pos[3 * (int64_t)gj + 2]};
@@ -327,10 +334,8 @@ __global__ __launch_bounds__(1024) void k_build_binned(const ynb_sys *sys, const
       nl.cnt[gi] = cnt;
       if (cnt > nl.kcap) atomicOr(nl.status, 1);
     }
-    rowmax = max(rowmax, active ? cnt : 0);
   }
   if (bad) atomicOr(nl.status, 2);
-  (void)rowmax;
 }
 
 // All-pairs build for systems without bins (nvec < 3): every partner and
@@ -397,15 +402,34 @@ __device__ inline bool canonical_entry(int64_t gi, int32_t gj, int32_t img) {
   return in_half_space(n);
 }
 
-__global__ void k_rows_count(const ynb_sys *sys, int s, int64_t npad, ynb_nlist nl, int32_t *rowcnt) {
+// Entries of the list are Yaff rows when they are canonical and pass the
+// exact test (the build may keep entries within 2e-4 of rc^2 tentatively).
+__device__ inline bool exact_row(const ynb_sys &S, int li, int lj, const double *pi, const double *pj,
+                                 const int nn[3], double v[3], int r[3]) {
+  const bool in_box = yaff_row_vector(pi, pj, li == lj, nn, S.rvecs, S.gvecs, S.nvec, S.rmax, v, r);
+  return in_box && (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] < S.rc * S.rc);
+}
+
+__global__ void k_rows_count(const ynb_sys *sys, int s, int64_t npad, const double *pos, ynb_nlist nl,
+                             int32_t *rowcnt) {
   const ynb_sys &S = sys[s];
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= S.natom) return;
   const int64_t gi = S.atom0 + li;
   const int n = min(nl.cnt[gi], nl.kcap);
+  const double pi[3] = {pos[3 * gi], pos[3 * gi + 1], pos[3 * gi + 2]};
   int c = 0;
-  for (int k = 0; k < n; ++k)
-    c += canonical_entry(gi, nl.j[gi * nl.kcap + k], nl.img[gi * nl.kcap + k]);
+  for (int k = 0; k < n; ++k) {
+    const int32_t gj = nl.j[gi * nl.kcap + k];
+    const int32_t img = nl.img[gi * nl.kcap + k];
+    if (!canonical_entry(gi, gj, img)) continue;
+    int nn[3];
+    unpack_img(img, nn);
+    const double pj[3] = {pos[3 * (int64_t)gj], pos[3 * (int64_t)gj + 1], pos[3 * (int64_t)gj + 2]};
+    double v[3];
+    int r[3];
+    c += exact_row(S, li, (int)(gj - S.atom0), pi, pj, nn, v, r);
+  }
   rowcnt[li] = c;
 }
 
@@ -433,7 +457,7 @@ __global__ void k_rows_emit(const ynb_sys *sys, int s, int64_t npad, const doubl
     const double pj[3] = {pos[3 * (int64_t)gj], pos[3 * (int64_t)gj + 1], pos[3 * (int64_t)gj + 2]};
     double v[3];
     int r[3];
-    yaff_row_vector(pi, pj, self, nn, S.rvecs, S.gvecs, nv, S.rmax, v, r);
+    if (!exact_row(S, li, lj, pi, pj, nn, v, r)) continue;
     const double d2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
     ynb_neigh_row row;
     const bool plus = in_half_space(r);
@@ -552,36 +576,43 @@ int ynb_sys_plan(ynb_sys *s) {
   }
   if (s->nvec < 3 || s->natom == 0) return YNB_OK;
   const double rcp = s->rc * (1.0 + 1e-8) + 1e-6;
+  // Bins at least rc/2 wide (2-3 bins searched per side: about 4x fewer
+  // candidates per atom than rc-wide bins); rc-wide bins when the search box
+  // would exceed MAXNB partner bins (very skewed or small cells).
   int64_t nbin = 1;
-  for (int k = 0; k < 3; ++k) {
-    const double *g = s->gvecs + 3 * k;
-    const double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);  // 1/plane spacing
-    const double spacing = 1.0 / gn;
-    int grid = (int)floor(spacing / rcp);
-    if (grid < 1) grid = 1;
-    if (grid > 1024) grid = 1024;
-    // Bins searched on each side: the fractional extent of a sphere of
-    // radius rc along axis k is rc*|g_k|.
-    int range = (int)ceil(rcp * gn * grid);
-    if (range < 1) range = 1;
-    s->grid[k] = grid;
-    s->range[k] = range;
-    nbin *= grid;
+  for (int div = 2; div >= 1; --div) {
+    nbin = 1;
+    for (int k = 0; k < 3; ++k) {
+      const double *g = s->gvecs + 3 * k;
+      const double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);  // 1/plane spacing
+      const double spacing = 1.0 / gn;
+      int grid = (int)floor(spacing * div / rcp);
+      if (grid < 1) grid = 1;
+      if (grid > 1024) grid = 1024;
+      // Bins searched on each side: the fractional extent of a sphere of
+      // radius rc along axis k is rc*|g_k|.
+      int range = (int)ceil(rcp * gn * grid);
+      if (range < 1) range = 1;
+      s->grid[k] = grid;
+      s->range[k] = range;
+      nbin *= grid;
+    }
+    // Keep at least a few atoms per bin on average.
+    while (nbin > 1 && nbin * 4 > (int64_t)s->natom * 8) {
+      int kmax = 0;
+      for (int k = 1; k < 3; ++k)
+        if (s->grid[k] > s->grid[kmax]) kmax = k;
+      if (s->grid[kmax] == 1) break;
+      s->grid[kmax] = (s->grid[kmax] + 1) / 2;
+      const double *g = s->gvecs + 3 * kmax;
+      const double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+      s->range[kmax] = (int)ceil(rcp * gn * s->grid[kmax]);
+      if (s->range[kmax] < 1) s->range[kmax] = 1;
+      nbin = (int64_t)s->grid[0] * s->grid[1] * s->grid[2];
+    }
+    if ((int64_t)(2 * s->range[0] + 1) * (2 * s->range[1] + 1) * (2 * s->range[2] + 1) <= MAXNB) break;
   }
-  // Keep at least a few atoms per bin on average.
-  while (nbin > 1 && nbin * 4 > (int64_t)s->natom * 8) {
-    int kmax = 0;
-    for (int k = 1; k < 3; ++k)
-      if (s->grid[k] > s->grid[kmax]) kmax = k;
-    if (s->grid[kmax] == 1) break;
-    s->grid[kmax] = (s->grid[kmax] + 1) / 2;
-    const double *g = s->gvecs + 3 * kmax;
-    const double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-    s->range[kmax] = (int)ceil(rcp * gn * s->grid[kmax]);
-    if (s->range[kmax] < 1) s->range[kmax] = 1;
-    nbin = (int64_t)s->grid[0] * s->grid[1] * s->grid[2];
-  }
-  if ((int64_t)(2 * s->range[0] + 1) * (2 * s->range[1] + 1) * (2 * s->range[2] + 1) > 4096) {
+  if ((int64_t)(2 * s->range[0] + 1) * (2 * s->range[1] + 1) * (2 * s->range[2] + 1) > MAXNB) {
     // Cutoff spans too many periodic copies for the binned build's LDS
     // tables: use the all-pairs path.
     s->nbin = 0;
@@ -636,13 +667,19 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
     YNB_LAUNCH("nlist_bin_order", k_bin_order, dim3(b->nbin), dim3(256), 0, st, cb, w.bin_count, w.slot_idx, w.xw,
                w.wr, w.sxw, w.sw, w.bin_start, w.bin_end);
     const int ychunks = (int)ceil_div(cb, BW) < 64 ? (int)ceil_div(cb, BW) : 64;
-    YNB_LAUNCH("nlist_build_binned", k_build_binned, dim3(b->nbin, ychunks), dim3(1024), 0, st, b->sys, b->bin_sys,
-               npad, pos, w.bin_start, w.bin_end, w.sxw, w.sw, *nl);
+    if (b->flags & YNB_BATCH_EXACT_BUILD)
+      YNB_LAUNCH("nlist_build_binned", k_build_binned<true>, dim3(b->nbin, ychunks), dim3(BW * 64), 0, st, b->sys,
+                 b->bin_sys, npad, pos, w.bin_start, w.bin_end, w.sxw, w.sw, *nl);
+    else
+      YNB_LAUNCH("nlist_build_binned", k_build_binned<false>, dim3(b->nbin, ychunks), dim3(BW * 64), 0, st, b->sys,
+                 b->bin_sys, npad, pos, w.bin_start, w.bin_end, w.sxw, w.sw, *nl);
     YNB_LAUNCHED();
   }
-  YNB_LAUNCH("nlist_build_allpairs", k_build_allpairs, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos,
-             *nl);
-  YNB_LAUNCHED();
+  if (b->flags & YNB_BATCH_ALLPAIRS) {
+    YNB_LAUNCH("nlist_build_allpairs", k_build_allpairs, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos,
+               *nl);
+    YNB_LAUNCHED();
+  }
   return YNB_OK;
 }
 
@@ -696,7 +733,7 @@ int ynb_nlist_rows(const ynb_batch *b, int32_t s, const double *pos, const ynb_n
   }
   const int tb = 256;
   const int nb = (int)ceil_div(natom, tb);
-  hipLaunchKernelGGL(k_rows_count, dim3(nb), dim3(tb), 0, st, b->sys, s, b->npad, *nl, rowcnt);
+  hipLaunchKernelGGL(k_rows_count, dim3(nb), dim3(tb), 0, st, b->sys, s, b->npad, pos, *nl, rowcnt);
   YNB_LAUNCHED();
   size_t cb = cub_bytes;
   YNB_HIP(hipcub::DeviceScan::ExclusiveSum(cub, cb, rowcnt, rowoff, (int)natom, st));

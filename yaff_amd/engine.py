@@ -393,7 +393,7 @@ class Engine(object):
         self.bin_sys_d = self._t(self.bin_sys)
         self.bin_cap = self._max_bin_atoms()
         self.batch = YnbBatch(self.sys_d.data_ptr(), self.wave_sys_d.data_ptr(), self.bin_sys_d.data_ptr(),
-                              self.nsys, self.nbin, self.npad, self.bin_cap, 0)
+                              self.nsys, self.nbin, self.npad, self.bin_cap, self._batch_flags())
         self.pos_d = torch.zeros((self.npad, 3), dtype=torch.float64, device=dev)
         self.gpos_d = torch.zeros((self.npad, 3), dtype=torch.float64, device=dev)
         self.par_d = self._t(self.par_h)
@@ -429,6 +429,18 @@ class Engine(object):
         self.dt_d = torch.zeros(self.nsys, dtype=torch.float64, device=dev)
         self.ekin2_d = torch.zeros(self.nsys, dtype=torch.float64, device=dev)
         self.md_partial_d = torch.zeros(self.npad//64, dtype=torch.float64, device=dev)
+
+    def _batch_flags(self):
+        """Build paths the batch needs: all-pairs (systems without bins) and the
+        exact in-build re-test (binned systems that are not box_safe)."""
+        f = 0
+        for i in range(self.nsys):
+            h = self.hsys[i]
+            if h.natom > 0 and h.nbin == 0:
+                f |= _lib.BATCH_ALLPAIRS
+            if h.nbin > 0 and not h.box_safe:
+                f |= _lib.BATCH_EXACT_BUILD
+        return f
 
     def _max_bin_atoms(self):
         """Bin capacity: twice the largest mean occupancy plus a margin."""
