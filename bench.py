@@ -57,6 +57,9 @@ def parse():
     ap.add_argument('--no-served', action='store_true')
     ap.add_argument('--serve-requests', type=int, default=150)
     ap.add_argument('--serve-rates', type=str, default='25,100,400')
+    ap.add_argument('--copy-probe-mb', type=int, default=0,
+                    help='after the timed region, run an elementwise FP64 copy of this many MiB (a kernel of known '
+                         'HBM traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)')
     return ap.parse_args()
 
 
@@ -377,6 +380,14 @@ def main():
                         'virial_rel_err': float(np.abs(vm - v64).max()/np.abs(v64).max()),
                         'kernels': kernel_table(profm, K)}
 
+    if args.copy_probe_mb:
+        src = torch.ones(args.copy_probe_mb*(1 << 20)//8, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        out['copy_probe'] = {'bytes_read': src.numel()*8, 'bytes_written': dst.numel()*8,
+                             'kernel': 'torch elementwise mul (out=)'}
+        del src, dst
     progress('single-system FP64/mixed done')
     if not args.no_batched:
         out['batched'] = batched(args.batch_systems, args.batch_steps, 2)

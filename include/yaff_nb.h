@@ -9,8 +9,9 @@
  *
  * Work is batched: a launch covers every system of a ynb_batch.  Atoms of
  * system s live at [sys.atom0, sys.atom0 + sys.natom) of the padded atom
- * arrays; atom0 is a multiple of 64 so that one wavefront never holds atoms
- * of two systems.  A single system is a batch of one.
+ * arrays; atom0 is a multiple of 128 so that neither a wavefront nor the
+ * two-wavefront atom blocks of the Ewald force pass hold atoms of two
+ * systems.  A single system is a batch of one.
  *
  * Reference interfaces replaced (molmod/yaff):
  *   ynb_nlist_build + ynb_nlist_rows   <- nlist_build_low   yaff/pes/nlist.h:37-39
@@ -81,7 +82,7 @@ typedef struct ynb_sys {
   double volume;
   int32_t nvec;       /* 0..3 periodic directions */
   int32_t natom;
-  int32_t atom0;      /* offset into the padded atom arrays, multiple of 64 */
+  int32_t atom0;      /* offset into the padded atom arrays, multiple of 128 */
   int32_t npot;
   int32_t rmax[3];    /* image range of nlist.py:132 */
   int32_t nlow, nhigh;
@@ -109,7 +110,7 @@ typedef struct ynb_batch {
   const int32_t *bin_sys;    /* device [nbin]: system of each bin */
   int32_t nsys;
   int32_t nbin;              /* total bins (0 when every system uses the all-pairs path) */
-  int64_t npad;              /* padded atom count, multiple of 64 */
+  int64_t npad;              /* padded atom count, multiple of 128 */
   int32_t max_bin_atoms;     /* capacity of a bin, 1..1024 (ynb_nlist_build flags overflow) */
   int32_t flags;             /* YNB_BATCH_*: which build paths the batch needs */
 } ynb_batch;

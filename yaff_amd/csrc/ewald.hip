@@ -310,6 +310,93 @@ __global__ __launch_bounds__(64) void k_ewald_force(const ynb_sys *sys, const in
   o[2] = G2 * scale;
 }
 
+// Without nlow/nhigh groups: two atoms per lane (atoms g and g + 64 of a
+// 128-atom block; systems start at multiples of 128 atoms), so each k's data,
+// read from LDS as broadcasts, serve two atoms.  k data are packed as
+// (kx, ky, kz, 2c S_s) and (2c S_c, row start) per k.
+template <int PREC>
+__global__ __launch_bounds__(64) void k_ewald_force2(const ynb_sys *sys, const int32_t *wave_sys, ynb_ewald ew,
+                                                     int64_t npad) {
+#pragma clang fp contract(fast)
+  __shared__ double4 l_ka[64];
+  __shared__ double2 l_kb[64];
+  __shared__ int4 l_g[64];
+  const int64_t blk = blockIdx.x;
+  const int kc = blockIdx.y;
+  const int lane = threadIdx.x;
+  const int sid = uniform_int(wave_sys[2 * blk]);
+  if (sid < 0) return;
+  const ynb_sys &S = sys[sid];
+  if (S.nk == 0) return;
+  const int64_t ga = blk * 128 + lane, gb = ga + 64;
+  const bool act_a = ga - S.atom0 < S.natom, act_b = gb - S.atom0 < S.natom;
+  const double fa0 = ew.frac[ga], fa1 = ew.frac[npad + ga], fa2 = ew.frac[2 * npad + ga];
+  const double fb0 = ew.frac[gb], fb1 = ew.frac[npad + gb], fb2 = ew.frac[2 * npad + gb];
+  const double cra = ew.step[ga], cia = ew.step[npad + ga];
+  const double crb = ew.step[gb], cib = ew.step[npad + gb];
+  const int rb = S.row0 + (int)(((int64_t)S.nrow * kc) / ew.kchunks);
+  const int re = S.row0 + (int)(((int64_t)S.nrow * (kc + 1)) / ew.kchunks);
+  double A0 = 0.0, A1 = 0.0, A2 = 0.0, B0 = 0.0, B1 = 0.0, B2 = 0.0;
+  if (re > rb) {
+    const int kbeg = ew.krow_k[rb];
+    const int kend = ew.krow_k[re - 1] + ew.krow[4 * (re - 1) + 3];
+    double zar = 1.0, zai = 0.0, zbr = 1.0, zbi = 0.0;
+    for (int k0 = kbeg; k0 < kend; k0 += 64) {
+      const int kk = k0 + lane;
+      __syncthreads();
+      if (kk < kend) {
+        const double4 kv = *(const double4 *)(ew.kvec + 4 * (int64_t)kk);
+        const double *cf = ew.coef + 6 * (int64_t)kk;
+        const int4 gg = *(const int4 *)(ew.kg + 4 * (int64_t)kk);
+        l_ka[lane] = make_double4(kv.x, kv.y, kv.z, cf[1]);
+        l_kb[lane] = make_double2(cf[0], gg.w ? 1.0 : 0.0);
+        l_g[lane] = gg;
+      }
+      __syncthreads();
+      const int nt = min(64, kend - k0);
+      for (int t = 0; t < nt; ++t) {
+        const double4 ka = l_ka[t];
+        const double2 kb = l_kb[t];
+        if (kb.y != 0.0) {  // row start (the same k for the whole wave): seed exp(2 pi i g.f)
+          const int4 gg = l_g[t];
+          double s, c;
+          seed_sincospi<PREC>(2.0 * reduced_phase(gg.x, gg.y, gg.z, fa0, fa1, fa2), &s, &c);
+          zar = c;
+          zai = s;
+          seed_sincospi<PREC>(2.0 * reduced_phase(gg.x, gg.y, gg.z, fb0, fb1, fb2), &s, &c);
+          zbr = c;
+          zbi = s;
+        }
+        // ewald.c:104-111: x = 2c (S_c (-q sin) + S_s (q cos)); q applied below.
+        const double xa = ka.w * zar - kb.x * zai;
+        const double xb = ka.w * zbr - kb.x * zbi;
+        A0 = fma(ka.x, xa, A0);
+        A1 = fma(ka.y, xa, A1);
+        A2 = fma(ka.z, xa, A2);
+        B0 = fma(ka.x, xb, B0);
+        B1 = fma(ka.y, xb, B1);
+        B2 = fma(ka.z, xb, B2);
+        const double nar = zar * cra - zai * cia;
+        zai = zar * cia + zai * cra;
+        zar = nar;
+        const double nbr = zbr * crb - zbi * cib;
+        zbi = zbr * cib + zbi * crb;
+        zbr = nbr;
+      }
+    }
+  }
+  const double sa = act_a ? ew.frac[3 * npad + ga] / S.dielectric : 0.0;
+  const double sb = act_b ? ew.frac[3 * npad + gb] / S.dielectric : 0.0;
+  double *oa = ew.gpart + ((int64_t)kc * npad + ga) * 3;
+  double *ob = ew.gpart + ((int64_t)kc * npad + gb) * 3;
+  oa[0] = A0 * sa;
+  oa[1] = A1 * sa;
+  oa[2] = A2 * sa;
+  ob[0] = B0 * sb;
+  ob[1] = B1 * sb;
+  ob[2] = B2 * sb;
+}
+
 __global__ void k_ewald_gsum(const ynb_sys *sys, const int32_t *wave_sys, ynb_ewald ew, int64_t npad, double *gpos,
                              int accumulate) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -404,7 +491,7 @@ using namespace ynb;
 
 extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const double *pos, const double *charges,
                               int32_t precision, double *gpos, int32_t accumulate, double *out, ynb_stream stream) {
-  if (!b || !ew || !pos || !charges || !out || b->npad % 64 != 0 || ew->nk_tot % 256 != 0 || ew->kchunks < 1 ||
+  if (!b || !ew || !pos || !charges || !out || b->npad % 128 != 0 || ew->nk_tot % 256 != 0 || ew->kchunks < 1 ||
       (ew->seglen != 4 && ew->seglen != 8 && ew->seglen != 16) ||
       (precision != YNB_FP64 && precision != YNB_MIXED)) {
     set_error("ynb_ewald_reci: invalid arguments");
@@ -437,16 +524,17 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
     YNB_LAUNCHED();
   }
   const dim3 fgrid((unsigned)(npad / 64), (unsigned)ew->kchunks);
+  const dim3 fgrid2((unsigned)(npad / 128), (unsigned)ew->kchunks);
   if (precision == YNB_FP64) {
     if (ew->groups)
       YNB_LAUNCH("ewald_force", (k_ewald_force<YNB_FP64, true>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
     else
-      YNB_LAUNCH("ewald_force", (k_ewald_force<YNB_FP64, false>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
+      YNB_LAUNCH("ewald_force", (k_ewald_force2<YNB_FP64>), fgrid2, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
   } else {
     if (ew->groups)
       YNB_LAUNCH("ewald_force", (k_ewald_force<YNB_MIXED, true>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
     else
-      YNB_LAUNCH("ewald_force", (k_ewald_force<YNB_MIXED, false>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
+      YNB_LAUNCH("ewald_force", (k_ewald_force2<YNB_MIXED>), fgrid2, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
   }
   YNB_LAUNCHED();
   if (gpos) {

@@ -1,8 +1,9 @@
 """Batched device engine: lays out systems in HBM and drives libyaff_nb.so.
 
 A batch is any number of independent systems.  Their atoms are concatenated in
-padded ranges (each system starts at a multiple of 64 atoms, so one wavefront
-never spans two systems); every launch covers the whole batch.  A single
+padded ranges (each system starts at a multiple of 128 atoms, so neither a
+wavefront nor a two-wavefront block spans two systems); every launch covers
+the whole batch.  A single
 system -- the drop-in ForceField path -- is a batch of one.
 
 Device memory is held in torch tensors (plumbing only); every kernel is a
@@ -23,6 +24,7 @@ __all__ = ['PotSpec', 'EwaldSpec', 'NBSystem', 'Engine', 'ewald_kvectors']
 TWO_PI = 6.2831853071795864769
 FOUR_PI = 12.5663706143591729538
 CHUNK = 256          # atoms per structure-factor chunk (k_ewald_sk LDS tile)
+ALIGN = 128          # systems start at multiples of 128 atoms (k_ewald_force2 takes 128-atom blocks)
 KBLOCK = 256         # k vectors per reduction block
 
 
@@ -212,8 +214,8 @@ class Engine(object):
         off = 0
         for s in self.systems:
             self.atom0.append(off)
-            off += _round_up(max(s.natom, 1), 64)
-        self.npad = max(off, 64)
+            off += _round_up(max(s.natom, 1), ALIGN)
+        self.npad = max(off, ALIGN)
         self.wave_sys = np.full(self.npad//64, -1, np.int32)
         for i, s in enumerate(self.systems):
             a0 = self.atom0[i]
@@ -373,8 +375,9 @@ class Engine(object):
         self.nwork = int(sum(len(w) for w in work))
         self.spart_entries = max(spart0, 1)
         if kchunks is None:
-            nwaves = self.npad//64
-            kchunks = max(1, min(64, int(math.ceil(4096.0/max(nwaves, 1)))))
+            # about 4096 single-wave blocks in the force pass (128 atoms each)
+            nblk = self.npad//ALIGN
+            kchunks = max(1, min(64, int(math.ceil(4096.0/max(nblk, 1)))))
             maxrow = max([self.hsys[i].nrow for i in range(self.nsys)] + [1])
             kchunks = min(kchunks, maxrow)
         self.kchunks = kchunks

@@ -32,7 +32,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .engine import Engine, _round_up, ewald_kvectors_cached, CHUNK
+from .engine import Engine, _round_up, ewald_kvectors_cached, CHUNK, ALIGN
 from .batch import nbsystem, timestep, maxwell_boltzmann
 
 __all__ = ['Request', 'Server', 'estimate_bytes', 'poisson_requests', 'latency_summary', 'SERVER_TUNING']
@@ -97,7 +97,7 @@ def estimate_bytes(nbs, tuning=SERVER_TUNING):
     capacity, the Ewald tables and partial sums)."""
     if not nbs:
         return 0
-    npad = sum(_round_up(max(s.natom, 1), 64) for s in nbs)
+    npad = sum(_round_up(max(s.natom, 1), ALIGN) for s in nbs)
     kcap = max(_kcap_estimate(s) for s in nbs)
     total = npad*(8*kcap + 8*3*3 + 8*3*_lib.MAX_POT + 8*3 + 4 + 2*_lib.MAX_POT*8 + 64)
     chunk = tuning.get('ewald_chunk') or CHUNK
