@@ -309,8 +309,10 @@ def main():
 
     s, nb = rocksalt_system()
     n = s.natom
+    from yaff_amd import autotune
     eng = Engine([nb], precision='fp64')
     eng.ensure_nlist()
+    tuning = autotune.tune(eng, cache=None)
     eng.compute(build=False)
     eng.set_velocities([maxwell_boltzmann(s.masses, 300.0, 1)])
     eng.set_timestep(1.0*femtosecond)
@@ -351,7 +353,8 @@ def main():
            'dtype': 'f64', 'data': 'synthetic',
            'config': {'workload': 'rocksalt 6912 atoms (12x12x6 cells), ExpRep+LJ+EI rcut 11 A + Ewald reci '
                                   'gcut 0.32/A, FP64, velocity-Verlet MD step with nlist rebuild',
-                      'natom': n, 'half_pairs': half_pairs, 'kvectors': nk_real, 'parallelism': 'single GPU'},
+                      'natom': n, 'half_pairs': half_pairs, 'kvectors': nk_real, 'parallelism': 'single GPU',
+                      'tuning': tuning},
            'roofline': roofline, 'kernels': kt,
            'timing': 'value: K replays of a HIP graph holding one MD step; kernels: HIP events around each launch '
                      'of K eager steps on the launch stream',
@@ -365,6 +368,7 @@ def main():
         e64, g64, v64 = eng.results(0)
         engm = Engine([nb], precision='mixed')
         engm.ensure_nlist()
+        tuning_m = autotune.tune(engm, cache=None)
         engm.compute(build=False)
         em, gm, vm = engm.results(0)
         engm.set_velocities([maxwell_boltzmann(s.masses, 300.0, 1)])
@@ -378,6 +382,7 @@ def main():
                         'force_rms_rel_err': float(rms),
                         'force_max_abs_err': float(np.abs(gm - g64).max()),
                         'virial_rel_err': float(np.abs(vm - v64).max()/np.abs(v64).max()),
+                        'tuning': tuning_m,
                         'kernels': kernel_table(profm, K)}
 
     if args.copy_probe_mb:

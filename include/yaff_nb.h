@@ -101,7 +101,18 @@ typedef struct ynb_sys {
   int32_t chunk0, nchunk; /* atom chunks of the structure-factor pass */
   int64_t spart0;         /* offset of this system's partial structure factors, in k entries */
   int32_t nchunk_low, nchunk_high; /* chunks [0, low) below nlow, the last `high` above nhigh */
+  int32_t pair_form;      /* YNB_FORM_*: specialised pair kernel for this system's potential set */
+  int32_t pad1_;
 } ynb_sys;
+
+/* ynb_sys.pair_form: potential sets with a specialised totals kernel (no
+ * scaling rows; ynb_pairpar.rec holds the per-atom record).  Any other set,
+ * per-potential results (gpos_slots) or a slot mask that leaves out one of
+ * the system's potentials use the generic kernel. */
+#define YNB_FORM_GENERIC 0
+#define YNB_FORM_LJ 1     /* one LJ, no truncation or Switch3; rec = (sigma, sqrt(eps), 0, 0) */
+#define YNB_FORM_IONIC 2  /* ExpRep, LJ, EI (alpha > 0, point charges), no truncation, in this order;
+                             rec = (LJ sigma, sqrt(LJ eps), charge / sqrt(dielectric), ExpRep type) */
 
 /* Device tables shared by all launches over one batch. */
 typedef struct ynb_batch {
@@ -118,6 +129,9 @@ typedef struct ynb_batch {
 /* ynb_batch.flags */
 #define YNB_BATCH_ALLPAIRS 1    /* some system has natom > 0 and no bins (nvec < 3 or a huge search box) */
 #define YNB_BATCH_EXACT_BUILD 2 /* some binned system is not box_safe: exact re-test inside the build */
+#define YNB_BATCH_FORM_GENERIC 4 /* some system has pair_form YNB_FORM_GENERIC */
+#define YNB_BATCH_FORM_LJ 8      /* some system has pair_form YNB_FORM_LJ */
+#define YNB_BATCH_FORM_IONIC 16  /* some system has pair_form YNB_FORM_IONIC */
 
 /* Full neighbour list, one row of kcap entries per atom: entry k of atom i is
  * at i*kcap + k.
@@ -140,6 +154,7 @@ typedef struct ynb_pairpar {
                           EI charge, radius; ExpRep ffatype id (as double) */
   const double *cross; /* ExpRep amp/b tables, row-major nffatype^2 each */
   const ynb_scaling_row *stab; /* scaling rows (a > b, local indices), sorted per system */
+  const double *rec;   /* [npad][4] per-atom record of the specialised forms (YNB_FORM_*) */
   int32_t lanes;       /* lanes per atom in ynb_pair_compute: 16, 32 or 64 (0: 16) */
   int32_t pad_;
 } ynb_pairpar;

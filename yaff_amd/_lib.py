@@ -16,6 +16,8 @@ POT_LJ, POT_MM3, POT_EXPREP, POT_EI = 0, 1, 2, 3
 TR_NONE, TR_SWITCH3, TR_HAMMER = 0, 1, 2
 FP64, MIXED = 0, 1
 BATCH_ALLPAIRS, BATCH_EXACT_BUILD = 1, 2
+BATCH_FORM_GENERIC, BATCH_FORM_LJ, BATCH_FORM_IONIC = 4, 8, 16
+FORM_GENERIC, FORM_LJ, FORM_IONIC = 0, 1, 2
 
 ROW_DTYPE = np.dtype([('a', np.int64), ('b', np.int64), ('d', np.float64), ('dx', np.float64),
                       ('dy', np.float64), ('dz', np.float64), ('r0', np.int64), ('r1', np.int64),
@@ -38,7 +40,7 @@ class YnbSys(C.Structure):
                 ('alpha', C.c_double), ('dielectric', C.c_double), ('k0', C.c_int32), ('nk', C.c_int32),
                 ('row0', C.c_int32), ('nrow', C.c_int32), ('seg0', C.c_int32), ('nseg', C.c_int32),
                 ('chunk0', C.c_int32), ('nchunk', C.c_int32), ('spart0', C.c_int64), ('nchunk_low', C.c_int32),
-                ('nchunk_high', C.c_int32)]
+                ('nchunk_high', C.c_int32), ('pair_form', C.c_int32), ('pad1_', C.c_int32)]
 
 
 class YnbBatch(C.Structure):
@@ -52,8 +54,8 @@ class YnbNlist(C.Structure):
 
 
 class YnbPairpar(C.Structure):
-    _fields_ = [('par', C.c_void_p), ('cross', C.c_void_p), ('stab', C.c_void_p), ('lanes', C.c_int32),
-                ('pad_', C.c_int32)]
+    _fields_ = [('par', C.c_void_p), ('cross', C.c_void_p), ('stab', C.c_void_p), ('rec', C.c_void_p),
+                ('lanes', C.c_int32), ('pad_', C.c_int32)]
 
 
 class YnbEwald(C.Structure):

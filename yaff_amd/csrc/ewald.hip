@@ -397,40 +397,67 @@ __global__ __launch_bounds__(64) void k_ewald_force2(const ynb_sys *sys, const i
   ob[2] = B2 * sb;
 }
 
+// Gradient: sum of the kchunks partial gradients, one thread per (atom,
+// component), four interleaved accumulators combined in a fixed order.
 __global__ void k_ewald_gsum(const ynb_sys *sys, const int32_t *wave_sys, ynb_ewald ew, int64_t npad, double *gpos,
                              int accumulate) {
-  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= npad) return;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 3 * npad) return;
+  const int64_t g = t / 3;
   const int sid = wave_sys[g >> 6];
   if (sid < 0) return;
   const ynb_sys &S = sys[sid];
   if (S.nk == 0 || g - S.atom0 >= S.natom) return;
-  double G[3] = {0.0, 0.0, 0.0};
-  for (int kc = 0; kc < ew.kchunks; ++kc)
-    for (int c = 0; c < 3; ++c) G[c] += ew.gpart[((int64_t)kc * npad + g) * 3 + c];
-  for (int c = 0; c < 3; ++c) {
-    if (accumulate)
-      gpos[3 * g + c] += G[c];
-    else
-      gpos[3 * g + c] = G[c];
+  const int kcn = ew.kchunks;
+  const double *src = ew.gpart + t;  // gpart[(kc * npad + g) * 3 + c] = src[kc * 3 * npad]
+  const int64_t stride = 3 * npad;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int kc = 0;
+  for (; kc + 4 <= kcn; kc += 4) {
+    a0 += src[(int64_t)kc * stride];
+    a1 += src[(int64_t)(kc + 1) * stride];
+    a2 += src[(int64_t)(kc + 2) * stride];
+    a3 += src[(int64_t)(kc + 3) * stride];
   }
+  for (; kc < kcn; ++kc) a0 += src[(int64_t)kc * stride];
+  const double G = (a0 + a1) + (a2 + a3);
+  if (accumulate)
+    gpos[t] += G;
+  else
+    gpos[t] = G;
 }
 
 // out[s] = {E, V (3x3)}; ewald.c:128-149 (virial diagonal -= E, then 1/eps_r).
-__global__ void k_ewald_sysreduce(const ynb_sys *sys, int nsys, ynb_ewald ew, double *out) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsys) return;
+// One block per system: 256 threads take the k blocks t, t + 256, ... and a
+// fixed LDS tree adds them.
+__global__ __launch_bounds__(256) void k_ewald_sysreduce(const ynb_sys *sys, int nsys, ynb_ewald ew, double *out) {
+  const int s = blockIdx.x;
+  const int t = threadIdx.x;
   const ynb_sys &S = sys[s];
+  __shared__ double red[7][256];
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int b = S.k0 / 32; b < (S.k0 + S.nk) / 32; ++b)
-    for (int c = 0; c < 7; ++c) acc[c] += ew.kpart[8 * b + c];
-  const double e = acc[0];
-  const double f = (S.nk > 0) ? 1.0 / S.dielectric : 0.0;
-  double *o = out + 10 * (int64_t)s;
-  o[0] = e * f;
-  o[1] = (acc[1] - e) * f; o[2] = acc[4] * f; o[3] = acc[5] * f;
-  o[4] = acc[4] * f; o[5] = (acc[2] - e) * f; o[6] = acc[6] * f;
-  o[7] = acc[5] * f; o[8] = acc[6] * f; o[9] = (acc[3] - e) * f;
+  const int b0 = S.k0 / 32, b1 = (S.k0 + S.nk) / 32;
+  for (int b = b0 + t; b < b1; b += 256)
+#pragma unroll
+    for (int c = 0; c < 7; ++c) acc[c] += ew.kpart[8 * (int64_t)b + c];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) red[c][t] = acc[c];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off)
+#pragma unroll
+      for (int c = 0; c < 7; ++c) red[c][t] += red[c][t + off];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double e = red[0][0];
+    const double f = (S.nk > 0) ? 1.0 / S.dielectric : 0.0;
+    double *o = out + 10 * (int64_t)s;
+    o[0] = e * f;
+    o[1] = (red[1][0] - e) * f; o[2] = red[4][0] * f; o[3] = red[5][0] * f;
+    o[4] = red[4][0] * f; o[5] = (red[2][0] - e) * f; o[6] = red[6][0] * f;
+    o[7] = red[5][0] * f; o[8] = red[6][0] * f; o[9] = (red[3][0] - e) * f;
+  }
 }
 
 // ewald.c:278-349 for one system, rows processed in order by lane 0 of one
@@ -538,11 +565,11 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
   }
   YNB_LAUNCHED();
   if (gpos) {
-    hipLaunchKernelGGL(k_ewald_gsum, dim3((unsigned)ceil_div(npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys, *ew,
+    hipLaunchKernelGGL(k_ewald_gsum, dim3((unsigned)ceil_div(3 * npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys, *ew,
                        npad, gpos, accumulate);
     YNB_LAUNCHED();
   }
-  hipLaunchKernelGGL(k_ewald_sysreduce, dim3((unsigned)ceil_div(b->nsys, 64)), dim3(64), 0, st, b->sys, b->nsys, *ew,
+  hipLaunchKernelGGL(k_ewald_sysreduce, dim3((unsigned)b->nsys), dim3(256), 0, st, b->sys, b->nsys, *ew,
                      out);
   YNB_LAUNCHED();
   return YNB_OK;

@@ -230,6 +230,13 @@ __device__ inline T group_sum(T v) {
 // FP64; mixed mode takes exp/erfc in FP32 on FP64-reduced arguments,
 // accumulating gradient, energy and virial in FP64.
 //
+// A system goes to its specialised kernel when it has a form and the slot
+// mask keeps all of its potentials.
+__device__ inline bool form_applies(const ynb_sys &S, uint32_t mask) {
+  const uint32_t all = (1u << S.npot) - 1u;
+  return S.pair_form != YNB_FORM_GENERIC && (mask & all) == all;
+}
+
 // SLOTS: per-potential energy, virial and gradient (the drop-in parts need
 // them): one pass over the row per potential.  Otherwise one pass in which
 // the potentials of a pair are summed first, so that only totals are
@@ -250,6 +257,7 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
   const int sid = uniform_int(wave_sys[gi >> 6]);
   if (sid < 0) return;
   const ynb_sys &S = sys[sid];
+  if (!SLOTS && form_applies(S, mask)) return;  // a specialised kernel handles this system
   const int li = (int)(gi - S.atom0);
   const bool active = li < S.natom;
   const int npot = S.npot;
@@ -399,6 +407,160 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
     const int q = threadIdx.x;
     partial[(int64_t)blockIdx.x * (YNB_MAX_POT * 8) + q] =
         (q < ngroup * 8) ? ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q] : 0.0;
+  }
+}
+
+
+// Specialised totals kernels for the common potential sets (YNB_FORM_*): the
+// same lane layout, image vectors, exact near-cutoff treatment and block
+// partials as k_pair, with the potentials inlined and the per-atom
+// parameters read as one 32-byte record per atom.
+//   FORM_LJ:    pair_fn_lj (+ switch3), pair_pot.c:277-301, truncation.c:59-76
+//   FORM_IONIC: pair_fn_exprep + pair_fn_lj + pair_fn_ei (alpha > 0, point
+//               charges), pair_pot.c:473-489, 277-301, 711-751
+template <int PREC, int FORM, int G>
+__global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
+                                                   ynb_pairpar pp, const double *pos, ynb_nlist nl, uint32_t mask,
+                                                   double *gpos, int accumulate, double *partial) {
+  using PE = PotEval<PREC>;
+  using PF = PotFast<PREC>;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t gi = t / G;
+  const int s = (int)(t % G);
+  const int lane = threadIdx.x & 63;
+  if (gi >= npad) return;
+  const int sid = uniform_int(wave_sys[gi >> 6]);
+  if (sid < 0) return;
+  const ynb_sys &S = sys[sid];
+  if (S.pair_form != FORM || !form_applies(S, mask)) return;
+  const int li = (int)(gi - S.atom0);
+  const bool active = li < S.natom;
+  const int npot = S.npot;
+  // per-system constants (uniform)
+  const double rc0 = S.pot[0].rcut;
+  const double rc1 = FORM == YNB_FORM_IONIC ? S.pot[1].rcut : 0.0;
+  const double rc2 = FORM == YNB_FORM_IONIC ? S.pot[2].rcut : 0.0;
+  const double rcmax = fmax(rc0, fmax(rc1, rc2));
+  const double alpha = FORM == YNB_FORM_IONIC ? S.pot[2].alpha : 0.0;
+  const int trunc0 = S.pot[0].trunc;
+  const double w0 = S.pot[0].trunc_par;
+  const int nf = FORM == YNB_FORM_IONIC ? S.pot[0].nffatype : 0;
+  const double *amp = pp.cross + (FORM == YNB_FORM_IONIC ? S.pot[0].cross0 : 0);
+  const double *bex = amp + (int64_t)nf * nf;
+
+  double acc_e = 0.0, acc_v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, acc_g[3] = {0.0, 0.0, 0.0};
+  if (active) {
+    const double pi[3] = {pos[3 * gi], pos[3 * gi + 1], pos[3 * gi + 2]};
+    const double4 ri = *(const double4 *)(pp.rec + 4 * gi);
+    const int ti = FORM == YNB_FORM_IONIC ? (int)ri.w : 0;
+    const int n = min(nl.cnt[gi], nl.kcap);
+    const int32_t *rowj = nl.j + gi * nl.kcap;
+    const int32_t *rowi = nl.img + gi * nl.kcap;
+    for (int k = s; k < n; k += G) {
+      const int64_t gj = rowj[k];
+      int nn[3];
+      unpack_img(rowi[k], nn);
+      const double pj[3] = {pos[3 * gj], pos[3 * gj + 1], pos[3 * gj + 2]};
+      double u[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+      for (int a = 0; a < 3; ++a) {
+        const double na = (double)nn[a];
+        u[0] = fma(na, S.rvecs[3 * a], u[0]);
+        u[1] = fma(na, S.rvecs[3 * a + 1], u[1]);
+        u[2] = fma(na, S.rvecs[3 * a + 2], u[2]);
+      }
+      double dd = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      bool near_cut = false;
+      for (int p = 0; p < npot; ++p) near_cut |= fabs(dd - S.pot[p].rcut) <= 1e-12 * S.pot[p].rcut;
+      if (near_cut) {
+        // Yaff's own row vector decides the cutoff tests (no scaling rows in a form)
+        const int lj = (int)(gj - S.atom0);
+        const bool self = (gj == gi);
+        const bool i_is_a = li > lj || (li == lj);
+        const int nab[3] = {i_is_a ? nn[0] : -nn[0], i_is_a ? nn[1] : -nn[1], i_is_a ? nn[2] : -nn[2]};
+        double v[3];
+        int r[3];
+        yaff_row_vector(i_is_a ? pi : pj, i_is_a ? pj : pi, self, nab, S.rvecs, S.gvecs, S.nvec, S.rmax, v, r);
+        u[0] = i_is_a ? v[0] : -v[0];
+        u[1] = i_is_a ? v[1] : -v[1];
+        u[2] = i_is_a ? v[2] : -v[2];
+        dd = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      }
+      if (!(dd < rcmax)) continue;
+      const double d = dd;
+      const double inv = 1.0 / d;
+      const double inv2 = inv * inv;
+      const double4 rj = *(const double4 *)(pp.rec + 4 * gj);
+      double e_sum = 0.0, vg_sum = 0.0;
+      if (FORM == YNB_FORM_LJ) {
+        if (d < rc0) {
+          double g;
+          double e = PF::lj(ri.x, rj.x, ri.y * rj.y, inv, inv2, g);
+          if (trunc0 == YNB_TR_SWITCH3 && (e != 0.0 || g != 0.0)) {
+            double hg;
+            const double h = PE::switch3(d, rc0, w0, hg);
+            g = g * h + e * hg * inv;
+            e *= h;
+          }
+          e_sum = e;
+          vg_sum = g;
+        }
+      } else {
+        if (d < rc0) {  // ExpRep
+          const int ci = ti * nf + (int)rj.w;
+          double g;
+          e_sum += PF::exprep(amp[ci], bex[ci], d, inv, g);
+          vg_sum += g;
+        }
+        if (d < rc1) {  // LJ
+          double g;
+          e_sum += PF::lj(ri.x, rj.x, ri.y * rj.y, inv, inv2, g);
+          vg_sum += g;
+        }
+        if (d < rc2) {  // EI
+          double g;
+          e_sum += PF::ei_point(ri.z * rj.z, alpha, d, inv, inv2, g);
+          vg_sum += g;
+        }
+      }
+      acc_e += e_sum;
+      acc_g[0] -= u[0] * vg_sum;
+      acc_g[1] -= u[1] * vg_sum;
+      acc_g[2] -= u[2] * vg_sum;
+      acc_v[0] += u[0] * (u[0] * vg_sum);
+      acc_v[1] += u[1] * (u[1] * vg_sum);
+      acc_v[2] += u[2] * (u[2] * vg_sum);
+      acc_v[3] += u[0] * (u[1] * vg_sum);
+      acc_v[4] += u[0] * (u[2] * vg_sum);
+      acc_v[5] += u[1] * (u[2] * vg_sum);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) acc_g[c] = group_sum<G>(acc_g[c]);
+  if (active && s == 0 && gpos) {
+    for (int c = 0; c < 3; ++c) {
+      if (accumulate)
+        gpos[3 * gi + c] += acc_g[c];
+      else
+        gpos[3 * gi + c] = acc_g[c];
+    }
+  }
+  __shared__ double red[4][8];
+  const int wib = threadIdx.x >> 6;
+  const double e = wave_sum(acc_e);
+  double v[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) v[c] = wave_sum(acc_v[c]);
+  if (lane == 0) {
+    red[wib][0] = e;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) red[wib][1 + c] = v[c];
+    red[wib][7] = 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x < npot * 8) {
+    const int q = threadIdx.x;
+    partial[(int64_t)blockIdx.x * (YNB_MAX_POT * 8) + q] =
+        (q < 8) ? ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q] : 0.0;
   }
 }
 
@@ -562,9 +724,26 @@ extern "C" int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const
   else if (G == 32) YNB_PAIR(P, SL, 32); \
   else YNB_PAIR(P, SL, 64)
   if (precision == YNB_FP64) {
-    if (slots) { YNB_PAIR_G(YNB_FP64, true); } else { YNB_PAIR_G(YNB_FP64, false); }
+    if (slots) { YNB_PAIR_G(YNB_FP64, true); } else if (b->flags & YNB_BATCH_FORM_GENERIC) { YNB_PAIR_G(YNB_FP64, false); }
   } else {
-    if (slots) { YNB_PAIR_G(YNB_MIXED, true); } else { YNB_PAIR_G(YNB_MIXED, false); }
+    if (slots) { YNB_PAIR_G(YNB_MIXED, true); } else if (b->flags & YNB_BATCH_FORM_GENERIC) { YNB_PAIR_G(YNB_MIXED, false); }
+  }
+  if (!slots) {
+#define YNB_FORMK(P, F, GG)                                                                                    \
+  YNB_LAUNCH("pair", (k_pair_form<P, F, GG>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,  \
+             slot_mask, gpos, accumulate, partial)
+#define YNB_FORM_G(P, F)                 \
+  if (G == 16) YNB_FORMK(P, F, 16);      \
+  else if (G == 32) YNB_FORMK(P, F, 32); \
+  else YNB_FORMK(P, F, 64)
+    if (b->flags & YNB_BATCH_FORM_LJ) {
+      if (precision == YNB_FP64) { YNB_FORM_G(YNB_FP64, YNB_FORM_LJ); } else { YNB_FORM_G(YNB_MIXED, YNB_FORM_LJ); }
+    }
+    if (b->flags & YNB_BATCH_FORM_IONIC) {
+      if (precision == YNB_FP64) { YNB_FORM_G(YNB_FP64, YNB_FORM_IONIC); } else { YNB_FORM_G(YNB_MIXED, YNB_FORM_IONIC); }
+    }
+#undef YNB_FORM_G
+#undef YNB_FORMK
   }
 #undef YNB_PAIR_G
 #undef YNB_PAIR

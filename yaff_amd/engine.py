@@ -278,6 +278,7 @@ class Engine(object):
         for i in range(self.nsys):
             self.bin_sys[self.hsys[i].bin0:self.hsys[i].bin0 + self.hsys[i].nbin] = i
         self.par_h = par
+        self.rec_h = self._plan_forms()
         self.cross_h = np.concatenate(cross) if cross else np.zeros(1)
         self.stab_h = np.concatenate(stabs) if stabs else np.zeros(1, SCALING_DTYPE)
         if self.has_ewald:
@@ -402,8 +403,9 @@ class Engine(object):
         self.par_d = self._t(self.par_h)
         self.cross_d = self._t(self.cross_h)
         self.stab_d = self._t(np.frombuffer(self.stab_h.tobytes(), np.uint8))
+        self.rec_d = self._t(self.rec_h)
         self.pairpar = YnbPairpar(self.par_d.data_ptr(), self.cross_d.data_ptr(), self.stab_d.data_ptr(),
-                                  self.pair_lanes, 0)
+                                  self.rec_d.data_ptr(), self.pair_lanes, 0)
         # neighbour list
         self._alloc_nlist_work()
         self.cnt_d = torch.zeros(self.npad, dtype=torch.int32, device=dev)
@@ -433,6 +435,33 @@ class Engine(object):
         self.ekin2_d = torch.zeros(self.nsys, dtype=torch.float64, device=dev)
         self.md_partial_d = torch.zeros(self.npad//64, dtype=torch.float64, device=dev)
 
+    def _plan_forms(self):
+        """pair_form of every system (a specialised pair kernel for the
+        potential sets of the LJ fluid and of the rocksalt-type ionic force
+        field) and the per-atom records those kernels read."""
+        rec = np.zeros((self.npad, 4))
+        for i, s in enumerate(self.systems):
+            h = self.hsys[i]
+            h.pair_form = _lib.FORM_GENERIC
+            pots = s.pots
+            if any(len(s.stab if p.stab is None else p.stab) for p in pots) or s.natom == 0:
+                continue
+            a0, n = self.atom0[i], s.natom
+            kinds = [p.kind for p in pots]
+            if kinds == [_lib.POT_LJ] and pots[0].trunc in (_lib.TR_NONE, _lib.TR_SWITCH3) and pots[0].p1 is not None:
+                h.pair_form = _lib.FORM_LJ
+                rec[a0:a0 + n, 0] = np.asarray(pots[0].p0, dtype=float)
+                rec[a0:a0 + n, 1] = np.sqrt(np.asarray(pots[0].p1, dtype=float))
+            elif (kinds == [_lib.POT_EXPREP, _lib.POT_LJ, _lib.POT_EI] and all(p.trunc == TR_NONE for p in pots)
+                  and pots[2].alpha > 0 and (pots[2].p1 is None or not np.any(np.asarray(pots[2].p1)))
+                  and pots[1].p1 is not None):
+                h.pair_form = _lib.FORM_IONIC
+                rec[a0:a0 + n, 0] = np.asarray(pots[1].p0, dtype=float)
+                rec[a0:a0 + n, 1] = np.sqrt(np.asarray(pots[1].p1, dtype=float))
+                rec[a0:a0 + n, 2] = np.asarray(pots[2].p0, dtype=float)/math.sqrt(pots[2].dielectric)
+                rec[a0:a0 + n, 3] = np.asarray(pots[0].p0, dtype=float)
+        return rec
+
     def _batch_flags(self):
         """Build paths the batch needs: all-pairs (systems without bins) and the
         exact in-build re-test (binned systems that are not box_safe)."""
@@ -443,6 +472,8 @@ class Engine(object):
                 f |= _lib.BATCH_ALLPAIRS
             if h.nbin > 0 and not h.box_safe:
                 f |= _lib.BATCH_EXACT_BUILD
+            f |= {_lib.FORM_GENERIC: _lib.BATCH_FORM_GENERIC, _lib.FORM_LJ: _lib.BATCH_FORM_LJ,
+                  _lib.FORM_IONIC: _lib.BATCH_FORM_IONIC}[h.pair_form]
         return f
 
     def _max_bin_atoms(self):
