@@ -266,7 +266,22 @@ def batched(nsys, steps, warmup):
                                   'kernels_ms_per_step': {k: v['ms_per_step'] for k, v in kt.items()}}
         del md
         torch.cuda.empty_cache()
+    # the same MD with a Verlet skin of 1 A: lists rebuilt (on the device) only after a displacement of 0.5 A
+    from yaff_amd.units import angstrom
+    md = BatchMD(systems, precision='mixed', skin=1.0*angstrom)
+    md.engine.set_tuning(**out['batched_mixed']['tuning'])
+    md.warm(warmup)
+    md.capture(1)
+    dt = md.run(steps)
+    out['batched_mixed_skin1A'] = {'value': natom*steps/dt, 'ms_per_step': dt/steps*1e3, 'steps': steps,
+                                   'skin_angstrom': 1.0,
+                                   'note': 'same systems and tuning; Verlet skin, per-system rebuild decided on '
+                                           'the device inside the graph'}
+    progress('batched mixed skin 1A %.3g atom-evals/s' % (natom*steps/dt))
+    del md
+    torch.cuda.empty_cache()
     out['mixed_batched_vs_dropin_fp64'] = out['batched_mixed']['value']/out['dropin_fp64']['value']
+    out['mixed_batched_skin_vs_dropin_fp64'] = out['batched_mixed_skin1A']['value']/out['dropin_fp64']['value']
     out['mixed_batched_vs_fp64_batched'] = out['batched_mixed']['value']/out['batched_fp64']['value']
     return out
 

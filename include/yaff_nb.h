@@ -142,8 +142,11 @@ typedef struct ynb_nlist {
   int32_t *img;
   int32_t *cnt;        /* [npad] entries per atom (also counted past kcap) */
   int32_t *status;     /* [4]: [0] flags: 1 row > kcap, 2 image shift > 511 cells, 4 bin > max_bin_atoms.
-                          On a flag the caller reads cnt (rows) or the bin counts
+                          Flags are sticky (builds OR into them, the caller clears them), so
+                          a check after many graph-replayed steps sees every overflow.  On a
+                          flag the caller reads cnt (rows) or the bin counts
                           (ynb_nlist_bin_counts) to size the retry. */
+  const int32_t *rebuild; /* [nsys] per-system rebuild flags (ynb_nlist_skin_check) or NULL: rebuild all */
   int32_t kcap;
   int32_t pad_;
 } ynb_nlist;
@@ -221,6 +224,17 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
  * workspace to `counts` (device); used to size max_bin_atoms after a bin
  * overflow. */
 int ynb_nlist_bin_counts(const ynb_batch *b, const void *work, int32_t *counts, ynb_stream stream);
+
+/* Verlet skin: rebuild[s] = 1 when skin[s] <= 0 or some atom of system s
+ * moved by skin[s]/2 or more since pos_ref (then pos_ref of that system is
+ * set to pos); 0 otherwise.  Every pair within rcut = rc - skin then stays in
+ * the list built with rc: the entries carry fixed image translations, so a
+ * pair distance changes by at most the sum of the two displacements.
+ * (Yaff's row lists re-image with the minimum image, hence the factor
+ * 2 (rmax + 1) of yaff/pes/nlist.py:207-218.)  pos_ref starts as a huge
+ * value so that the first test rebuilds. */
+int ynb_nlist_skin_check(const ynb_batch *b, const double *pos, double *pos_ref, const double *skin,
+                         int32_t *rebuild, ynb_stream stream);
 
 /* Materialises system `s`'s half list as Yaff rows (local atom indices) in
  * nlist_build_low's order.  Synchronises.  Writes at most `cap` rows to

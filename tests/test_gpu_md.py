@@ -91,3 +91,68 @@ def test_nve_energy_conservation_lj864():
     ekin = np.array(ekin)
     print('std econs', econs.std(), 'std ekin', ekin.std(), flush=True)
     assert econs.std() < 0.05*ekin.std()
+
+
+def test_skin_md_matches_rebuild_every_step():
+    """A Verlet skin (rebuild only after a displacement of skin/2, decided on
+    the device) gives the trajectory of rebuilding every step: the pair set
+    within rcut is the same, only summation orders differ."""
+    import torch
+    from yaff_amd import workloads
+    from yaff_amd.batch import nbsystem, maxwell_boltzmann
+    from yaff_amd.engine import Engine
+    from yaff_amd.units import femtosecond, angstrom
+    print('case rocksalt 432 atoms, skin 0.15 A vs 0, 60 steps', flush=True)
+    s = workloads.rocksalt((3, 3, 6), seed=5, angles=(87.0, 92.0, 95.0))
+    vel = maxwell_boltzmann(s.masses, 300.0, 4)
+    engs = []
+    for skin in (0.0, 0.15*angstrom):
+        e = Engine([nbsystem(s, skin)], precision='fp64')
+        e.ensure_nlist()
+        e.compute(build=False)
+        e.set_velocities([vel])
+        e.set_timestep(1.0*femtosecond)
+        engs.append(e)
+    rebuilds = 0
+    for step in range(60):
+        for e in engs:
+            e.md_step()
+        rebuilds += int(engs[1].rebuild_d[0].item())
+    assert engs[1].nlist_status()[0] == 0 and engs[0].nlist_status()[0] == 0
+    p0, p1 = engs[0].pos_d.cpu().numpy(), engs[1].pos_d.cpu().numpy()
+    e0, e1 = engs[0].total_d.cpu().numpy()[0, 0], engs[1].total_d.cpu().numpy()[0, 0]
+    print('rebuilds with skin: %d of 60; max |dpos| %.2e, rel dE %.2e' % (rebuilds, np.abs(p0 - p1).max(),
+                                                                      abs(e0 - e1)/abs(e0)), flush=True)
+    assert 1 <= rebuilds < 60
+    assert np.abs(p0 - p1).max() < 1e-9
+    assert abs(e0 - e1) <= 1e-10*abs(e0)
+
+
+def test_skin_graph_replay_matches_eager():
+    import torch
+    from yaff_amd import workloads
+    from yaff_amd.batch import nbsystem, maxwell_boltzmann
+    from yaff_amd.engine import Engine
+    from yaff_amd.units import femtosecond, angstrom
+    print('case LJ 256 + rocksalt 256, skin 0.8 A, graph vs eager', flush=True)
+    systems = [workloads.lj_fluid((4, 4, 4), seed=1, angles=(85.0, 95.0, 90.0)),
+               workloads.rocksalt((4, 4, 2), seed=2, angles=(91.0, 88.0, 93.0))]
+    vel = [maxwell_boltzmann(x.masses, 300.0, 7 + i) for i, x in enumerate(systems)]
+
+    def make():
+        e = Engine([nbsystem(x, 0.8*angstrom) for x in systems], precision='mixed')
+        e.ensure_nlist()
+        e.compute(build=False)
+        e.set_velocities(vel)
+        e.set_timestep([2.0*femtosecond, 1.0*femtosecond])
+        return e
+    ea, eb = make(), make()
+    for _ in range(12):
+        ea.md_step()
+    g = eb.capture_md(3)
+    for _ in range(4):
+        g.replay()
+        torch.cuda.synchronize()
+    assert ea.nlist_status()[0] == 0 and eb.nlist_status()[0] == 0
+    assert torch.equal(ea.pos_d, eb.pos_d) and torch.equal(ea.vel_d, eb.vel_d)
+    assert torch.equal(ea.total_d, eb.total_d)

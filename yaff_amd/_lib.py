@@ -50,7 +50,7 @@ class YnbBatch(C.Structure):
 
 class YnbNlist(C.Structure):
     _fields_ = [('j', C.c_void_p), ('img', C.c_void_p), ('cnt', C.c_void_p), ('status', C.c_void_p),
-                ('kcap', C.c_int32), ('pad_', C.c_int32)]
+                ('rebuild', C.c_void_p), ('kcap', C.c_int32), ('pad_', C.c_int32)]
 
 
 class YnbPairpar(C.Structure):
@@ -70,7 +70,8 @@ class YnbEwald(C.Structure):
 EXPORTS = ['ynb_version', 'ynb_last_error', 'ynb_sys_plan', 'ynb_nlist_workspace_bytes',
            'ynb_rows_workspace_bytes', 'ynb_nlist_build', 'ynb_nlist_rows', 'ynb_nlist_recompute',
            'ynb_pair_compute', 'ynb_pair_rows', 'ynb_ewald_reci', 'ynb_ewald_corr', 'ynb_verlet_kick_drift', 'ynb_verlet_kick',
-           'ynb_total', 'ynb_profile_enable', 'ynb_profile_count', 'ynb_profile_read', 'ynb_nlist_bin_counts']
+           'ynb_total', 'ynb_profile_enable', 'ynb_profile_count', 'ynb_profile_read', 'ynb_nlist_bin_counts',
+           'ynb_nlist_skin_check']
 
 
 class YnbError(RuntimeError):
@@ -98,6 +99,7 @@ def load(path=LIB_PATH):
     L.ynb_rows_workspace_bytes.argtypes = [i64, i64]
     L.ynb_nlist_build.argtypes = [P(YnbBatch), vp, vp, sz, P(YnbNlist), vp]
     L.ynb_nlist_bin_counts.argtypes = [P(YnbBatch), vp, vp, vp]
+    L.ynb_nlist_skin_check.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp]
     L.ynb_nlist_rows.argtypes = [P(YnbBatch), i32, vp, P(YnbNlist), vp, sz, vp, i64, P(i64), vp]
     L.ynb_nlist_recompute.argtypes = [P(YnbSys), vp, vp, vp, i64, vp]
     L.ynb_pair_compute.argtypes = [P(YnbBatch), P(YnbPairpar), vp, P(YnbNlist), C.c_uint32, i32, vp, i32, vp,

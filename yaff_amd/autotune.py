@@ -19,7 +19,7 @@ from . import _lib
 __all__ = ['tune', 'cache_key']
 
 CANDIDATES = [
-    ('pair_lanes', (16, 32, 64), ('pair',)),
+    ('pair_lanes', (16, 32, 64), ('pair', 'pair_reduce')),
     ('seglen', (4, 8, 16), ('ewald_sk', 'ewald_kreduce')),
     ('ewald_chunk', (32, 64, 128, 256), ('ewald_sk', 'ewald_kreduce')),
     ('kchunks', None, ('ewald_force',)),

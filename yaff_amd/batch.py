@@ -22,19 +22,21 @@ __all__ = ['nbsystem', 'forcefield', 'timestep', 'maxwell_boltzmann', 'BatchMD']
 
 
 def nbsystem(s, skin=0.0):
-    """Engine input for a workloads.lj_fluid or workloads.rocksalt system."""
+    """Engine input for a workloads.lj_fluid or workloads.rocksalt system;
+    with skin > 0 the engine's MD steps rebuild its list only when an atom
+    has moved by skin/2."""
     sp = s.spec
     n = s.natom
     if sp.kind == 'lj':
         pots = [PotSpec(_lib.POT_LJ, sp.rcut, _lib.TR_SWITCH3, sp.width, np.full(n, sp.sigma),
                         np.full(n, sp.epsilon))]
-        return NBSystem(s.pos, s.cell, pots, rc=sp.rcut + skin, masses=s.masses)
+        return NBSystem(s.pos, s.cell, pots, masses=s.masses, skin=skin)
     pots = [PotSpec(_lib.POT_EXPREP, sp.rcut, p0=s.ffatype_ids.astype(float),
                     amp_cross=np.full((2, 2), sp.exprep_a), b_cross=np.full((2, 2), sp.exprep_b)),
             PotSpec(_lib.POT_LJ, sp.rcut, p0=np.full(n, sp.lj_sigma), p1=np.full(n, sp.lj_epsilon)),
             PotSpec(_lib.POT_EI, sp.rcut, p0=s.charges, alpha=sp.alpha)]
-    return NBSystem(s.pos, s.cell, pots, rc=sp.rcut + skin, ewald=EwaldSpec(sp.alpha, sp.gcut), charges=s.charges,
-                    masses=s.masses)
+    return NBSystem(s.pos, s.cell, pots, ewald=EwaldSpec(sp.alpha, sp.gcut), charges=s.charges, masses=s.masses,
+                    skin=skin)
 
 
 def forcefield(s, skin=0.0):
@@ -79,9 +81,9 @@ def maxwell_boltzmann(masses, temp, seed):
 class BatchMD(object):
     """Velocity-Verlet MD of a batch of systems, HIP-graph replayed."""
 
-    def __init__(self, systems, precision='mixed', tuning=None, seed=0, temps=None):
+    def __init__(self, systems, precision='mixed', tuning=None, seed=0, temps=None, skin=0.0):
         self.systems = list(systems)
-        self.engine = Engine([nbsystem(s) for s in self.systems], precision=precision)
+        self.engine = Engine([nbsystem(s, skin) for s in self.systems], precision=precision)
         if tuning:
             self.engine.set_tuning(**tuning)
         eng = self.engine

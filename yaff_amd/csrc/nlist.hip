@@ -61,11 +61,12 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
 // Fractional wrap and bin of every atom of the binned systems; each atom
 // takes a slot of its bin (arrival order, made deterministic by k_bin_order).
 __global__ void k_bin(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, const double *pos, int32_t cb,
-                      int32_t *bin_count, int32_t *slot_idx, double4 *xw, int4 *wr, int32_t *status) {
+                      int32_t *bin_count, int32_t *slot_idx, double4 *xw, int4 *wr, int32_t *status,
+                      const int32_t *rebuild) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (g >= npad) return;  // whole waves: npad is a multiple of 64
   const int sid = uniform_int(wave_sys[g >> 6]);
-  if (sid < 0) return;
+  if (sid < 0 || (rebuild && !rebuild[sid])) return;
   const ynb_sys &S = sys[sid];
   int fill = 0;
   if (g - S.atom0 < S.natom && S.nbin != 0) {
@@ -101,9 +102,11 @@ __global__ void k_bin(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
 // does not depend on the order of the atomics in k_bin.
 __global__ __launch_bounds__(256) void k_bin_order(int32_t cb, const int32_t *bin_count, const int32_t *slot_idx,
                                                    const double4 *xw, const int4 *wr, double4 *sxw, int4 *sw,
-                                                   int32_t *bin_start, int32_t *bin_end) {
+                                                   int32_t *bin_start, int32_t *bin_end, const int32_t *bin_sys,
+                                                   const int32_t *rebuild) {
   __shared__ int32_t l_idx[1024];
   const int b = blockIdx.x;
+  if (rebuild && !rebuild[bin_sys[b]]) return;
   const int n = min(bin_count[b], cb);
   for (int e = threadIdx.x; e < n; e += blockDim.x) l_idx[e] = slot_idx[(int64_t)b * cb + e];
   __syncthreads();
@@ -170,6 +173,7 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *sys, co
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int sid = bin_sys[b];
+  if (nl.rebuild && !nl.rebuild[sid]) return;
   const ynb_sys &S = sys[sid];
   const int start = bin_start[b], end = bin_end[b];
   if (start + (int)blockIdx.y * BW >= end) return;
@@ -338,6 +342,45 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *sys, co
   if (bad) atomicOr(nl.status, 2);
 }
 
+
+// One block per system: the largest squared displacement since pos_ref
+// (order-independent maximum), the rebuild flag, and pos_ref := pos when set.
+__global__ __launch_bounds__(256) void k_skin_check(const ynb_sys *sys, const double *pos, double *pos_ref,
+                                                    const double *skin, int32_t *rebuild) {
+  const int s = blockIdx.x;
+  const ynb_sys &S = sys[s];
+  const int t = threadIdx.x;
+  __shared__ double red[256];
+  __shared__ int flag;
+  double m = 0.0;
+  for (int a = t; a < S.natom; a += 256) {
+    const int64_t g = S.atom0 + a;
+    const double dx = pos[3 * g] - pos_ref[3 * g], dy = pos[3 * g + 1] - pos_ref[3 * g + 1],
+                 dz = pos[3 * g + 2] - pos_ref[3 * g + 2];
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    m = (d2 > m || d2 != d2) ? (d2 != d2 ? INFINITY : d2) : m;
+  }
+  red[t] = m;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off) red[t] = fmax(red[t], red[t + off]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double h = 0.5 * skin[s];
+    flag = (skin[s] <= 0.0 || !(red[0] < h * h)) ? 1 : 0;
+    rebuild[s] = flag;
+  }
+  __syncthreads();
+  if (flag)
+    for (int a = t; a < S.natom; a += 256) {
+      const int64_t g = S.atom0 + a;
+      pos_ref[3 * g] = pos[3 * g];
+      pos_ref[3 * g + 1] = pos[3 * g + 1];
+      pos_ref[3 * g + 2] = pos[3 * g + 2];
+    }
+}
+
 // All-pairs build for systems without bins (nvec < 3): every partner and
 // every image of the full box [-rmax, rmax]^nvec, tested exactly.
 __global__ void k_build_allpairs(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
@@ -345,7 +388,7 @@ __global__ void k_build_allpairs(const ynb_sys *sys, const int32_t *wave_sys, in
   const int64_t gi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (gi >= npad) return;
   const int sid = wave_sys[gi >> 6];
-  if (sid < 0) return;
+  if (sid < 0 || (nl.rebuild && !nl.rebuild[sid])) return;
   const ynb_sys &S = sys[sid];
   const int li = (int)(gi - S.atom0);
   if (li >= S.natom || S.nbin != 0) return;
@@ -657,15 +700,13 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
   const int64_t npad = b->npad;
   const int tb = 256;
   const int nb = (int)ceil_div(npad, tb);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, nl->status, 4, 0);
-  YNB_LAUNCHED();
   if (b->nbin > 0) {
     hipLaunchKernelGGL(k_fill_i32, dim3((int)ceil_div(b->nbin, tb)), dim3(tb), 0, st, w.bin_count,
                        (int64_t)b->nbin, 0);
     YNB_LAUNCH("nlist_bin", k_bin, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, cb, w.bin_count,
-               w.slot_idx, w.xw, w.wr, nl->status);
+               w.slot_idx, w.xw, w.wr, nl->status, nl->rebuild);
     YNB_LAUNCH("nlist_bin_order", k_bin_order, dim3(b->nbin), dim3(256), 0, st, cb, w.bin_count, w.slot_idx, w.xw,
-               w.wr, w.sxw, w.sw, w.bin_start, w.bin_end);
+               w.wr, w.sxw, w.sw, w.bin_start, w.bin_end, b->bin_sys, nl->rebuild);
     const int ychunks = (int)ceil_div(cb, BW) < 64 ? (int)ceil_div(cb, BW) : 64;
     if (b->flags & YNB_BATCH_EXACT_BUILD)
       YNB_LAUNCH("nlist_build_binned", k_build_binned<true>, dim3(b->nbin, ychunks), dim3(BW * 64), 0, st, b->sys,
@@ -683,6 +724,19 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
   return YNB_OK;
 }
 
+
+int ynb_nlist_skin_check(const ynb_batch *b, const double *pos, double *pos_ref, const double *skin,
+                         int32_t *rebuild, ynb_stream stream) {
+  if (!b || !pos || !pos_ref || !skin || !rebuild) {
+    set_error("ynb_nlist_skin_check: invalid arguments");
+    return YNB_EINVAL;
+  }
+  if (b->nsys == 0) return YNB_OK;
+  YNB_LAUNCH("nlist_skin_check", k_skin_check, dim3((unsigned)b->nsys), dim3(256), 0, (hipStream_t)stream, b->sys,
+             pos, pos_ref, skin, rebuild);
+  YNB_LAUNCHED();
+  return YNB_OK;
+}
 
 int ynb_nlist_bin_counts(const ynb_batch *b, const void *work, int32_t *counts, ynb_stream stream) {
   if (!b || !work || !counts) {

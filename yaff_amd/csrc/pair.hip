@@ -564,38 +564,42 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
   }
 }
 
-// One block per system: for each of the 32 values, 32 threads sum strided
-// subsets of the system's block partials, then a fixed tree adds the 32.
-// out[s][p] = {E, V00 V01 V02 V10 V11 V12 V20 V21 V22} with the 1/2 of the
-// full list applied.
+// One block per system: the block partials of the system's atoms are summed
+// by nval/8 value groups x (1024 / nval) strided parts (nval = 8: totals in
+// slot 0; nval = 8 npot: per-potential slots), then a fixed LDS tree adds
+// the parts.  out[s][p] = {E, V00 V01 V02 V10 V11 V12 V20 V21 V22} with the
+// 1/2 of the full list applied; slots past nval/8 are written as zeros.
 __global__ __launch_bounds__(1024) void k_pair_reduce(const ynb_sys *sys, const double *partial, double *out,
-                                                      int apb) {
+                                                      int apb, int totals) {
   const int s = blockIdx.x;
   const ynb_sys &S = sys[s];
   const int t = threadIdx.x;
-  const int q = t & 31, part = t >> 5;  // value, one of 32 strided parts
-  // blocks of apb atoms covering the system (atom0 is a multiple of 64)
+  const int nval = totals ? 8 : 8 * S.npot;
+  const int nval_p2 = nval <= 8 ? 8 : (nval <= 16 ? 16 : 32);
+  const int nparts = 1024 / nval_p2;
+  const int q = t % nval_p2, part = t / nval_p2;
+  // blocks of apb atoms covering the system (atom0 is a multiple of 128)
   const int64_t b0 = S.atom0 / apb, b1 = (S.atom0 + S.natom + apb - 1) / apb;
-  __shared__ double red[32][33];
+  __shared__ double red[1024];
   double a[4] = {0, 0, 0, 0};
-  if (q < S.npot * 8) {
-    for (int64_t b = b0 + part; b < b1; b += 128) {
+  if (q < nval) {
+    for (int64_t b = b0 + part; b < b1; b += 4 * nparts) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int64_t bu = b + 32 * u;
+        const int64_t bu = b + (int64_t)nparts * u;
         if (bu < b1) a[u] += partial[bu * (YNB_MAX_POT * 8) + q];
       }
     }
   }
-  red[part][q] = (a[0] + a[1]) + (a[2] + a[3]);
+  red[t] = (a[0] + a[1]) + (a[2] + a[3]);
   __syncthreads();
-  for (int off = 16; off > 0; off >>= 1) {
-    if (part < off) red[part][q] += red[part + off][q];
+  for (int off = nparts / 2; off > 0; off >>= 1) {
+    if (part < off) red[t] += red[t + off * nval_p2];
     __syncthreads();
   }
   if (t < S.npot) {
     double r[8];
-    for (int c = 0; c < 8; ++c) r[c] = 0.5 * red[0][t * 8 + c];
+    for (int c = 0; c < 8; ++c) r[c] = (t * 8 < nval) ? 0.5 * red[t * 8 + c] : 0.0;
     double *o = out + ((int64_t)s * YNB_MAX_POT + t) * 10;
     o[0] = r[0];
     o[1] = r[1]; o[2] = r[4]; o[3] = r[5];
@@ -748,7 +752,8 @@ extern "C" int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const
 #undef YNB_PAIR_G
 #undef YNB_PAIR
   YNB_LAUNCHED();
-  YNB_LAUNCH("pair_reduce", k_pair_reduce, dim3(b->nsys), dim3(1024), 0, st, b->sys, partial, out, 256 / G);
+  YNB_LAUNCH("pair_reduce", k_pair_reduce, dim3(b->nsys), dim3(1024), 0, st, b->sys, partial, out, 256 / G,
+             slots ? 0 : 1);
   YNB_LAUNCHED();
   return YNB_OK;
 }

@@ -55,16 +55,21 @@ class EwaldSpec(object):
 
 class NBSystem(object):
     """Input of the engine for one system.  ``rc`` is the neighbour-list radius
-    (largest pair cutoff plus skin); rmax defaults to nlist.py:132."""
+    (default: largest pair cutoff plus ``skin``); rmax defaults to
+    nlist.py:132.  With skin > 0 the engine's MD steps rebuild the list of the
+    system only when an atom has moved by skin/2 (ynb_nlist_skin_check)."""
 
     def __init__(self, pos, cell, pots, rc=None, rmax=None, nlow=0, nhigh=-1, stab=None, ewald=None,
-                 charges=None, masses=None):
+                 charges=None, masses=None, skin=0.0):
         self.pos = np.ascontiguousarray(pos, dtype=np.float64)
         self.cell = cell
         self.pots = list(pots)
         if len(self.pots) > MAX_POT:
             raise ValueError('at most %d pair potentials per system' % MAX_POT)
-        self.rc = float(rc if rc is not None else max([p.rcut for p in self.pots] + [0.0]))
+        if skin < 0:
+            raise ValueError('The skin parameter must be positive.')
+        self.skin = float(skin)
+        self.rc = float(rc if rc is not None else max([p.rcut for p in self.pots] + [0.0]) + self.skin)
         if rmax is None:
             rmax = np.ceil(self.rc/cell.rspacings - 0.5).astype(np.int64) if cell.nvec > 0 else np.zeros(0, int)
         self.rmax = np.asarray(rmax, dtype=np.int64)
@@ -410,6 +415,11 @@ class Engine(object):
         self._alloc_nlist_work()
         self.cnt_d = torch.zeros(self.npad, dtype=torch.int32, device=dev)
         self.status_d = torch.zeros(4, dtype=torch.int32, device=dev)
+        skins = np.array([s.skin for s in self.systems], dtype=np.float64)
+        self.use_skin = bool(np.any(skins > 0))
+        self.skin_d = self._t(skins if len(skins) else np.zeros(1))
+        self.rebuild_d = torch.ones(max(self.nsys, 1), dtype=torch.int32, device=dev)
+        self.pos_ref_d = torch.full((self.npad, 3), 1e150, dtype=torch.float64, device=dev)
         self._alloc_nlist(kcap if kcap is not None else self._estimate_kcap())
         self.partial_d = torch.zeros((self.npad//4)*MAX_POT*8, dtype=torch.float64, device=dev)
         self.pair_out_d = torch.zeros((self.nsys, MAX_POT, 10), dtype=torch.float64, device=dev)
@@ -550,7 +560,9 @@ class Engine(object):
         self.nbr_j_d = torch.zeros(self.kcap*self.npad, dtype=torch.int32, device=self.device)
         self.nbr_img_d = torch.zeros(self.kcap*self.npad, dtype=torch.int32, device=self.device)
         self.nlist = YnbNlist(self.nbr_j_d.data_ptr(), self.nbr_img_d.data_ptr(), self.cnt_d.data_ptr(),
-                              self.status_d.data_ptr(), self.kcap, 0)
+                              self.status_d.data_ptr(), None, self.kcap, 0)
+        if hasattr(self, 'pos_ref_d'):
+            self.pos_ref_d.fill_(1e150)   # new capacity: the next skin test rebuilds everything
 
     # ------------------------------------------------------------------
     @staticmethod
@@ -569,16 +581,33 @@ class Engine(object):
         a0 = self.atom0[i]
         self.pos_d[a0:a0 + len(pos)].copy_(torch.from_numpy(np.ascontiguousarray(pos, dtype=np.float64)))
 
-    def build_nlist(self):
-        check(self.lib.ynb_nlist_build(self.batch, self.pos_d.data_ptr(), self.nl_work_d.data_ptr(),
-                                       self.nl_work_bytes, self.nlist, self.stream()))
+    def build_nlist(self, force=True):
+        """Builds the list of every system (force) or, with a skin, of the
+        systems whose atoms moved by skin/2 since their last build (decided
+        on the device, so the step can be captured in a graph)."""
+        if force or not self.use_skin:
+            self.nlist.rebuild = None
+            check(self.lib.ynb_nlist_build(self.batch, self.pos_d.data_ptr(), self.nl_work_d.data_ptr(),
+                                           self.nl_work_bytes, self.nlist, self.stream()))
+            if self.use_skin:
+                self.pos_ref_d.copy_(self.pos_d)
+        else:
+            check(self.lib.ynb_nlist_skin_check(self.batch, self.pos_d.data_ptr(), self.pos_ref_d.data_ptr(),
+                                                self.skin_d.data_ptr(), self.rebuild_d.data_ptr(), self.stream()))
+            self.nlist.rebuild = self.rebuild_d.data_ptr()
+            check(self.lib.ynb_nlist_build(self.batch, self.pos_d.data_ptr(), self.nl_work_d.data_ptr(),
+                                           self.nl_work_bytes, self.nlist, self.stream()))
+            self.nlist.rebuild = None
         self.nbuilds += 1
 
     def nlist_status(self):
-        """(flags, longest row) of the last build; synchronises.  Flags: 1 a
-        row exceeded kcap, 2 an image shift exceeded 511 cells, 4 a bin
-        exceeded its capacity.  The longest row is read only after a flag."""
+        """(flags, longest row) of the builds since the last call; synchronises
+        and clears the (sticky) flags.  Flags: 1 a row exceeded kcap, 2 an
+        image shift exceeded 511 cells, 4 a bin exceeded its capacity.  The
+        longest row is read only after a flag."""
         flags = int(self.status_d[0].item())
+        if flags:
+            self.status_d.zero_()
         longest = 0
         if flags & 1:
             longest = int(self.cnt_d.max().item())
@@ -602,6 +631,7 @@ class Engine(object):
             self._alloc_nlist_work()
         if flags & 1:
             self._alloc_nlist(int(longest*1.1) + 8)
+        self.pos_ref_d.fill_(1e150)   # lists of the failed build are partial: rebuild every system next
 
     def ensure_nlist(self):
         """Builds the list, growing capacities until every row and bin fits."""
@@ -640,7 +670,7 @@ class Engine(object):
         system; energy+virial totals in total_d, gradient in gpos_d.
         Asynchronous (call nlist_status() to validate the list)."""
         if build:
-            self.build_nlist()
+            self.build_nlist(force=False)
         self.pair(gpos=gpos, accumulate=False)
         self.ewald_reci(gpos=gpos, accumulate=True)
         self.total()
