@@ -57,6 +57,9 @@ def parse():
     ap.add_argument('--no-served', action='store_true')
     ap.add_argument('--serve-requests', type=int, default=150)
     ap.add_argument('--serve-rates', type=str, default='25,100,400')
+    ap.add_argument('--tuning', type=str, default=None,
+                    help='JSON launch parameters for the single-system engines (skips the autotuner, e.g. '
+                         'for profiler runs whose kernel averages must not mix parameter sets)')
     ap.add_argument('--copy-probe-mb', type=int, default=0,
                     help='after the timed region, run an elementwise FP64 copy of this many MiB (a kernel of known '
                          'HBM traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)')
@@ -325,9 +328,17 @@ def main():
     s, nb = rocksalt_system()
     n = s.natom
     from yaff_amd import autotune
+    fixed = json.loads(args.tuning) if args.tuning else None
+
+    def tune(e):
+        if fixed:
+            e.set_tuning(**fixed)
+            return e.tuning()
+        return autotune.tune(e, cache=None)
+
     eng = Engine([nb], precision='fp64')
     eng.ensure_nlist()
-    tuning = autotune.tune(eng, cache=None)
+    tuning = tune(eng)
     eng.compute(build=False)
     eng.set_velocities([maxwell_boltzmann(s.masses, 300.0, 1)])
     eng.set_timestep(1.0*femtosecond)
@@ -383,7 +394,7 @@ def main():
         e64, g64, v64 = eng.results(0)
         engm = Engine([nb], precision='mixed')
         engm.ensure_nlist()
-        tuning_m = autotune.tune(engm, cache=None)
+        tuning_m = tune(engm)
         engm.compute(build=False)
         em, gm, vm = engm.results(0)
         engm.set_velocities([maxwell_boltzmann(s.masses, 300.0, 1)])

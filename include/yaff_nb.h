@@ -10,8 +10,8 @@
  * Work is batched: a launch covers every system of a ynb_batch.  Atoms of
  * system s live at [sys.atom0, sys.atom0 + sys.natom) of the padded atom
  * arrays; atom0 is a multiple of 128 so that neither a wavefront nor the
- * two-wavefront atom blocks of the Ewald force pass hold atoms of two
- * systems.  A single system is a batch of one.
+ * 128-atom blocks of the Ewald force pass hold atoms of two systems.  A
+ * single system is a batch of one.
  *
  * Reference interfaces replaced (molmod/yaff):
  *   ynb_nlist_build + ynb_nlist_rows   <- nlist_build_low   yaff/pes/nlist.h:37-39
@@ -96,9 +96,9 @@ typedef struct ynb_sys {
   /* Ewald reciprocal part (nk == 0: none) */
   double alpha, dielectric;
   int32_t k0, nk;         /* k vectors [k0, k0+nk) of ynb_ewald.kvec; k0 and nk multiples of 256 */
-  int32_t row0, nrow;     /* k rows (runs of consecutive g0 at fixed g1, g2) */
-  int32_t seg0, nseg;     /* segments of <= seglen k vectors of one row */
-  int32_t chunk0, nchunk; /* atom chunks of the structure-factor pass */
+  int32_t frb0, nfrb;     /* row blocks of the force pass: [frb0, frb0+nfrb) of ynb_ewald.fblk */
+  int32_t nslot, pad2_;   /* partial-gradient slots of the force pass: ceil(nfrb / 4) */
+  int32_t chunk0, nchunk; /* atom chunks of the structure-factor pass (64 * apl atoms each) */
   int64_t spart0;         /* offset of this system's partial structure factors, in k entries */
   int32_t nchunk_low, nchunk_high; /* chunks [0, low) below nlow, the last `high` above nhigh */
   int32_t pair_form;      /* YNB_FORM_*: specialised pair kernel for this system's potential set */
@@ -162,29 +162,36 @@ typedef struct ynb_pairpar {
   int32_t pad_;
 } ynb_pairpar;
 
-/* Ewald reciprocal-space tables and work buffers (all device). */
+/* Ewald reciprocal-space tables and work buffers (all device).
+ *
+ * The k vectors of a system form rows (runs of consecutive g0 at fixed g1,
+ * g2, in the order g2, g1 major, g0 ascending).  Both passes walk blocks of
+ * whole rows: a lane holds atoms, seeds exp(i k.r) at each row start and
+ * steps it along g0.  The structure-factor pass covers every (k block, atom
+ * chunk) work item; the force pass gives each wavefront one k block of a
+ * system and 128 atoms, and a block of 4 wavefronts sums its 4 k blocks. */
 typedef struct ynb_ewald {
   const double *kvec;     /* [nk_tot][4]: kx, ky, kz and c_k = 4 pi/V exp(-k^2/4 alpha^2)/k^2 (0: padding) */
-  const int32_t *krow;    /* [nrow_tot][4]: g1, g2, first g0, count */
+  const int32_t *krow;    /* [nrow_tot][4]: g0 of the row's first k, g1, g2, number of k in the row */
   const int32_t *krow_k;  /* [nrow_tot]: global index of the row's first k */
-  const int32_t *seg;     /* [nseg_tot][4]: first g0, g1, g2, global index of the first k */
-  const int32_t *seg_len; /* [nseg_tot]: k vectors in the segment */
+  const int32_t *skblk;   /* [nskblk][2]: global row range [begin, end) of each structure-factor block */
+  const int32_t *fblk;    /* [nfblk][2]: global row range of each force block (system s: ynb_sys.frb0..) */
   const int32_t *chunk;   /* [nchunk_tot][4]: system, first local atom, atom count, group 0 mid/1 low/2 high */
-  const int32_t *work;    /* [nwork][4]: system, first global segment, global chunk, segments (<= 64) */
+  const int32_t *work;    /* [nwork][4]: system, structure-factor block, global chunk, 0 */
   const int32_t *kblk_sys;/* [nk_tot/256]: system of each block of 256 k vectors */
-  const int32_t *kg;      /* [nk_tot][4]: g0, g1, g2 and 1 when the k vector starts a row */
   int32_t nwork;
   int32_t nk_tot;
-  int32_t seglen;         /* 4, 8 or 16 */
-  int32_t kchunks;        /* the force pass splits each system's rows in this many parts */
+  int32_t apl;            /* atoms per lane of the structure-factor pass: 2, 4 or 8 */
+  int32_t fslots;         /* partial-gradient slots: the largest ynb_sys.nslot */
   int32_t groups;         /* nonzero when some system has nlow > 0 or nhigh < natom */
   int32_t pad_;
   double *frac;           /* work [4][npad]: wrapped fractional coordinates and charge */
   double *step;           /* work [2][npad]: cos, sin of 2 pi f0 */
-  double *spart;          /* work: partial structure factors, 2 doubles per entry */
+  double *spart;          /* work: partial structure factors, 2 doubles per (chunk, k) */
   double *coef;           /* work [nk_tot][6]: 2 c_k S(k) of the mid, low and high groups (cos, sin) */
-  double *kpart;          /* work [nk_tot/256][8] */
-  double *gpart;          /* work [kchunks][npad][3] */
+  double *fk;             /* work [nk_tot][8]: kx, ky, kz, 2 c_k S_sin, 2 c_k S_cos (mid group), 0, 0, 0 */
+  double *kpart;          /* work [nk_tot/32][8] */
+  double *gpart;          /* work [fslots][npad][3] */
 } ynb_ewald;
 
 /* ------------------------------------------------------------------ */

@@ -139,7 +139,7 @@ def test_batch_equals_alone():
     print('case batch of 6 vs alone', flush=True)
     systems = workloads.batch_of_systems(6, (150, 900), (80, 100), seed=4)
     for precision in ('fp64', 'mixed'):
-        tun = dict(pair_lanes=32, seglen=8, ewald_chunk=64, kchunks=4)
+        tun = dict(pair_lanes=32, apl=4, skb=64, fkb=64)
         big = Engine([nbsystem(s) for s in systems], precision=precision)
         big.set_tuning(**tun)
         big.ensure_nlist()
@@ -158,7 +158,7 @@ def test_batch_equals_alone():
             assert e == eb and (g == gb).all() and (v == vb).all(), (precision, i)
             one.set_tuning(pair_lanes=16)
             if one.has_ewald:
-                one.set_tuning(seglen=4, ewald_chunk=128, kchunks=2)
+                one.set_tuning(apl=2, skb=256, fkb=32)
             one.compute(build=False)
             e, g, v = one.results(0)
             de = abs(e - eb)/abs(eb)

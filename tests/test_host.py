@@ -48,36 +48,45 @@ def test_kvectors_brute_force(angles):
 def test_ewald_local_plan_covers_every_k_and_atom():
     from yaff_amd import workloads
     from yaff_amd.batch import nbsystem
-    from yaff_amd.engine import _ewald_local_plan
+    from yaff_amd.engine import _ewald_local_plan, ewald_kvectors_cached
     s = workloads.rocksalt((3, 2, 2), angles=(88.0, 93.0, 95.0))
     nb = nbsystem(s)
     nb.ewald.nlow, nb.ewald.nhigh = 5, s.natom - 9
-    for seglen, chunk in ((4, 32), (8, 64), (16, 256)):
-        p = _ewald_local_plan(nb, seglen, chunk)
+    _, g, _ = ewald_kvectors_cached(nb)
+    for apl, skb, fkb in ((2, 16, 32), (4, 64, 128), (8, 512, 16)):
+        p = _ewald_local_plan(nb, apl, skb, fkb)
         nk = p['nk']
-        # segments tile the k vectors: consecutive g0 within one row, at most seglen long
+        assert nk == len(g)
+        # rows tile the k vectors: consecutive g0 at fixed (g1, g2), starting at krow_k
         covered = np.zeros(nk, int)
-        for (g0, g1, g2, k), ln in zip(p['seg'], p['seg_len']):
-            assert 1 <= ln <= seglen
+        for (g0, g1, g2, ln), k in zip(p['krow'], p['krow_k']):
+            assert ln >= 1
             covered[k:k + ln] += 1
-            g = p['gpad'][k:k + ln, :3]
-            assert (g[:, 0] == g0 + np.arange(ln)).all() and (g[:, 1] == g1).all() and (g[:, 2] == g2).all()
+            gg = g[k:k + ln]
+            assert (gg[:, 0] == g0 + np.arange(ln)).all() and (gg[:, 1] == g1).all() and (gg[:, 2] == g2).all()
         assert (covered == 1).all()
-        # rows: starts flagged in the k table
-        assert p['gpad'][:, 3].sum() == p['nrow'] == len(p['krow'])
+        assert (np.diff(p['krow_k']) == p['krow'][:-1, 3]).all()
+        # row blocks tile the rows in order; a block exceeds its k limit only when it is one row
+        for blk, kmax in ((p['skblk'], skb), (p['fblk'], fkb)):
+            assert blk[0, 0] == 0 and blk[-1, 1] == p['nrow'] and (blk[1:, 0] == blk[:-1, 1]).all()
+            for r0, r1 in blk:
+                assert r1 > r0
+                nkb = p['krow'][r0:r1, 3].sum()
+                assert nkb <= kmax or r1 == r0 + 1
+            # greedy: a block and the next row together exceed the limit
+            for (r0, r1), (q0, _) in zip(blk[:-1], blk[1:]):
+                assert p['krow'][r0:r1, 3].sum() + p['krow'][q0, 3] > kmax
         # chunks tile the atoms, with the nlow/nhigh groups
         cov = np.zeros(s.natom, int)
         for (_, a, ln, grp) in p['chunks']:
-            assert 1 <= ln <= chunk
+            assert 1 <= ln <= 64*apl
             cov[a:a + ln] += 1
             want = 1 if a < 5 else (2 if a >= s.natom - 9 else 0)
             assert grp == want and (a + ln <= 5 or a >= 5) and (a + ln <= s.natom - 9 or a >= s.natom - 9)
         assert (cov == 1).all()
-        # work items: every (block of 64 segments, chunk) pair once
-        pairs = set((int(w[1]), int(w[2])) for w in p['work'])
-        nseg = len(p['seg'])
-        assert pairs == set(itertools.product(range(0, nseg, 64), range(len(p['chunks']))))
-        assert all(w[3] == min(64, nseg - w[1]) for w in p['work'])
+        # work items: every (structure-factor block, chunk) pair once
+        pairs = [(int(w[1]), int(w[2])) for w in p['work']]
+        assert sorted(pairs) == sorted(itertools.product(range(len(p['skblk'])), range(len(p['chunks']))))
 
 
 def test_scalings_match_oracle_table():

@@ -38,7 +38,7 @@ class YnbSys(C.Structure):
                 ('grid', C.c_int32 * 3), ('range', C.c_int32 * 3), ('bin0', C.c_int32), ('nbin', C.c_int32),
                 ('any_stab', C.c_int32), ('box_safe', C.c_int32), ('pot', YnbPot * MAX_POT),
                 ('alpha', C.c_double), ('dielectric', C.c_double), ('k0', C.c_int32), ('nk', C.c_int32),
-                ('row0', C.c_int32), ('nrow', C.c_int32), ('seg0', C.c_int32), ('nseg', C.c_int32),
+                ('frb0', C.c_int32), ('nfrb', C.c_int32), ('nslot', C.c_int32), ('pad2_', C.c_int32),
                 ('chunk0', C.c_int32), ('nchunk', C.c_int32), ('spart0', C.c_int64), ('nchunk_low', C.c_int32),
                 ('nchunk_high', C.c_int32), ('pair_form', C.c_int32), ('pad1_', C.c_int32)]
 
@@ -59,12 +59,12 @@ class YnbPairpar(C.Structure):
 
 
 class YnbEwald(C.Structure):
-    _fields_ = [('kvec', C.c_void_p), ('krow', C.c_void_p), ('krow_k', C.c_void_p), ('seg', C.c_void_p),
-                ('seg_len', C.c_void_p), ('chunk', C.c_void_p), ('work', C.c_void_p), ('kblk_sys', C.c_void_p),
-                ('kg', C.c_void_p),
-                ('nwork', C.c_int32), ('nk_tot', C.c_int32), ('seglen', C.c_int32), ('kchunks', C.c_int32),
+    _fields_ = [('kvec', C.c_void_p), ('krow', C.c_void_p), ('krow_k', C.c_void_p), ('skblk', C.c_void_p),
+                ('fblk', C.c_void_p), ('chunk', C.c_void_p), ('work', C.c_void_p), ('kblk_sys', C.c_void_p),
+                ('nwork', C.c_int32), ('nk_tot', C.c_int32), ('apl', C.c_int32), ('fslots', C.c_int32),
                 ('groups', C.c_int32), ('pad_', C.c_int32), ('frac', C.c_void_p), ('step', C.c_void_p),
-                ('spart', C.c_void_p), ('coef', C.c_void_p), ('kpart', C.c_void_p), ('gpart', C.c_void_p)]
+                ('spart', C.c_void_p), ('coef', C.c_void_p), ('fk', C.c_void_p), ('kpart', C.c_void_p),
+                ('gpart', C.c_void_p)]
 
 
 EXPORTS = ['ynb_version', 'ynb_last_error', 'ynb_sys_plan', 'ynb_nlist_workspace_bytes',

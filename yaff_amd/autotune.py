@@ -1,8 +1,8 @@
 """Autotuner of the launch parameters of a batch.
 
-Parameters: lanes per atom of the pair kernel (16/32/64), k vectors per
-segment of the structure-factor pass (4/8/16), atoms per structure-factor
-chunk (32..256) and the number of k-row parts of the force pass.  Each
+Parameters: lanes per atom of the pair kernel (16/32/64), atoms per lane of
+the structure-factor pass (2/4/8) and the k vectors per row block of the
+structure-factor and force passes.  Each
 parameter affects a known set of kernels; coordinate descent times only
 those kernels (HIP events on the launch stream, median of a few runs) with
 the other parameters fixed.  Results are cached in a JSON file keyed by
@@ -20,9 +20,9 @@ __all__ = ['tune', 'cache_key']
 
 CANDIDATES = [
     ('pair_lanes', (16, 32, 64), ('pair', 'pair_reduce')),
-    ('seglen', (4, 8, 16), ('ewald_sk', 'ewald_kreduce')),
-    ('ewald_chunk', (32, 64, 128, 256), ('ewald_sk', 'ewald_kreduce')),
-    ('kchunks', None, ('ewald_force',)),
+    ('apl', (2, 4, 8), ('ewald_sk', 'ewald_kreduce')),
+    ('skb', None, ('ewald_sk', 'ewald_kreduce')),
+    ('fkb', None, ('ewald_force',)),
 ]
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'yaff_amd', 'autotune.json')
@@ -75,9 +75,9 @@ def tune(engine, cache=DEFAULT_CACHE, verbose=False):
     for name, values, kernels in CANDIDATES:
         if name != 'pair_lanes' and not engine.has_ewald:
             continue
-        if values is None:  # kchunks: around the current choice
-            k0 = engine.kchunks
-            values = sorted(set(max(1, int(k0*f)) for f in (0.25, 0.5, 1.0, 2.0)))
+        if values is None:  # row-block sizes: around the current choice
+            k0 = getattr(engine, name)
+            values = sorted(set(max(16, int(k0*f)) for f in (0.25, 0.5, 1.0, 2.0)))
         results = []
         for v in values:
             engine.set_tuning(**{name: v})

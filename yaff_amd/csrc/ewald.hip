@@ -4,18 +4,21 @@
 // Phases.  With f = gvecs.r the fractional coordinates, k.r = 2 pi g.f, so
 // exp(i k.r) for consecutive g0 at fixed (g1, g2) follows by multiplying with
 // exp(2 pi i f0).  The k vectors are grouped in rows (runs of consecutive g0);
-// a row or segment starts from one sincospi of the reduced phase
-// g.f - rint(g.f) (computed in FP64 in both precisions) and continues by the
-// recurrence.  Fractional coordinates are wrapped to [0, 1), which leaves
-// every phase unchanged modulo 2 pi.
+// at a row start each lane seeds exp(i k.r) of its atoms from the reduced
+// phase g.f - rint(g.f) (FP64 in both precisions) and steps it along the row.
+// Fractional coordinates are wrapped to [0, 1), which leaves every phase
+// unchanged modulo 2 pi.
 //
-// Pass 1 (k_ewald_sk): one lane per segment of <= L k vectors, atoms of a
-//   chunk read with scalar loads; partial S(k) per (chunk, k).
+// Pass 1 (k_ewald_sk): lanes hold atoms (apl per lane) of one chunk and walk
+//   a block of whole rows; per k a lane sums its atoms, and every 8 k the
+//   wavefront transposes its per-lane sums through LDS so that each k's sum
+//   over the chunk is one fixed-order tree.  Partial S(k) per (chunk, k).
 // Pass 2 (k_ewald_kreduce): one thread per k sums its chunks in a fixed
 //   order per group (atoms below nlow, between, above nhigh), forms the
 //   energy and virial terms of ewald.c:82-127 and the coefficients 2 c S(k).
-// Pass 3 (k_ewald_force): one lane per atom, the system's rows split in
-//   kchunks parts over blockIdx.y; partial gradients per part.
+// Pass 3 (k_ewald_force): lanes hold two atoms each (128 per wavefront); the
+//   k data are uniform and come through scalar loads; the 4 wavefronts of a
+//   block walk 4 k blocks of the system and are summed in LDS.
 // Pass 4 (k_ewald_gsum, k_ewald_sysreduce): fixed-order sums.
 // No floating-point atomics: results do not depend on scheduling or on the
 // other systems of a batch.
@@ -23,8 +26,6 @@
 
 namespace ynb {
 namespace {
-
-constexpr int CHUNK = 256;  // atoms per structure-factor chunk (LDS)
 
 __global__ void k_ewald_prep(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, const double *pos,
                              const double *charges, double *frac, double *step) {
@@ -52,16 +53,52 @@ __global__ void k_ewald_prep(const ynb_sys *sys, const int32_t *wave_sys, int64_
   step[npad + g] = s;
 }
 
-// Seed of a recurrence: sin and cos of pi x for x = 2 (phase in turns) in
-// [-1, 1].  Mixed mode evaluates it in FP32 on the FP64-reduced argument;
-// the recurrences and all sums stay in FP64 in both modes.
+// sin and cos of 2 pi t for |t| <= 1/2 (a phase in turns), FP64: quadrant
+// q = rint(4 t), remainder x = 2 pi (t - q/4) with |x| <= pi/4, Taylor series
+// of sin (to x^17) and cos (to x^16), whose truncation errors on that range
+// are below 1e-17; then a rotation by q quarter turns.
+__device__ inline void sincos_turns(double t, double *s, double *c) {
+  constexpr double S3 = -1.0 / 6.0, S5 = 1.0 / 120.0, S7 = -1.0 / 5040.0, S9 = 1.0 / 362880.0,
+                   S11 = -1.0 / 39916800.0, S13 = 1.0 / 6227020800.0, S15 = -1.0 / 1307674368000.0,
+                   S17 = 1.0 / 355687428096000.0;
+  constexpr double C4 = 1.0 / 24.0, C6 = -1.0 / 720.0, C8 = 1.0 / 40320.0, C10 = -1.0 / 3628800.0,
+                   C12 = 1.0 / 479001600.0, C14 = -1.0 / 87178291200.0, C16 = 1.0 / 20922789888000.0;
+  const double qd = rint(4.0 * t);
+  const double x = fma(-0.25, qd, t) * 6.283185307179586477;
+  const double x2 = x * x;
+  double ps = fma(x2, S17, S15);
+  ps = fma(x2, ps, S13);
+  ps = fma(x2, ps, S11);
+  ps = fma(x2, ps, S9);
+  ps = fma(x2, ps, S7);
+  ps = fma(x2, ps, S5);
+  ps = fma(x2, ps, S3);
+  const double sn = fma(x * x2, ps, x);
+  double pc = fma(x2, C16, C14);
+  pc = fma(x2, pc, C12);
+  pc = fma(x2, pc, C10);
+  pc = fma(x2, pc, C8);
+  pc = fma(x2, pc, C6);
+  pc = fma(x2, pc, C4);
+  pc = fma(x2, pc, -0.5);
+  const double cn = fma(x2, pc, 1.0);
+  const int q = (int)qd & 3;
+  const double a = (q & 1) ? cn : sn;
+  const double b = (q & 1) ? sn : cn;
+  *s = (q & 2) ? -a : a;
+  *c = ((q + 1) & 2) ? -b : b;
+}
+
+// Seed of a recurrence: sin and cos of 2 pi t for a reduced phase t in
+// [-1/2, 1/2] turns.  Mixed mode evaluates it in FP32 on the FP64-reduced
+// argument; the recurrences and all sums stay in FP64 in both modes.
 template <int PREC>
-__device__ inline void seed_sincospi(double x, double *s, double *c) {
+__device__ inline void seed_sincos(double t, double *s, double *c) {
   if (PREC == YNB_FP64) {
-    sincospi(x, s, c);
+    sincos_turns(t, s, c);
   } else {
     float sf, cf;
-    sincospif((float)x, &sf, &cf);
+    sincospif(2.0f * (float)t, &sf, &cf);
     *s = sf;
     *c = cf;
   }
@@ -73,58 +110,98 @@ __device__ inline double reduced_phase(int g0, int g1, int g2, double f0, double
   return t - rint(t);
 }
 
-// One lane per segment of <= L consecutive k of a row; the atoms of the work
-// item's chunk are walked in order and their data are the same for every
-// lane, so they come through scalar loads (no LDS staging).
-template <int PREC, int L>
+// Structure factors.  One wavefront per work item (row block, atom chunk):
+// lane l holds atoms l, l + 64, ... of the chunk (P per lane).  Per k the
+// lane adds its atoms' q exp(i k.r) and stores the sum in row m of an LDS
+// tile of T = 8 k; when the tile is full (or the block ends), lane (part p,
+// row m) adds entries p, p + 8, ..., p + 56 of row m and three xor-shuffle
+// levels combine the 8 parts.  Every sum has a fixed order.  Tile rows are
+// RS = 66 entries apart so that both the row writes and the strided reads
+// are free of bank conflicts.  Rows are walked with a countdown; the next
+// row record is prefetched.
+template <int PREC, int P>
 __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys, const int32_t *__restrict__ work,
-                                                 const int32_t *__restrict__ chunk, const int32_t *__restrict__ seg,
-                                                 const int32_t *__restrict__ seg_len,
+                                                 const int32_t *__restrict__ chunk,
+                                                 const int32_t *__restrict__ skblk, const int32_t *__restrict__ krow,
+                                                 const int32_t *__restrict__ krow_k,
                                                  const double *__restrict__ frac, const double *__restrict__ step,
                                                  double *__restrict__ spart, int nwork, int64_t npad) {
 #pragma clang fp contract(fast)
+  constexpr int T = 8, RS = 66;
+  __shared__ double2 l_t[T * RS];
   const int lane = threadIdx.x;
+  const int m_r = lane & (T - 1), part = lane >> 3;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const int4 wk = *(const int4 *)(work + 4 * w);
     const ynb_sys &S = sys[wk.x];
     const int4 ch = *(const int4 *)(chunk + 4 * wk.z);
-    const int64_t a0 = S.atom0 + ch.y;
-    const int na = ch.z;
-    const bool valid = lane < wk.w;
-    int4 sg = make_int4(0, 0, 0, 0);
-    int len = 0;
-    if (valid) {
-      sg = *(const int4 *)(seg + 4 * (wk.y + lane));
-      len = seg_len[wk.y + lane];
+    const int rb = skblk[2 * wk.y], re = skblk[2 * wk.y + 1];
+    double f0[P], f1[P], f2[P], q[P], cr[P], ci[P], zr[P], zi[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int la = lane + 64 * p;
+      const bool valid = la < ch.z;
+      const int64_t g = S.atom0 + ch.y + (valid ? la : 0);
+      f0[p] = frac[g];
+      f1[p] = frac[npad + g];
+      f2[p] = frac[2 * npad + g];
+      q[p] = valid ? frac[3 * npad + g] : 0.0;
+      cr[p] = step[g];
+      ci[p] = step[npad + g];
+      zr[p] = zi[p] = 0.0;
     }
-    double ar[L], ai[L];
+    double *out = spart + 2 * (S.spart0 + (int64_t)(wk.z - S.chunk0) * S.nk - S.k0);
+    int kt = krow_k[rb];  // first k of the current tile
+    int m = 0;            // k stored in the tile
+    int4 rown = *(const int4 *)(krow + 4 * rb);
+    for (int r = rb; r < re; ++r) {
+      const int4 row = rown;
+      if (r + 1 < re) rown = *(const int4 *)(krow + 4 * (r + 1));
 #pragma unroll
-    for (int m = 0; m < L; ++m) ar[m] = ai[m] = 0.0;
-#pragma unroll 2
-    for (int t = 0; t < na; ++t) {
-      const int64_t g = a0 + t;
-      const double ph = reduced_phase(sg.x, sg.y, sg.z, frac[g], frac[npad + g], frac[2 * npad + g]);
-      double sn, cs;
-      seed_sincospi<PREC>(2.0 * ph, &sn, &cs);
-      const double q = frac[3 * npad + g];
-      double zr = q * cs, zi = q * sn;
-      const double cr = step[g], ci = step[npad + g];
-#pragma unroll
-      for (int m = 0; m < L; ++m) {
-        ar[m] += zr;
-        ai[m] += zi;
-        const double nr = zr * cr - zi * ci;
-        zi = zr * ci + zi * cr;
-        zr = nr;
+      for (int p = 0; p < P; ++p) {  // row start: seed q exp(2 pi i g.f)
+        double sn, cs;
+        seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, f0[p], f1[p], f2[p]), &sn, &cs);
+        zr[p] = q[p] * cs;
+        zi[p] = q[p] * sn;
       }
-    }
-    if (valid) {
-      double *out = spart + 2 * (S.spart0 + (int64_t)(wk.z - S.chunk0) * S.nk + (sg.w - S.k0));
+      for (int j = 0; j < row.w; ++j) {
+        double sr = zr[0], si = zi[0];
 #pragma unroll
-      for (int m = 0; m < L; ++m) {
-        if (m < len) {
-          out[2 * m] = ar[m];
-          out[2 * m + 1] = ai[m];
+        for (int p = 1; p < P; ++p) {
+          sr += zr[p];
+          si += zi[p];
+        }
+        l_t[m * RS + lane] = make_double2(sr, si);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const double nr = zr[p] * cr[p] - zi[p] * ci[p];
+          zi[p] = zr[p] * ci[p] + zi[p] * cr[p];
+          zr[p] = nr;
+        }
+        ++m;
+        if (m == T || (r + 1 == re && j + 1 == row.w)) {  // flush the tile
+          __syncthreads();
+          double ar = 0.0, ai = 0.0;
+          if (m_r < m) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const double2 v = l_t[m_r * RS + part + 8 * u];
+              ar += v.x;
+              ai += v.y;
+            }
+          }
+#pragma unroll
+          for (int off = 8; off < 64; off <<= 1) {
+            ar += __shfl_xor(ar, off, 64);
+            ai += __shfl_xor(ai, off, 64);
+          }
+          if (part == 0 && m_r < m) {
+            out[2 * (kt + m_r)] = ar;
+            out[2 * (kt + m_r) + 1] = ai;
+          }
+          __syncthreads();
+          kt += m;
+          m = 0;
         }
       }
     }
@@ -205,6 +282,9 @@ __global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_e
     double *o = ew.coef + 6 * (int64_t)k;
 #pragma unroll
     for (int c = 0; c < 6; ++c) o[c] = cf[c];
+    double *r = ew.fk + 8 * (int64_t)k;
+    *(double4 *)r = make_double4(kv.x, kv.y, kv.z, cf[1]);
+    *(double4 *)(r + 4) = make_double4(cf[0], 0.0, 0.0, 0.0);
   }
   __syncthreads();
   __shared__ double red[7][32];
@@ -222,160 +302,94 @@ __global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_e
   if (t < 7) ew.kpart[8 * (int64_t)blockIdx.x + t] = red[t][0];
 }
 
-// One lane per atom; blockIdx.y selects a contiguous part of the system's
-// k rows.  The part's k data (k vector, coefficients, g and a row-start flag)
-// is staged through LDS 64 k at a time and read as broadcasts; at a row start
-// the lane seeds exp(i k.r) with one sincospi, then steps it along g0.
+// Forces (ewald.c:104-111).  Block (128-atom block, slot y) of 4 wavefronts:
+// wavefront w walks row block 4 y + w of the system with two atoms per lane
+// (atoms g and g + 64).  The k data are the same for every lane and come
+// through scalar loads: one 64-byte record per k (k vector and 2 c S(k),
+// written by k_ewald_kreduce), prefetched one k ahead, and one row record per
+// row, prefetched one row ahead.  The 4 partial gradients are added in LDS
+// in wavefront order and scaled by q / dielectric.  GROUPS: per-atom
+// coefficients of the nlow/nhigh split (ewald.c:105-107) from kvec and coef.
 template <int PREC, bool GROUPS>
-__global__ __launch_bounds__(64) void k_ewald_force(const ynb_sys *sys, const int32_t *wave_sys, ynb_ewald ew,
-                                                    int64_t npad) {
+__global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__ sys,
+                                                     const int32_t *__restrict__ wave_sys,
+                                                     const double *__restrict__ kvec, const double *__restrict__ coef,
+                                                     const double *__restrict__ fk, const int32_t *__restrict__ krow,
+                                                     const int32_t *__restrict__ krow_k,
+                                                     const int32_t *__restrict__ fblk,
+                                                     const double *__restrict__ frac, const double *__restrict__ step,
+                                                     double *__restrict__ gpart, int64_t npad) {
 #pragma clang fp contract(fast)
-  __shared__ double l_k[3][64];
-  __shared__ double l_c[6][64];
-  __shared__ int4 l_g[64];
-  const int64_t wave = blockIdx.x;
-  const int kc = blockIdx.y;
-  const int lane = threadIdx.x;
-  const int sid = uniform_int(wave_sys[wave]);
-  if (sid < 0) return;
-  const ynb_sys &S = sys[sid];
-  if (S.nk == 0) return;
-  const int64_t g = wave * 64 + lane;
-  const int li = (int)(g - S.atom0);
-  const bool active = li < S.natom;
-  const double f0 = ew.frac[g], f1 = ew.frac[npad + g], f2 = ew.frac[2 * npad + g];
-  const double q = ew.frac[3 * npad + g];
-  const double cr = ew.step[g], ci = ew.step[npad + g];
-  double wl = 0.0, wh = 0.0;
-  if (GROUPS) {
-    wh = (li < S.nhigh) ? 1.0 : 0.0;
-    wl = (li >= S.nlow) ? 1.0 : 0.0;
-  }
-  const int rb = S.row0 + (int)(((int64_t)S.nrow * kc) / ew.kchunks);
-  const int re = S.row0 + (int)(((int64_t)S.nrow * (kc + 1)) / ew.kchunks);
-  double G0 = 0.0, G1 = 0.0, G2 = 0.0;
-  if (re > rb) {
-    const int kbeg = ew.krow_k[rb];
-    const int kend = ew.krow_k[re - 1] + ew.krow[4 * (re - 1) + 3];
-    double zr = 1.0, zi = 0.0;
-    for (int k0 = kbeg; k0 < kend; k0 += 64) {
-      const int kk = k0 + lane;
-      __syncthreads();
-      if (kk < kend) {
-        const double4 kv = *(const double4 *)(ew.kvec + 4 * (int64_t)kk);
-        l_k[0][lane] = kv.x;
-        l_k[1][lane] = kv.y;
-        l_k[2][lane] = kv.z;
-        const double *cf = ew.coef + 6 * (int64_t)kk;
-        l_c[0][lane] = cf[0];
-        l_c[1][lane] = cf[1];
-        if (GROUPS) {
-          l_c[2][lane] = cf[2];
-          l_c[3][lane] = cf[3];
-          l_c[4][lane] = cf[4];
-          l_c[5][lane] = cf[5];
-        }
-        l_g[lane] = *(const int4 *)(ew.kg + 4 * (int64_t)kk);
-      }
-      __syncthreads();
-      const int nt = min(64, kend - k0);
-      for (int t = 0; t < nt; ++t) {
-        const int4 gg = l_g[t];
-        if (uniform_int(gg.w)) {  // row start (same k for the whole wave): seed exp(2 pi i g.f)
-          double s, c;
-          seed_sincospi<PREC>(2.0 * reduced_phase(gg.x, gg.y, gg.z, f0, f1, f2), &s, &c);
-          zr = c;
-          zi = s;
-        }
-        double ac = l_c[0][t], as = l_c[1][t];
-        if (GROUPS) {
-          ac += wh * l_c[4][t] + wl * l_c[2][t];
-          as += wh * l_c[5][t] + wl * l_c[3][t];
-        }
-        // ewald.c:104-111: x = 2c (S_c (-q sin) + S_s (q cos)); q applied below.
-        const double x = as * zr - ac * zi;
-        G0 = fma(l_k[0][t], x, G0);
-        G1 = fma(l_k[1][t], x, G1);
-        G2 = fma(l_k[2][t], x, G2);
-        const double nr = zr * cr - zi * ci;
-        zi = zr * ci + zi * cr;
-        zr = nr;
-      }
-    }
-  }
-  const double scale = active ? q / S.dielectric : 0.0;
-  double *o = ew.gpart + ((int64_t)kc * npad + g) * 3;
-  o[0] = G0 * scale;
-  o[1] = G1 * scale;
-  o[2] = G2 * scale;
-}
-
-// Without nlow/nhigh groups: two atoms per lane (atoms g and g + 64 of a
-// 128-atom block; systems start at multiples of 128 atoms), so each k's data,
-// read from LDS as broadcasts, serve two atoms.  k data are packed as
-// (kx, ky, kz, 2c S_s) and (2c S_c, row start) per k.
-template <int PREC>
-__global__ __launch_bounds__(64) void k_ewald_force2(const ynb_sys *sys, const int32_t *wave_sys, ynb_ewald ew,
-                                                     int64_t npad) {
-#pragma clang fp contract(fast)
-  __shared__ double4 l_ka[64];
-  __shared__ double2 l_kb[64];
-  __shared__ int4 l_g[64];
+  __shared__ double l_g[3][4][128];
+  const int wv = uniform_int(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int64_t blk = blockIdx.x;
-  const int kc = blockIdx.y;
-  const int lane = threadIdx.x;
-  const int sid = uniform_int(wave_sys[2 * blk]);
+  const int sid = wave_sys[2 * blk];
   if (sid < 0) return;
   const ynb_sys &S = sys[sid];
-  if (S.nk == 0) return;
+  if (S.nk == 0 || (int)blockIdx.y >= S.nslot) return;  // the same for the whole block
   const int64_t ga = blk * 128 + lane, gb = ga + 64;
-  const bool act_a = ga - S.atom0 < S.natom, act_b = gb - S.atom0 < S.natom;
-  const double fa0 = ew.frac[ga], fa1 = ew.frac[npad + ga], fa2 = ew.frac[2 * npad + ga];
-  const double fb0 = ew.frac[gb], fb1 = ew.frac[npad + gb], fb2 = ew.frac[2 * npad + gb];
-  const double cra = ew.step[ga], cia = ew.step[npad + ga];
-  const double crb = ew.step[gb], cib = ew.step[npad + gb];
-  const int rb = S.row0 + (int)(((int64_t)S.nrow * kc) / ew.kchunks);
-  const int re = S.row0 + (int)(((int64_t)S.nrow * (kc + 1)) / ew.kchunks);
+  const int la = (int)(ga - S.atom0), lb = la + 64;
+  const double fa0 = frac[ga], fa1 = frac[npad + ga], fa2 = frac[2 * npad + ga];
+  const double fb0 = frac[gb], fb1 = frac[npad + gb], fb2 = frac[2 * npad + gb];
+  const double cra = step[ga], cia = step[npad + ga];
+  const double crb = step[gb], cib = step[npad + gb];
+  double wha = 0.0, wla = 0.0, whb = 0.0, wlb = 0.0;
+  if (GROUPS) {
+    wha = la < S.nhigh ? 1.0 : 0.0;
+    wla = la >= S.nlow ? 1.0 : 0.0;
+    whb = lb < S.nhigh ? 1.0 : 0.0;
+    wlb = lb >= S.nlow ? 1.0 : 0.0;
+  }
   double A0 = 0.0, A1 = 0.0, A2 = 0.0, B0 = 0.0, B1 = 0.0, B2 = 0.0;
-  if (re > rb) {
-    const int kbeg = ew.krow_k[rb];
-    const int kend = ew.krow_k[re - 1] + ew.krow[4 * (re - 1) + 3];
-    double zar = 1.0, zai = 0.0, zbr = 1.0, zbi = 0.0;
-    for (int k0 = kbeg; k0 < kend; k0 += 64) {
-      const int kk = k0 + lane;
-      __syncthreads();
-      if (kk < kend) {
-        const double4 kv = *(const double4 *)(ew.kvec + 4 * (int64_t)kk);
-        const double *cf = ew.coef + 6 * (int64_t)kk;
-        const int4 gg = *(const int4 *)(ew.kg + 4 * (int64_t)kk);
-        l_ka[lane] = make_double4(kv.x, kv.y, kv.z, cf[1]);
-        l_kb[lane] = make_double2(cf[0], gg.w ? 1.0 : 0.0);
-        l_g[lane] = gg;
-      }
-      __syncthreads();
-      const int nt = min(64, kend - k0);
-      for (int t = 0; t < nt; ++t) {
-        const double4 ka = l_ka[t];
-        const double2 kb = l_kb[t];
-        if (kb.y != 0.0) {  // row start (the same k for the whole wave): seed exp(2 pi i g.f)
-          const int4 gg = l_g[t];
-          double s, c;
-          seed_sincospi<PREC>(2.0 * reduced_phase(gg.x, gg.y, gg.z, fa0, fa1, fa2), &s, &c);
-          zar = c;
-          zai = s;
-          seed_sincospi<PREC>(2.0 * reduced_phase(gg.x, gg.y, gg.z, fb0, fb1, fb2), &s, &c);
-          zbr = c;
-          zbi = s;
+  const int fb = 4 * (int)blockIdx.y + wv;
+  if (fb < S.nfrb) {
+    const int rb = fblk[2 * (S.frb0 + fb)], re = fblk[2 * (S.frb0 + fb) + 1];
+    int k = krow_k[rb];
+    double4 r0n, r1n;  // next k record
+    if (!GROUPS) {
+      r0n = *(const double4 *)(fk + 8 * (int64_t)k);
+      r1n = *(const double4 *)(fk + 8 * (int64_t)k + 4);
+    }
+    int4 rown = *(const int4 *)(krow + 4 * rb);
+    for (int r = rb; r < re; ++r) {
+      const int4 row = rown;
+      if (r + 1 < re) rown = *(const int4 *)(krow + 4 * (r + 1));
+      double zar, zai, zbr, zbi;
+      seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, fa0, fa1, fa2), &zai, &zar);
+      seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, fb0, fb1, fb2), &zbi, &zbr);
+#pragma unroll 1
+      for (int j = 0; j < row.w; ++j, ++k) {
+        double kx, ky, kz, xa, xb;
+        if (GROUPS) {
+          const double4 kv = *(const double4 *)(kvec + 4 * (int64_t)k);
+          const double *cf = coef + 6 * (int64_t)k;
+          const double c0 = cf[0], c1 = cf[1], c2 = cf[2], c3 = cf[3], c4 = cf[4], c5 = cf[5];
+          const double aca = c0 + wha * c4 + wla * c2, asa = c1 + wha * c5 + wla * c3;
+          const double acb = c0 + whb * c4 + wlb * c2, asb = c1 + whb * c5 + wlb * c3;
+          kx = kv.x;
+          ky = kv.y;
+          kz = kv.z;
+          // x = 2c (S_s cos - S_c sin); q is applied at the end
+          xa = asa * zar - aca * zai;
+          xb = asb * zbr - acb * zbi;
+        } else {
+          const double4 r0 = r0n, r1 = r1n;
+          const bool more = (r + 1 < re) || (j + 1 < row.w);
+          const int64_t kn = more ? k + 1 : k;
+          r0n = *(const double4 *)(fk + 8 * kn);
+          r1n = *(const double4 *)(fk + 8 * kn + 4);
+          kx = r0.x;
+          ky = r0.y;
+          kz = r0.z;
+          xa = r0.w * zar - r1.x * zai;
+          xb = r0.w * zbr - r1.x * zbi;
         }
-        // ewald.c:104-111: x = 2c (S_c (-q sin) + S_s (q cos)); q applied below.
-        const double xa = ka.w * zar - kb.x * zai;
-        const double xb = ka.w * zbr - kb.x * zbi;
-        A0 = fma(ka.x, xa, A0);
-        A1 = fma(ka.y, xa, A1);
-        A2 = fma(ka.z, xa, A2);
-        B0 = fma(ka.x, xb, B0);
-        B1 = fma(ka.y, xb, B1);
-        B2 = fma(ka.z, xb, B2);
+        A0 = fma(kx, xa, A0);
+        A1 = fma(ky, xa, A1);
+        A2 = fma(kz, xa, A2);
+        B0 = fma(kx, xb, B0);
+        B1 = fma(ky, xb, B1);
+        B2 = fma(kz, xb, B2);
         const double nar = zar * cra - zai * cia;
         zai = zar * cia + zai * cra;
         zar = nar;
@@ -385,21 +399,25 @@ __global__ __launch_bounds__(64) void k_ewald_force2(const ynb_sys *sys, const i
       }
     }
   }
-  const double sa = act_a ? ew.frac[3 * npad + ga] / S.dielectric : 0.0;
-  const double sb = act_b ? ew.frac[3 * npad + gb] / S.dielectric : 0.0;
-  double *oa = ew.gpart + ((int64_t)kc * npad + ga) * 3;
-  double *ob = ew.gpart + ((int64_t)kc * npad + gb) * 3;
-  oa[0] = A0 * sa;
-  oa[1] = A1 * sa;
-  oa[2] = A2 * sa;
-  ob[0] = B0 * sb;
-  ob[1] = B1 * sb;
-  ob[2] = B2 * sb;
+  l_g[0][wv][lane] = A0;
+  l_g[1][wv][lane] = A1;
+  l_g[2][wv][lane] = A2;
+  l_g[0][wv][lane + 64] = B0;
+  l_g[1][wv][lane + 64] = B1;
+  l_g[2][wv][lane + 64] = B2;
+  __syncthreads();
+  for (int t = threadIdx.x; t < 384; t += 256) {
+    const int a = t / 3, c = t - 3 * a;
+    const int64_t g = blk * 128 + a;
+    const double v = ((l_g[c][0][a] + l_g[c][1][a]) + l_g[c][2][a]) + l_g[c][3][a];
+    const double sc = (g - S.atom0 < S.natom) ? frac[3 * npad + g] / S.dielectric : 0.0;
+    gpart[((int64_t)blockIdx.y * npad + blk * 128) * 3 + t] = v * sc;
+  }
 }
 
-// Gradient: sum of the kchunks partial gradients, one thread per (atom,
-// component), four interleaved accumulators combined in a fixed order.
-__global__ void k_ewald_gsum(const ynb_sys *sys, const int32_t *wave_sys, ynb_ewald ew, int64_t npad, double *gpos,
+// Gradient: sum of the system's nslot partial gradients, one thread per
+// (atom, component), four interleaved accumulators combined in a fixed order.
+__global__ void k_ewald_gsum(const ynb_sys *sys, const int32_t *wave_sys, double *gpart_, int64_t npad, double *gpos,
                              int accumulate) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= 3 * npad) return;
@@ -408,18 +426,18 @@ __global__ void k_ewald_gsum(const ynb_sys *sys, const int32_t *wave_sys, ynb_ew
   if (sid < 0) return;
   const ynb_sys &S = sys[sid];
   if (S.nk == 0 || g - S.atom0 >= S.natom) return;
-  const int kcn = ew.kchunks;
-  const double *src = ew.gpart + t;  // gpart[(kc * npad + g) * 3 + c] = src[kc * 3 * npad]
+  const int ns = S.nslot;
+  const double *src = gpart_ + t;  // gpart[(slot * npad + g) * 3 + c] = src[slot * 3 * npad]
   const int64_t stride = 3 * npad;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int kc = 0;
-  for (; kc + 4 <= kcn; kc += 4) {
-    a0 += src[(int64_t)kc * stride];
-    a1 += src[(int64_t)(kc + 1) * stride];
-    a2 += src[(int64_t)(kc + 2) * stride];
-    a3 += src[(int64_t)(kc + 3) * stride];
+  int sl = 0;
+  for (; sl + 4 <= ns; sl += 4) {
+    a0 += src[(int64_t)sl * stride];
+    a1 += src[(int64_t)(sl + 1) * stride];
+    a2 += src[(int64_t)(sl + 2) * stride];
+    a3 += src[(int64_t)(sl + 3) * stride];
   }
-  for (; kc < kcn; ++kc) a0 += src[(int64_t)kc * stride];
+  for (; sl < ns; ++sl) a0 += src[(int64_t)sl * stride];
   const double G = (a0 + a1) + (a2 + a3);
   if (accumulate)
     gpos[t] += G;
@@ -518,9 +536,8 @@ using namespace ynb;
 
 extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const double *pos, const double *charges,
                               int32_t precision, double *gpos, int32_t accumulate, double *out, ynb_stream stream) {
-  if (!b || !ew || !pos || !charges || !out || b->npad % 128 != 0 || ew->nk_tot % 256 != 0 || ew->kchunks < 1 ||
-      (ew->seglen != 4 && ew->seglen != 8 && ew->seglen != 16) ||
-      (precision != YNB_FP64 && precision != YNB_MIXED)) {
+  if (!b || !ew || !pos || !charges || !out || b->npad % 128 != 0 || ew->nk_tot % 256 != 0 || ew->fslots < 0 ||
+      (ew->apl != 2 && ew->apl != 4 && ew->apl != 8) || (precision != YNB_FP64 && precision != YNB_MIXED)) {
     set_error("ynb_ewald_reci: invalid arguments");
     return YNB_EINVAL;
   }
@@ -530,18 +547,18 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
                      pos, charges, ew->frac, ew->step);
   YNB_LAUNCHED();
   if (ew->nwork > 0) {
-    const dim3 grid((unsigned)(ew->nwork < 65535 ? ew->nwork : 65535));
-#define YNB_SK(P, LL)                                                                                          \
-  YNB_LAUNCH("ewald_sk", (k_ewald_sk<P, LL>), grid, dim3(64), 0, st, b->sys, ew->work, ew->chunk, ew->seg,       \
-             ew->seg_len, ew->frac, ew->step, ew->spart, ew->nwork, npad)
+    const dim3 grid((unsigned)(ew->nwork < (1 << 30) ? ew->nwork : (1 << 30)));
+#define YNB_SK(PR, PP)                                                                                        \
+  YNB_LAUNCH("ewald_sk", (k_ewald_sk<PR, PP>), grid, dim3(64), 0, st, b->sys, ew->work, ew->chunk, ew->skblk,  \
+             ew->krow, ew->krow_k, ew->frac, ew->step, ew->spart, ew->nwork, npad)
     if (precision == YNB_FP64) {
-      if (ew->seglen == 4) YNB_SK(YNB_FP64, 4);
-      else if (ew->seglen == 8) YNB_SK(YNB_FP64, 8);
-      else YNB_SK(YNB_FP64, 16);
+      if (ew->apl == 2) YNB_SK(YNB_FP64, 2);
+      else if (ew->apl == 4) YNB_SK(YNB_FP64, 4);
+      else YNB_SK(YNB_FP64, 8);
     } else {
-      if (ew->seglen == 4) YNB_SK(YNB_MIXED, 4);
-      else if (ew->seglen == 8) YNB_SK(YNB_MIXED, 8);
-      else YNB_SK(YNB_MIXED, 16);
+      if (ew->apl == 2) YNB_SK(YNB_MIXED, 2);
+      else if (ew->apl == 4) YNB_SK(YNB_MIXED, 4);
+      else YNB_SK(YNB_MIXED, 8);
     }
 #undef YNB_SK
     YNB_LAUNCHED();
@@ -550,23 +567,24 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
     YNB_LAUNCH("ewald_kreduce", k_ewald_kreduce, dim3(ew->nk_tot / 32), dim3(256), 0, st, b->sys, *ew);
     YNB_LAUNCHED();
   }
-  const dim3 fgrid((unsigned)(npad / 64), (unsigned)ew->kchunks);
-  const dim3 fgrid2((unsigned)(npad / 128), (unsigned)ew->kchunks);
-  if (precision == YNB_FP64) {
-    if (ew->groups)
-      YNB_LAUNCH("ewald_force", (k_ewald_force<YNB_FP64, true>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
-    else
-      YNB_LAUNCH("ewald_force", (k_ewald_force2<YNB_FP64>), fgrid2, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
-  } else {
-    if (ew->groups)
-      YNB_LAUNCH("ewald_force", (k_ewald_force<YNB_MIXED, true>), fgrid, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
-    else
-      YNB_LAUNCH("ewald_force", (k_ewald_force2<YNB_MIXED>), fgrid2, dim3(64), 0, st, b->sys, b->wave_sys, *ew, npad);
+  if (ew->fslots > 0) {
+    const dim3 fgrid((unsigned)(npad / 128), (unsigned)ew->fslots);
+#define YNB_FORCE(PR, GR)                                                                                     \
+  YNB_LAUNCH("ewald_force", (k_ewald_force<PR, GR>), fgrid, dim3(256), 0, st, b->sys, b->wave_sys, ew->kvec,     \
+             ew->coef, ew->fk, ew->krow, ew->krow_k, ew->fblk, ew->frac, ew->step, ew->gpart, npad)
+    if (precision == YNB_FP64) {
+      if (ew->groups) YNB_FORCE(YNB_FP64, true);
+      else YNB_FORCE(YNB_FP64, false);
+    } else {
+      if (ew->groups) YNB_FORCE(YNB_MIXED, true);
+      else YNB_FORCE(YNB_MIXED, false);
+    }
+#undef YNB_FORCE
+    YNB_LAUNCHED();
   }
-  YNB_LAUNCHED();
   if (gpos) {
-    hipLaunchKernelGGL(k_ewald_gsum, dim3((unsigned)ceil_div(3 * npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys, *ew,
-                       npad, gpos, accumulate);
+    hipLaunchKernelGGL(k_ewald_gsum, dim3((unsigned)ceil_div(3 * npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys,
+                       ew->gpart, npad, gpos, accumulate);
     YNB_LAUNCHED();
   }
   hipLaunchKernelGGL(k_ewald_sysreduce, dim3((unsigned)b->nsys), dim3(256), 0, st, b->sys, b->nsys, *ew,

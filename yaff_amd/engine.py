@@ -23,8 +23,7 @@ __all__ = ['PotSpec', 'EwaldSpec', 'NBSystem', 'Engine', 'ewald_kvectors']
 
 TWO_PI = 6.2831853071795864769
 FOUR_PI = 12.5663706143591729538
-CHUNK = 256          # atoms per structure-factor chunk (k_ewald_sk LDS tile)
-ALIGN = 128          # systems start at multiples of 128 atoms (k_ewald_force2 takes 128-atom blocks)
+ALIGN = 128          # systems start at multiples of 128 atoms (k_ewald_force takes 128-atom blocks)
 KBLOCK = 256         # k vectors per reduction block
 
 
@@ -125,15 +124,33 @@ def ewald_kvectors_cached(s):
     return cache[kkey]
 
 
-def _ewald_local_plan(s, seglen, chunk_atoms):
+def _row_blocks(cnt, kmax):
+    """Consecutive rows grouped into blocks of at most kmax k vectors (a
+    longer row is a block of its own): [n][2] local row ranges."""
+    blocks = []
+    r0, acc = 0, 0
+    for r, c in enumerate(cnt):
+        if acc > 0 and acc + c > kmax:
+            blocks.append((r0, r))
+            r0, acc = r, 0
+        acc += int(c)
+    if len(cnt):
+        blocks.append((r0, len(cnt)))
+    return np.asarray(blocks, np.int32).reshape(-1, 2)
+
+
+def _ewald_local_plan(s, apl, skb, fkb):
     """The Ewald tables of one system with system-local offsets, cached on
     the NBSystem (keyed by the cell, the Ewald settings and the launch
-    parameters) so that batches can be re-formed cheaply."""
+    parameters) so that batches can be re-formed cheaply: k vectors, rows,
+    row blocks of the structure-factor pass (<= skb k each) and of the force
+    pass (<= fkb k each), atom chunks of 64 * apl atoms and the
+    structure-factor work items (every row block x every chunk)."""
     ew = s.ewald
     gmax, g, kv = ewald_kvectors_cached(s)
     cache = s._plan_cache
     kkey = next(k for k in cache if k[0] == 'kv')
-    pkey = ('plan', kkey, seglen, chunk_atoms)
+    pkey = ('plan', kkey, apl, skb, fkb)
     if pkey in cache:
         return cache[pkey]
     nk = len(g)
@@ -142,35 +159,24 @@ def _ewald_local_plan(s, seglen, chunk_atoms):
     kpad[:nk] = kv
     rows = _runs(g)
     st, cnt = (rows[:, 0], rows[:, 1]) if len(rows) else (np.zeros(0, np.int64), np.zeros(0, np.int64))
-    gpad = np.zeros((nkp, 4), np.int32)
-    gpad[:nk, :3] = g
-    gpad[st, 3] = 1
-    krow = np.stack([g[st, 1], g[st, 2], g[st, 0], cnt], axis=1).astype(np.int32) if len(st) else \
+    krow = np.stack([g[st, 0], g[st, 1], g[st, 2], cnt], axis=1).astype(np.int32) if len(st) else \
         np.zeros((0, 4), np.int32)
-    # segments of at most seglen consecutive k along each row
-    nsr = (cnt + seglen - 1)//seglen
-    srow = np.repeat(np.arange(len(st)), nsr)
-    first = np.repeat(np.cumsum(nsr) - nsr, nsr)
-    off = (np.arange(len(srow)) - first)*seglen
-    kk = st[srow] + off
-    seg = np.stack([g[kk, 0], g[st[srow], 1], g[st[srow], 2], kk], axis=1).astype(np.int32) if len(kk) else \
-        np.zeros((0, 4), np.int32)
-    seg_len = np.minimum(seglen, cnt[srow] - off).astype(np.int32)
+    skblk = _row_blocks(cnt, skb)
+    fblk = _row_blocks(cnt, fkb)
     # atom chunks: below nlow (group 1), [nlow, nhigh) (group 0), above nhigh (group 2)
     nhigh = s.natom if ew.nhigh == -1 else ew.nhigh
     nlow = ew.nlow
+    csize = 64*apl
     parts = []
     for (lo, hi, grp) in ((0, nlow, 1), (nlow, nhigh, 0), (nhigh, s.natom, 2)):
-        a = np.arange(lo, hi, chunk_atoms)
-        parts.append(np.stack([np.zeros_like(a), a, np.minimum(chunk_atoms, hi - a), np.full_like(a, grp)], axis=1))
+        a = np.arange(lo, hi, csize)
+        parts.append(np.stack([np.zeros_like(a), a, np.minimum(csize, hi - a), np.full_like(a, grp)], axis=1))
     chunks = np.concatenate(parts).astype(np.int32)
-    nseg = len(seg)
-    sb = np.arange(0, nseg, 64)
-    wsb, wc = np.meshgrid(sb, np.arange(len(chunks)), indexing='ij')
-    work = np.stack([np.zeros(wsb.size, np.int64), wsb.ravel(), wc.ravel(), np.minimum(64, nseg - wsb.ravel())],
+    wb, wc = np.meshgrid(np.arange(len(skblk)), np.arange(len(chunks)), indexing='ij')
+    work = np.stack([np.zeros(wb.size, np.int64), wb.ravel(), wc.ravel(), np.zeros(wb.size, np.int64)],
                     axis=1).astype(np.int32)
-    plan = dict(gmax=gmax, nk=nk, nkp=nkp, kpad=kpad, gpad=gpad, krow=krow, krow_k=st.astype(np.int32),
-                seg=seg, seg_len=seg_len, chunks=chunks, work=work, nrow=len(st),
+    plan = dict(gmax=gmax, nk=nk, nkp=nkp, kpad=kpad, krow=krow, krow_k=st.astype(np.int32), nrow=len(st),
+                skblk=skblk, fblk=fblk, chunks=chunks, work=work,
                 grouped=bool(nlow > 0 or nhigh < s.natom),
                 nchunk_low=int((chunks[:, 3] == 1).sum()), nchunk_high=int((chunks[:, 3] == 2).sum()))
     cache[pkey] = plan
@@ -192,11 +198,15 @@ def _round_up(x, m):
     return ((x + m - 1)//m)*m
 
 
+def _ceil_div(a, b):
+    return -(-a//b)
+
+
 class Engine(object):
     """Device state and launches for a batch of NBSystem objects."""
 
-    def __init__(self, systems, precision='fp64', seglen=None, kchunks=None, kcap=None, device='cuda',
-                 pair_lanes=32, ewald_chunk=None, upload=True):
+    def __init__(self, systems, precision='fp64', apl=4, skb=None, fkb=None, kcap=None, device='cuda',
+                 pair_lanes=32, upload=True):
         self.lib = _lib.load()
         if not torch.cuda.is_available():
             raise RuntimeError('yaff_amd.Engine needs a HIP device (torch.cuda.is_available() is False)')
@@ -205,9 +215,9 @@ class Engine(object):
         self.precision = {'fp64': FP64, 'mixed': MIXED}[precision] if isinstance(precision, str) else int(precision)
         self.nsys = len(self.systems)
         self.pair_lanes = int(pair_lanes)
-        self._ewald_chunk_req = ewald_chunk
+        self.apl, self.skb, self.fkb = int(apl), skb, fkb
         self._layout()
-        self._plan_sys(seglen, kchunks)
+        self._plan_sys()
         self._alloc(kcap)
         if upload:
             self.upload_positions()
@@ -226,7 +236,7 @@ class Engine(object):
             a0 = self.atom0[i]
             self.wave_sys[a0//64:(a0 + max(s.natom, 1) + 63)//64] = i
 
-    def _plan_sys(self, seglen, kchunks):
+    def _plan_sys(self):
         L = self.lib
         self.hsys = (YnbSys*self.nsys)()
         par = np.zeros((MAX_POT, 3, self.npad), np.float64)
@@ -236,7 +246,6 @@ class Engine(object):
         stab_off = 0
         self.has_ewald = any(s.ewald is not None for s in self.systems)
         self.groups = 0
-        self.seglen = seglen
         for i, s in enumerate(self.systems):
             h = self.hsys[i]
             cell = s.cell
@@ -287,35 +296,43 @@ class Engine(object):
         self.cross_h = np.concatenate(cross) if cross else np.zeros(1)
         self.stab_h = np.concatenate(stabs) if stabs else np.zeros(1, SCALING_DTYPE)
         if self.has_ewald:
-            self._plan_ewald(seglen, kchunks)
+            self._plan_ewald()
 
-    def _plan_ewald(self, seglen, kchunks):
-        """k rows, segments, atom chunks and the work list of k_ewald_sk: the
-        cached per-system tables joined with their batch offsets."""
+    def _plan_ewald(self, apl=None, skb=None, fkb=None):
+        """Rows, row blocks, atom chunks and work items of the Ewald passes:
+        the cached per-system tables joined with their batch offsets.  Unset
+        parameters get defaults that give both passes a few thousand
+        wavefronts."""
         ew_sys = [i for i, s in enumerate(self.systems) if s.ewald is not None and s.natom > 0]
         for i in ew_sys:
             if self.systems[i].cell.nvec != 3:
                 raise ValueError('Ewald reciprocal needs a 3D periodic cell')
-        if seglen is None:
-            nk_all = max([len(ewald_kvectors_cached(self.systems[i])[1]) for i in ew_sys] + [0])
-            seglen = 16 if nk_all > 20000 else 8
-        self.seglen = seglen
-        natom_tot = sum(self.systems[i].natom for i in ew_sys)
-        if self._ewald_chunk_req is not None:
-            self.ewald_chunk = int(self._ewald_chunk_req)
-        else:
-            # enough (segment group, chunk) work items to give every SIMD a few waves
-            nk_all = max([len(ewald_kvectors_cached(self.systems[i])[1]) for i in ew_sys] + [0])
-            groups = max(1, (nk_all//seglen + 63)//64)
-            want = 4096
-            chunk = CHUNK
-            while chunk > 32 and groups*max(1, natom_tot//chunk) < want:
-                chunk //= 2
-            self.ewald_chunk = chunk
-        assert 1 <= self.ewald_chunk <= CHUNK
-        kvec, krow, krow_k, seg, seg_len, chunk, work, kblk_sys, kg = [], [], [], [], [], [], [], [], []
-        k0 = row0 = seg0 = chunk0 = 0
+        apl = self.apl if apl is None else int(apl)
+        if apl not in (2, 4, 8):
+            raise ValueError('apl must be 2, 4 or 8')
+        self.apl = apl
+        nks = {i: len(ewald_kvectors_cached(self.systems[i])[1]) for i in ew_sys}
+        if skb is None:
+            skb = self.skb
+        if skb is None:
+            # structure-factor wavefronts: sum over systems of blocks x chunks
+            skb = 512
+            while skb > 16 and sum(max(1, nks[i]//skb)*_ceil_div(self.systems[i].natom, 64*apl)
+                                   for i in ew_sys) < 4096:
+                skb //= 2
+        if fkb is None:
+            fkb = self.fkb
+        if fkb is None:
+            # force wavefronts: sum over systems of 128-atom blocks x row blocks
+            fkb = 512
+            while fkb > 16 and sum(max(1, nks[i]//fkb)*_ceil_div(self.systems[i].natom, 128)
+                                   for i in ew_sys) < 4096:
+                fkb //= 2
+        self.skb, self.fkb = int(skb), int(fkb)
+        kvec, krow, krow_k, skblk, fblk, chunk, work, kblk_sys = [], [], [], [], [], [], [], []
+        k0 = row0 = sb0 = fb0 = chunk0 = 0
         spart0 = 0
+        fslots = 0
         self.gmax = [None]*self.nsys
         self.groups = 0
         for i, s in enumerate(self.systems):
@@ -323,36 +340,34 @@ class Engine(object):
             if s.ewald is None or s.natom == 0:
                 h.nk = 0
                 h.k0 = k0
+                h.nfrb = h.nslot = 0
                 continue
             ew = s.ewald
-            p = _ewald_local_plan(s, self.seglen, self.ewald_chunk)
+            p = _ewald_local_plan(s, self.apl, self.skb, self.fkb)
             self.gmax[i] = p['gmax']
             nkp = p['nkp']
             kvec.append(p['kpad'])
-            kg.append(p['gpad'])
             kblk_sys.append(np.full(nkp//KBLOCK, i, np.int32))
             krow.append(p['krow'])
             krow_k.append(p['krow_k'] + k0)
-            sg = p['seg'].copy()
-            sg[:, 3] += k0
-            seg.append(sg)
-            seg_len.append(p['seg_len'])
+            skblk.append(p['skblk'] + row0)
+            fblk.append(p['fblk'] + row0)
             ch = p['chunks'].copy()
             ch[:, 0] = i
             chunk.append(ch)
             wk = p['work'].copy()
             wk[:, 0] = i
-            wk[:, 1] += seg0
+            wk[:, 1] += sb0
             wk[:, 2] += chunk0
             work.append(wk)
             if p['grouped']:
                 self.groups = 1
-            nseg = len(sg)
             nch = len(ch)
             h.alpha, h.dielectric = ew.alpha, ew.dielectric
             h.k0, h.nk = k0, nkp
-            h.row0, h.nrow = row0, p['nrow']
-            h.seg0, h.nseg = seg0, nseg
+            h.frb0, h.nfrb = fb0, len(p['fblk'])
+            h.nslot = _ceil_div(h.nfrb, 4)
+            fslots = max(fslots, h.nslot)
             h.chunk0, h.nchunk = chunk0, nch
             h.spart0 = spart0
             h.nchunk_low = p['nchunk_low']
@@ -360,7 +375,8 @@ class Engine(object):
             spart0 += nch*nkp
             k0 += nkp
             row0 += p['nrow']
-            seg0 += nseg
+            sb0 += len(p['skblk'])
+            fb0 += len(p['fblk'])
             chunk0 += nch
 
         def cat(parts, empty):
@@ -370,23 +386,16 @@ class Engine(object):
             kvec=cat(kvec, np.zeros((KBLOCK, 4))),
             krow=cat(krow, np.zeros((1, 4), np.int32)),
             krow_k=cat(krow_k, np.zeros(1, np.int32)),
-            seg=cat(seg, np.zeros((1, 4), np.int32)),
-            seg_len=cat(seg_len, np.zeros(1, np.int32)),
+            skblk=cat(skblk, np.zeros((1, 2), np.int32)),
+            fblk=cat(fblk, np.zeros((1, 2), np.int32)),
             chunk=cat(chunk, np.zeros((1, 4), np.int32)),
             work=cat(work, np.zeros((1, 4), np.int32)),
             kblk_sys=cat(kblk_sys, np.zeros(1, np.int32)),
-            kg=cat(kg, np.zeros((KBLOCK, 4), np.int32)),
         )
         self.nk_tot = k0
         self.nwork = int(sum(len(w) for w in work))
         self.spart_entries = max(spart0, 1)
-        if kchunks is None:
-            # about 4096 single-wave blocks in the force pass (128 atoms each)
-            nblk = self.npad//ALIGN
-            kchunks = max(1, min(64, int(math.ceil(4096.0/max(nblk, 1)))))
-            maxrow = max([self.hsys[i].nrow for i in range(self.nsys)] + [1])
-            kchunks = min(kchunks, maxrow)
-        self.kchunks = kchunks
+        self.fslots = fslots
 
     # ------------------------------------------------------------------
     def _t(self, arr, dtype=None):
@@ -506,22 +515,24 @@ class Engine(object):
         self.ew_step_d = torch.zeros(2*self.npad, dtype=torch.float64, device=dev)
         self.ew_spart_d = torch.zeros(2*self.spart_entries, dtype=torch.float64, device=dev)
         self.ew_coef_d = torch.zeros(6*max(self.nk_tot, KBLOCK), dtype=torch.float64, device=dev)
+        self.ew_fk_d = torch.zeros(8*max(self.nk_tot, KBLOCK), dtype=torch.float64, device=dev)
         self.ew_kpart_d = torch.zeros(8*max(self.nk_tot//32, 1), dtype=torch.float64, device=dev)
-        self.ew_gpart_d = torch.zeros(self.kchunks*self.npad*3, dtype=torch.float64, device=dev)
+        self.ew_gpart_d = torch.zeros(max(self.fslots, 1)*self.npad*3, dtype=torch.float64, device=dev)
         t = self.ew_tabs
         self.ewald = YnbEwald(t['kvec'].data_ptr(), t['krow'].data_ptr(), t['krow_k'].data_ptr(),
-                              t['seg'].data_ptr(), t['seg_len'].data_ptr(), t['chunk'].data_ptr(),
-                              t['work'].data_ptr(), t['kblk_sys'].data_ptr(), t['kg'].data_ptr(), self.nwork,
-                              self.nk_tot, self.seglen, self.kchunks, self.groups, 0, self.ew_frac_d.data_ptr(),
+                              t['skblk'].data_ptr(), t['fblk'].data_ptr(), t['chunk'].data_ptr(),
+                              t['work'].data_ptr(), t['kblk_sys'].data_ptr(), self.nwork,
+                              self.nk_tot, self.apl, self.fslots, self.groups, 0, self.ew_frac_d.data_ptr(),
                               self.ew_step_d.data_ptr(), self.ew_spart_d.data_ptr(), self.ew_coef_d.data_ptr(),
-                              self.ew_kpart_d.data_ptr(), self.ew_gpart_d.data_ptr())
+                              self.ew_fk_d.data_ptr(), self.ew_kpart_d.data_ptr(), self.ew_gpart_d.data_ptr())
 
     def tuning(self):
         """The launch parameters an autotuner may change."""
-        return {'pair_lanes': self.pair_lanes, 'seglen': self.seglen if self.has_ewald else None,
-                'ewald_chunk': getattr(self, 'ewald_chunk', None), 'kchunks': getattr(self, 'kchunks', None)}
+        e = self.has_ewald
+        return {'pair_lanes': self.pair_lanes, 'apl': self.apl if e else None, 'skb': self.skb if e else None,
+                'fkb': self.fkb if e else None}
 
-    def set_tuning(self, pair_lanes=None, seglen=None, ewald_chunk=None, kchunks=None):
+    def set_tuning(self, pair_lanes=None, apl=None, skb=None, fkb=None):
         """Changes launch parameters in place (results are unchanged up to
         summation order).  Ewald tables are re-planned when their parameters
         change."""
@@ -530,13 +541,8 @@ class Engine(object):
                 raise ValueError('pair_lanes must be 16, 32 or 64')
             self.pair_lanes = int(pair_lanes)
             self.pairpar.lanes = self.pair_lanes
-        if self.has_ewald and (seglen is not None or ewald_chunk is not None or kchunks is not None):
-            if ewald_chunk is not None:
-                self._ewald_chunk_req = int(ewald_chunk)
-            else:
-                self._ewald_chunk_req = self.ewald_chunk
-            self._plan_ewald(seglen if seglen is not None else self.seglen,
-                             kchunks if kchunks is not None else self.kchunks)
+        if self.has_ewald and (apl is not None or skb is not None or fkb is not None):
+            self._plan_ewald(apl, skb, fkb)
             self.sys_d = self._t(np.frombuffer(bytes(self.hsys), np.uint8))
             self.batch.sys = self.sys_d.data_ptr()
             self._alloc_ewald()

@@ -32,13 +32,13 @@ import numpy as np
 import torch
 
 from . import _lib
-from .engine import Engine, _round_up, ewald_kvectors_cached, CHUNK, ALIGN
+from .engine import Engine, _round_up, _ceil_div, ewald_kvectors_cached, ALIGN
 from .batch import nbsystem, timestep, maxwell_boltzmann
 
 __all__ = ['Request', 'Server', 'estimate_bytes', 'poisson_requests', 'latency_summary', 'SERVER_TUNING']
 
 # Fixed launch parameters of the server: results do not depend on the batch.
-SERVER_TUNING = dict(pair_lanes=32, seglen=8, ewald_chunk=64, kchunks=8)
+SERVER_TUNING = dict(pair_lanes=32, apl=4, skb=64, fkb=64)
 
 
 class Request(object):
@@ -100,17 +100,20 @@ def estimate_bytes(nbs, tuning=SERVER_TUNING):
     npad = sum(_round_up(max(s.natom, 1), ALIGN) for s in nbs)
     kcap = max(_kcap_estimate(s) for s in nbs)
     total = npad*(8*kcap + 8*3*3 + 8*3*_lib.MAX_POT + 8*3 + 4 + 2*_lib.MAX_POT*8 + 64)
-    chunk = tuning.get('ewald_chunk') or CHUNK
-    kchunks = tuning.get('kchunks') or 8
+    chunk = 64*(tuning.get('apl') or 4)
+    fkb = tuning.get('fkb') or 64
+    nslot = 0
     for s in nbs:
         if s.ewald is None or s.natom == 0:
             continue
         nk = len(ewald_kvectors_cached(s)[1])
         nkp = _round_up(max(nk, 1), 256)
-        nchunk = (s.natom + chunk - 1)//chunk
-        total += nkp*(32 + 16 + 48) + 16*nchunk*nkp
-    if any(s.ewald is not None for s in nbs):
-        total += npad*(kchunks*24 + 48)
+        nchunk = _ceil_div(s.natom, chunk)
+        total += nkp*(32 + 16 + 48 + 64) + 16*nchunk*nkp
+        # greedy row blocks: two consecutive blocks hold more than fkb k vectors
+        nslot = max(nslot, _ceil_div(2*_ceil_div(nk, fkb) + 1, 4))
+    if nslot:
+        total += npad*(nslot*24 + 48)
     return int(total)
 
 
@@ -184,8 +187,8 @@ class Server(object):
         old = self.engine
         old_members = self.engine_members
         eng = Engine([r.nb for r in self.active], precision=self.precision, upload=False,
-                     pair_lanes=self.tuning['pair_lanes'], seglen=self.tuning['seglen'],
-                     ewald_chunk=self.tuning['ewald_chunk'], kchunks=self.tuning['kchunks'])
+                     pair_lanes=self.tuning['pair_lanes'], apl=self.tuning['apl'], skb=self.tuning['skb'],
+                     fkb=self.tuning['fkb'])
         host_pos = np.zeros((eng.npad, 3))
         host_vel = np.zeros((eng.npad, 3))
         src, dst = [], []
