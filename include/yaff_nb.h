@@ -9,9 +9,9 @@
  *
  * Work is batched: a launch covers every system of a ynb_batch.  Atoms of
  * system s live at [sys.atom0, sys.atom0 + sys.natom) of the padded atom
- * arrays; atom0 is a multiple of 128 so that neither a wavefront nor the
- * 128-atom blocks of the Ewald force pass hold atoms of two systems.  A
- * single system is a batch of one.
+ * arrays; atom0 is a multiple of 256 so that neither a wavefront nor the
+ * atom blocks of the Ewald force pass (up to 256 atoms) hold atoms of two
+ * systems.  A single system is a batch of one.
  *
  * Reference interfaces replaced (molmod/yaff):
  *   ynb_nlist_build + ynb_nlist_rows   <- nlist_build_low   yaff/pes/nlist.h:37-39
@@ -82,7 +82,7 @@ typedef struct ynb_sys {
   double volume;
   int32_t nvec;       /* 0..3 periodic directions */
   int32_t natom;
-  int32_t atom0;      /* offset into the padded atom arrays, multiple of 128 */
+  int32_t atom0;      /* offset into the padded atom arrays, multiple of 256 */
   int32_t npot;
   int32_t rmax[3];    /* image range of nlist.py:132 */
   int32_t nlow, nhigh;
@@ -121,7 +121,7 @@ typedef struct ynb_batch {
   const int32_t *bin_sys;    /* device [nbin]: system of each bin */
   int32_t nsys;
   int32_t nbin;              /* total bins (0 when every system uses the all-pairs path) */
-  int64_t npad;              /* padded atom count, multiple of 128 */
+  int64_t npad;              /* padded atom count, multiple of 256 */
   int32_t max_bin_atoms;     /* capacity of a bin, 1..1024 (ynb_nlist_build flags overflow) */
   int32_t flags;             /* YNB_BATCH_*: which build paths the batch needs */
 } ynb_batch;
@@ -167,9 +167,10 @@ typedef struct ynb_pairpar {
  * The k vectors of a system form rows (runs of consecutive g0 at fixed g1,
  * g2, in the order g2, g1 major, g0 ascending).  Both passes walk blocks of
  * whole rows: a lane holds atoms, seeds exp(i k.r) at each row start and
- * steps it along g0.  The structure-factor pass covers every (k block, atom
- * chunk) work item; the force pass gives each wavefront one k block of a
- * system and 128 atoms, and a block of 4 wavefronts sums its 4 k blocks. */
+ * steps it along g0.  The structure-factor pass covers every (row block, atom
+ * chunk) work item; the force pass gives each wavefront one row block of a
+ * system and 64 * fapl atoms, and a block of 4 wavefronts sums its 4 row
+ * blocks. */
 typedef struct ynb_ewald {
   const double *kvec;     /* [nk_tot][4]: kx, ky, kz and c_k = 4 pi/V exp(-k^2/4 alpha^2)/k^2 (0: padding) */
   const int32_t *krow;    /* [nrow_tot][4]: g0 of the row's first k, g1, g2, number of k in the row */
@@ -179,17 +180,20 @@ typedef struct ynb_ewald {
   const int32_t *chunk;   /* [nchunk_tot][4]: system, first local atom, atom count, group 0 mid/1 low/2 high */
   const int32_t *work;    /* [nwork][4]: system, structure-factor block, global chunk, 0 */
   const int32_t *kblk_sys;/* [nk_tot/256]: system of each block of 256 k vectors */
+  const int32_t *kpos;    /* [nk_tot]: force record of each k (-1: padding k) */
+  const int32_t *krow_f;  /* [nrow_tot]: first force record of each row */
   int32_t nwork;
   int32_t nk_tot;
   int32_t apl;            /* atoms per lane of the structure-factor pass: 2, 4 or 8 */
   int32_t fslots;         /* partial-gradient slots: the largest ynb_sys.nslot */
   int32_t groups;         /* nonzero when some system has nlow > 0 or nhigh < natom */
-  int32_t pad_;
+  int32_t fapl;           /* atoms per lane of the force pass: 2 or 4 */
   double *frac;           /* work [4][npad]: wrapped fractional coordinates and charge */
   double *step;           /* work [2][npad]: cos, sin of 2 pi f0 */
   double *spart;          /* work: partial structure factors, 2 doubles per (chunk, k) */
   double *coef;           /* work [nk_tot][6]: 2 c_k S(k) of the mid, low and high groups (cos, sin) */
-  double *fk;             /* work [nk_tot][8]: kx, ky, kz, 2 c_k S_sin, 2 c_k S_cos (mid group), 0, 0, 0 */
+  double *frec;           /* work [nfrec][2]: 2 c_k S_sin, 2 c_k S_cos of the mid group per k; each row
+                             padded to a multiple of 4 records, padding and 4 spare records at the end zero */
   double *kpart;          /* work [nk_tot/32][8] */
   double *gpart;          /* work [fslots][npad][3] */
 } ynb_ewald;

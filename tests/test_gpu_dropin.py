@@ -158,7 +158,7 @@ def test_batch_equals_alone():
             assert e == eb and (g == gb).all() and (v == vb).all(), (precision, i)
             one.set_tuning(pair_lanes=16)
             if one.has_ewald:
-                one.set_tuning(apl=2, skb=256, fkb=32)
+                one.set_tuning(apl=2, skb=256, fkb=32, fapl=2)
             one.compute(build=False)
             e, g, v = one.results(0)
             de = abs(e - eb)/abs(eb)

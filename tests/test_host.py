@@ -84,6 +84,13 @@ def test_ewald_local_plan_covers_every_k_and_atom():
             want = 1 if a < 5 else (2 if a >= s.natom - 9 else 0)
             assert grp == want and (a + ln <= 5 or a >= 5) and (a + ln <= s.natom - 9 or a >= s.natom - 9)
         assert (cov == 1).all()
+        # force records: rows start at multiples of 4 records, each k at its row's start plus its place
+        assert (p['krow_f'] % 4 == 0).all() and p['krow_f'][0] == 0
+        assert (np.diff(p['krow_f']) == (p['krow'][:-1, 3] + 3)//4*4).all()
+        assert p['nrec'] == p['krow_f'][-1] + (p['krow'][-1, 3] + 3)//4*4
+        for (g0, g1, g2, ln), k, f in zip(p['krow'], p['krow_k'], p['krow_f']):
+            assert (p['kpos'][k:k + ln] == f + np.arange(ln)).all()
+        assert (p['kpos'][nk:] == -1).all() and len(set(p['kpos'][:nk])) == nk
         # work items: every (structure-factor block, chunk) pair once
         pairs = [(int(w[1]), int(w[2])) for w in p['work']]
         assert sorted(pairs) == sorted(itertools.product(range(len(p['skblk'])), range(len(p['chunks']))))

@@ -61,9 +61,10 @@ class YnbPairpar(C.Structure):
 class YnbEwald(C.Structure):
     _fields_ = [('kvec', C.c_void_p), ('krow', C.c_void_p), ('krow_k', C.c_void_p), ('skblk', C.c_void_p),
                 ('fblk', C.c_void_p), ('chunk', C.c_void_p), ('work', C.c_void_p), ('kblk_sys', C.c_void_p),
-                ('nwork', C.c_int32), ('nk_tot', C.c_int32), ('apl', C.c_int32), ('fslots', C.c_int32),
-                ('groups', C.c_int32), ('pad_', C.c_int32), ('frac', C.c_void_p), ('step', C.c_void_p),
-                ('spart', C.c_void_p), ('coef', C.c_void_p), ('fk', C.c_void_p), ('kpart', C.c_void_p),
+                ('kpos', C.c_void_p), ('krow_f', C.c_void_p), ('nwork', C.c_int32), ('nk_tot', C.c_int32),
+                ('apl', C.c_int32), ('fslots', C.c_int32),
+                ('groups', C.c_int32), ('fapl', C.c_int32), ('frac', C.c_void_p), ('step', C.c_void_p),
+                ('spart', C.c_void_p), ('coef', C.c_void_p), ('frec', C.c_void_p), ('kpart', C.c_void_p),
                 ('gpart', C.c_void_p)]
 
 

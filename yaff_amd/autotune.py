@@ -22,6 +22,7 @@ CANDIDATES = [
     ('pair_lanes', (16, 32, 64), ('pair', 'pair_reduce')),
     ('apl', (2, 4, 8), ('ewald_sk', 'ewald_kreduce')),
     ('skb', None, ('ewald_sk', 'ewald_kreduce')),
+    ('fapl', (2, 4), ('ewald_force',)),
     ('fkb', None, ('ewald_force',)),
 ]
 

@@ -153,4 +153,55 @@ __device__ inline void wave_status_max(int32_t *addr, int v) {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// a * b + c as the three-operand v_fma_f64 with the addend c a scalar
+// register (c: a constant coefficient).  For an addend that stays live (a
+// polynomial coefficient held in a vector register across a loop) the
+// compiler otherwise picks the two-operand v_fmac_f64, whose destination is
+// the addend, and copies the coefficient into it first: two instructions per
+// Horner step instead of one, and a vector register pair per coefficient.
+__device__ inline double fma3(double a, double b, double c) {
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+  return r;
+}
+
+// 1/x in FP64: the hardware reciprocal estimate refined by two Newton steps
+// (relative error about 1 ulp; no special handling of 0, inf or denormals --
+// callers pass finite normal arguments).
+__device__ inline double rcp_fp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// exp(a) in FP64 by Cody-Waite reduction: k = rint(a / ln 2),
+// r = a - k ln2_hi - k ln2_lo with ln2_hi holding the leading 32 bits of ln 2
+// (k ln2_hi is exact for |k| <= 1100), |r| <= ln 2 / 2; e^r from its Taylor
+// series to r^13 (truncation < 5e-18 relative), evaluated as
+// E(r^2) + r O(r^2) with its even and odd parts by Horner's rule in r^2 (two
+// independent chains of 6 steps); then 2^k.  Relative error about 1 ulp;
+// 0 below a = -760, inf above 709.8, NaN kept.
+__device__ inline double exp_fp64(double a) {
+  const double k = rint(a * 1.4426950408889634);
+  const double kc = fmax(fmin(k, 1100.0), -1100.0);
+  double r = fma(-kc, 0.6931471803691238, a);
+  r = fma(-kc, 1.9082149292705877e-10, r);
+  const double z = r * r;
+  double e = fma3(z, 2.08767569878681e-09, 2.755731922398589e-07);  // 1/12!, 1/10!
+  double o = fma3(z, 1.6059043836821613e-10, 2.505210838544172e-08);  // 1/13!, 1/11!
+  e = fma3(e, z, 2.48015873015873e-05);                               // 1/8!
+  o = fma3(o, z, 2.7557319223985893e-06);                             // 1/9!
+  e = fma3(e, z, 0.001388888888888889);                               // 1/6!
+  o = fma3(o, z, 0.0001984126984126984);                              // 1/7!
+  e = fma3(e, z, 0.041666666666666664);                               // 1/4!
+  o = fma3(o, z, 0.008333333333333333);                               // 1/5!
+  e = fma3(e, z, 0.5);                                                // 1/2!
+  o = fma3(o, z, 0.16666666666666666);                                // 1/3!
+  e = fma3(e, z, 1.0);                                                // 1/0!
+  o = fma3(o, z, 1.0);                                                // 1/1!
+  return ldexp(fma(r, o, e), (int)kc);
+}
+
 }  // namespace ynb

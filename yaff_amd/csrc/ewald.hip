@@ -16,9 +16,10 @@
 // Pass 2 (k_ewald_kreduce): one thread per k sums its chunks in a fixed
 //   order per group (atoms below nlow, between, above nhigh), forms the
 //   energy and virial terms of ewald.c:82-127 and the coefficients 2 c S(k).
-// Pass 3 (k_ewald_force): lanes hold two atoms each (128 per wavefront); the
-//   k data are uniform and come through scalar loads; the 4 wavefronts of a
-//   block walk 4 k blocks of the system and are summed in LDS.
+// Pass 3 (k_ewald_force): lanes hold 2 or 4 atoms each; the per-k
+//   coefficients are uniform and come through scalar loads; the k vector
+//   weighting is carried by two running sums per row; the 4 wavefronts of a
+//   block walk 4 row blocks of the system and are summed in LDS.
 // Pass 4 (k_ewald_gsum, k_ewald_sysreduce): fixed-order sums.
 // No floating-point atomics: results do not depend on scheduling or on the
 // other systems of a batch.
@@ -66,22 +67,22 @@ __device__ inline void sincos_turns(double t, double *s, double *c) {
   const double qd = rint(4.0 * t);
   const double x = fma(-0.25, qd, t) * 6.283185307179586477;
   const double x2 = x * x;
-  double ps = fma(x2, S17, S15);
-  ps = fma(x2, ps, S13);
-  ps = fma(x2, ps, S11);
-  ps = fma(x2, ps, S9);
-  ps = fma(x2, ps, S7);
-  ps = fma(x2, ps, S5);
-  ps = fma(x2, ps, S3);
+  double ps = fma3(x2, S17, S15);
+  ps = fma3(x2, ps, S13);
+  ps = fma3(x2, ps, S11);
+  ps = fma3(x2, ps, S9);
+  ps = fma3(x2, ps, S7);
+  ps = fma3(x2, ps, S5);
+  ps = fma3(x2, ps, S3);
   const double sn = fma(x * x2, ps, x);
-  double pc = fma(x2, C16, C14);
-  pc = fma(x2, pc, C12);
-  pc = fma(x2, pc, C10);
-  pc = fma(x2, pc, C8);
-  pc = fma(x2, pc, C6);
-  pc = fma(x2, pc, C4);
-  pc = fma(x2, pc, -0.5);
-  const double cn = fma(x2, pc, 1.0);
+  double pc = fma3(x2, C16, C14);
+  pc = fma3(x2, pc, C12);
+  pc = fma3(x2, pc, C10);
+  pc = fma3(x2, pc, C8);
+  pc = fma3(x2, pc, C6);
+  pc = fma3(x2, pc, C4);
+  pc = fma3(x2, pc, -0.5);
+  const double cn = fma3(x2, pc, 1.0);
   const int q = (int)qd & 3;
   const double a = (q & 1) ? cn : sn;
   const double b = (q & 1) ? sn : cn;
@@ -282,9 +283,8 @@ __global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_e
     double *o = ew.coef + 6 * (int64_t)k;
 #pragma unroll
     for (int c = 0; c < 6; ++c) o[c] = cf[c];
-    double *r = ew.fk + 8 * (int64_t)k;
-    *(double4 *)r = make_double4(kv.x, kv.y, kv.z, cf[1]);
-    *(double4 *)(r + 4) = make_double4(cf[0], 0.0, 0.0, 0.0);
+    const int kp = ew.kpos[k];
+    if (kp >= 0) *(double2 *)(ew.frec + 2 * (int64_t)kp) = make_double2(cf[1], cf[0]);
   }
   __syncthreads();
   __shared__ double red[7][32];
@@ -302,116 +302,200 @@ __global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_e
   if (t < 7) ew.kpart[8 * (int64_t)blockIdx.x + t] = red[t][0];
 }
 
-// Forces (ewald.c:104-111).  Block (128-atom block, slot y) of 4 wavefronts:
-// wavefront w walks row block 4 y + w of the system with two atoms per lane
-// (atoms g and g + 64).  The k data are the same for every lane and come
-// through scalar loads: one 64-byte record per k (k vector and 2 c S(k),
-// written by k_ewald_kreduce), prefetched one k ahead, and one row record per
-// row, prefetched one row ahead.  The 4 partial gradients are added in LDS
-// in wavefront order and scaled by q / dielectric.  GROUPS: per-atom
-// coefficients of the nlow/nhigh split (ewald.c:105-107) from kvec and coef.
-template <int PREC, bool GROUPS>
+// Forces (ewald.c:104-111), no nlow/nhigh groups.  Block (atom block of 64 P
+// atoms, slot y) of 4 wavefronts: wavefront w walks row block 4 y + w of the
+// system with P atoms per lane (atoms g + 64 p).
+//
+// Along a row k_j = k_0 + j b (b = 2 pi times the first reciprocal vector),
+// so with x_j = 2c (S_s cos - S_c sin) at k_j the row adds
+//   sum_j k_j x_j = k_0 T + b (L T - U),  T = sum_j x_j,  U = sum_j T_j,
+// T_j the running sum and L the number of steps; the inner loop carries T
+// and U (two adds) instead of three k-weighted sums.  The only per-k data
+// are the two coefficients: 16-byte records (k_ewald_kreduce), rows padded
+// to a multiple of 4 records with zeros, read 4 at a time with one scalar
+// load that is issued a group ahead.  A zero record adds x = 0 and leaves
+// the row sum unchanged when L counts the padded steps.  The 4 partial
+// gradients of a block are added in LDS in wavefront order and scaled by
+// q / dielectric.
+template <int PREC, int P>
 __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__ sys,
                                                      const int32_t *__restrict__ wave_sys,
-                                                     const double *__restrict__ kvec, const double *__restrict__ coef,
-                                                     const double *__restrict__ fk, const int32_t *__restrict__ krow,
-                                                     const int32_t *__restrict__ krow_k,
+                                                     const double *__restrict__ frec, const int32_t *__restrict__ krow,
+                                                     const int32_t *__restrict__ krow_f,
                                                      const int32_t *__restrict__ fblk,
                                                      const double *__restrict__ frac, const double *__restrict__ step,
                                                      double *__restrict__ gpart, int64_t npad) {
 #pragma clang fp contract(fast)
-  __shared__ double l_g[3][4][128];
+  constexpr int NA = 64 * P;
+  constexpr double TWO_PI = 6.283185307179586477;
+  __shared__ double l_g[3][4][NA];
   const int wv = uniform_int(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int64_t blk = blockIdx.x;
-  const int sid = wave_sys[2 * blk];
+  const int sid = wave_sys[P * blk];
   if (sid < 0) return;
   const ynb_sys &S = sys[sid];
   if (S.nk == 0 || (int)blockIdx.y >= S.nslot) return;  // the same for the whole block
-  const int64_t ga = blk * 128 + lane, gb = ga + 64;
-  const int la = (int)(ga - S.atom0), lb = la + 64;
-  const double fa0 = frac[ga], fa1 = frac[npad + ga], fa2 = frac[2 * npad + ga];
-  const double fb0 = frac[gb], fb1 = frac[npad + gb], fb2 = frac[2 * npad + gb];
-  const double cra = step[ga], cia = step[npad + ga];
-  const double crb = step[gb], cib = step[npad + gb];
-  double wha = 0.0, wla = 0.0, whb = 0.0, wlb = 0.0;
-  if (GROUPS) {
-    wha = la < S.nhigh ? 1.0 : 0.0;
-    wla = la >= S.nlow ? 1.0 : 0.0;
-    whb = lb < S.nhigh ? 1.0 : 0.0;
-    wlb = lb >= S.nlow ? 1.0 : 0.0;
+  double f0[P], f1[P], f2[P], cr[P], ci[P], zr[P], zi[P], T[P], U[P], G0[P], G1[P], G2[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int64_t g = blk * NA + lane + 64 * p;
+    f0[p] = frac[g];
+    f1[p] = frac[npad + g];
+    f2[p] = frac[2 * npad + g];
+    cr[p] = step[g];
+    ci[p] = step[npad + g];
+    G0[p] = G1[p] = G2[p] = 0.0;
   }
-  double A0 = 0.0, A1 = 0.0, A2 = 0.0, B0 = 0.0, B1 = 0.0, B2 = 0.0;
   const int fb = 4 * (int)blockIdx.y + wv;
   if (fb < S.nfrb) {
+    const double b0 = TWO_PI * S.gvecs[0], b1 = TWO_PI * S.gvecs[1], b2 = TWO_PI * S.gvecs[2];
     const int rb = fblk[2 * (S.frb0 + fb)], re = fblk[2 * (S.frb0 + fb) + 1];
-    int k = krow_k[rb];
-    double4 r0n, r1n;  // next k record
-    if (!GROUPS) {
-      r0n = *(const double4 *)(fk + 8 * (int64_t)k);
-      r1n = *(const double4 *)(fk + 8 * (int64_t)k + 4);
-    }
+    const double *rp = frec + 2 * (int64_t)krow_f[rb];
+    double4 na = *(const double4 *)rp, nb = *(const double4 *)(rp + 4);  // next 4 records
     int4 rown = *(const int4 *)(krow + 4 * rb);
     for (int r = rb; r < re; ++r) {
       const int4 row = rown;
       if (r + 1 < re) rown = *(const int4 *)(krow + 4 * (r + 1));
-      double zar, zai, zbr, zbi;
-      seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, fa0, fa1, fa2), &zai, &zar);
-      seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, fb0, fb1, fb2), &zbi, &zbr);
-#pragma unroll 1
-      for (int j = 0; j < row.w; ++j, ++k) {
-        double kx, ky, kz, xa, xb;
-        if (GROUPS) {
-          const double4 kv = *(const double4 *)(kvec + 4 * (int64_t)k);
-          const double *cf = coef + 6 * (int64_t)k;
-          const double c0 = cf[0], c1 = cf[1], c2 = cf[2], c3 = cf[3], c4 = cf[4], c5 = cf[5];
-          const double aca = c0 + wha * c4 + wla * c2, asa = c1 + wha * c5 + wla * c3;
-          const double acb = c0 + whb * c4 + wlb * c2, asb = c1 + whb * c5 + wlb * c3;
-          kx = kv.x;
-          ky = kv.y;
-          kz = kv.z;
-          // x = 2c (S_s cos - S_c sin); q is applied at the end
-          xa = asa * zar - aca * zai;
-          xb = asb * zbr - acb * zbi;
-        } else {
-          const double4 r0 = r0n, r1 = r1n;
-          const bool more = (r + 1 < re) || (j + 1 < row.w);
-          const int64_t kn = more ? k + 1 : k;
-          r0n = *(const double4 *)(fk + 8 * kn);
-          r1n = *(const double4 *)(fk + 8 * kn + 4);
-          kx = r0.x;
-          ky = r0.y;
-          kz = r0.z;
-          xa = r0.w * zar - r1.x * zai;
-          xb = r0.w * zbr - r1.x * zbi;
+#pragma unroll
+      for (int p = 0; p < P; ++p) {  // row start: seed exp(2 pi i g.f)
+        seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, f0[p], f1[p], f2[p]), &zi[p], &zr[p]);
+        T[p] = U[p] = 0.0;
+      }
+      const int ng = (row.w + 3) >> 2;
+      // Scalar loads complete out of order, so a wait for one of them waits
+      // for all.  Waiting here (the requests are long done after the seeds)
+      // leaves nothing in flight at the loop head, so the only wait in the
+      // loop falls after its arithmetic.
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+      for (int q = 0; q < ng; ++q) {
+        const double4 ca = na, cb = nb;
+        rp += 8;
+        na = *(const double4 *)rp;
+        nb = *(const double4 *)(rp + 4);
+        const double cs[4] = {ca.x, ca.z, cb.x, cb.z}, cc[4] = {ca.y, ca.w, cb.y, cb.w};
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            const double x = cs[m] * zr[p] - cc[m] * zi[p];
+            T[p] += x;
+            U[p] += T[p];
+            const double nr = zr[p] * cr[p] - zi[p] * ci[p];
+            zi[p] = zr[p] * ci[p] + zi[p] * cr[p];
+            zr[p] = nr;
+          }
         }
-        A0 = fma(kx, xa, A0);
-        A1 = fma(ky, xa, A1);
-        A2 = fma(kz, xa, A2);
-        B0 = fma(kx, xb, B0);
-        B1 = fma(ky, xb, B1);
-        B2 = fma(kz, xb, B2);
-        const double nar = zar * cra - zai * cia;
-        zai = zar * cia + zai * cra;
-        zar = nar;
-        const double nbr = zbr * crb - zbi * cib;
-        zbi = zbr * cib + zbi * crb;
-        zbr = nbr;
+      }
+      // k_0 of the row and the row's contribution
+      const double g0 = (double)row.x, g1 = (double)row.y, g2 = (double)row.z;
+      const double k0x = TWO_PI * (g0 * S.gvecs[0] + g1 * S.gvecs[3] + g2 * S.gvecs[6]);
+      const double k0y = TWO_PI * (g0 * S.gvecs[1] + g1 * S.gvecs[4] + g2 * S.gvecs[7]);
+      const double k0z = TWO_PI * (g0 * S.gvecs[2] + g1 * S.gvecs[5] + g2 * S.gvecs[8]);
+      const double Lp = (double)(4 * ng);
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const double sj = Lp * T[p] - U[p];
+        G0[p] += k0x * T[p] + b0 * sj;
+        G1[p] += k0y * T[p] + b1 * sj;
+        G2[p] += k0z * T[p] + b2 * sj;
       }
     }
   }
-  l_g[0][wv][lane] = A0;
-  l_g[1][wv][lane] = A1;
-  l_g[2][wv][lane] = A2;
-  l_g[0][wv][lane + 64] = B0;
-  l_g[1][wv][lane + 64] = B1;
-  l_g[2][wv][lane + 64] = B2;
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    l_g[0][wv][lane + 64 * p] = G0[p];
+    l_g[1][wv][lane + 64 * p] = G1[p];
+    l_g[2][wv][lane + 64 * p] = G2[p];
+  }
   __syncthreads();
-  for (int t = threadIdx.x; t < 384; t += 256) {
+  for (int t = threadIdx.x; t < 3 * NA; t += 256) {
     const int a = t / 3, c = t - 3 * a;
-    const int64_t g = blk * 128 + a;
+    const int64_t g = blk * NA + a;
     const double v = ((l_g[c][0][a] + l_g[c][1][a]) + l_g[c][2][a]) + l_g[c][3][a];
     const double sc = (g - S.atom0 < S.natom) ? frac[3 * npad + g] / S.dielectric : 0.0;
-    gpart[((int64_t)blockIdx.y * npad + blk * 128) * 3 + t] = v * sc;
+    gpart[((int64_t)blockIdx.y * npad + blk * NA) * 3 + t] = v * sc;
+  }
+}
+
+// Forces with the nlow/nhigh groups (ewald.c:104-111 with the per-atom
+// coefficients of ewald.c:105-107): the same blocks as k_ewald_force, k
+// vectors and the 6 group coefficients read per k from kvec and coef.
+template <int PREC, int P>
+__global__ __launch_bounds__(256) void k_ewald_force_groups(const ynb_sys *__restrict__ sys,
+                                                            const int32_t *__restrict__ wave_sys,
+                                                            const double *__restrict__ kvec,
+                                                            const double *__restrict__ coef,
+                                                            const int32_t *__restrict__ krow,
+                                                            const int32_t *__restrict__ krow_k,
+                                                            const int32_t *__restrict__ fblk,
+                                                            const double *__restrict__ frac,
+                                                            const double *__restrict__ step,
+                                                            double *__restrict__ gpart, int64_t npad) {
+#pragma clang fp contract(fast)
+  constexpr int NA = 64 * P;
+  __shared__ double l_g[3][4][NA];
+  const int wv = uniform_int(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int64_t blk = blockIdx.x;
+  const int sid = wave_sys[P * blk];
+  if (sid < 0) return;
+  const ynb_sys &S = sys[sid];
+  if (S.nk == 0 || (int)blockIdx.y >= S.nslot) return;  // the same for the whole block
+  double f0[P], f1[P], f2[P], cr[P], ci[P], zr[P], zi[P], wh[P], wl[P], G0[P], G1[P], G2[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int64_t g = blk * NA + lane + 64 * p;
+    f0[p] = frac[g];
+    f1[p] = frac[npad + g];
+    f2[p] = frac[2 * npad + g];
+    cr[p] = step[g];
+    ci[p] = step[npad + g];
+    const int la = (int)(g - S.atom0);
+    wh[p] = la < S.nhigh ? 1.0 : 0.0;
+    wl[p] = la >= S.nlow ? 1.0 : 0.0;
+    G0[p] = G1[p] = G2[p] = 0.0;
+  }
+  const int fb = 4 * (int)blockIdx.y + wv;
+  if (fb < S.nfrb) {
+    const int rb = fblk[2 * (S.frb0 + fb)], re = fblk[2 * (S.frb0 + fb) + 1];
+    int k = krow_k[rb];
+    for (int r = rb; r < re; ++r) {
+      const int4 row = *(const int4 *)(krow + 4 * r);
+#pragma unroll
+      for (int p = 0; p < P; ++p)  // row start: seed exp(2 pi i g.f)
+        seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, f0[p], f1[p], f2[p]), &zi[p], &zr[p]);
+      for (int j = 0; j < row.w; ++j, ++k) {
+        const double4 kv = *(const double4 *)(kvec + 4 * (int64_t)k);
+        const double *cf = coef + 6 * (int64_t)k;
+        const double c0 = cf[0], c1 = cf[1], c2 = cf[2], c3 = cf[3], c4 = cf[4], c5 = cf[5];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const double ac = c0 + wh[p] * c4 + wl[p] * c2;
+          const double as = c1 + wh[p] * c5 + wl[p] * c3;
+          // x = 2c (S_s cos - S_c sin); q is applied at the end
+          const double x = as * zr[p] - ac * zi[p];
+          G0[p] = fma(kv.x, x, G0[p]);
+          G1[p] = fma(kv.y, x, G1[p]);
+          G2[p] = fma(kv.z, x, G2[p]);
+          const double nr = zr[p] * cr[p] - zi[p] * ci[p];
+          zi[p] = zr[p] * ci[p] + zi[p] * cr[p];
+          zr[p] = nr;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    l_g[0][wv][lane + 64 * p] = G0[p];
+    l_g[1][wv][lane + 64 * p] = G1[p];
+    l_g[2][wv][lane + 64 * p] = G2[p];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 3 * NA; t += 256) {
+    const int a = t / 3, c = t - 3 * a;
+    const int64_t g = blk * NA + a;
+    const double v = ((l_g[c][0][a] + l_g[c][1][a]) + l_g[c][2][a]) + l_g[c][3][a];
+    const double sc = (g - S.atom0 < S.natom) ? frac[3 * npad + g] / S.dielectric : 0.0;
+    gpart[((int64_t)blockIdx.y * npad + blk * NA) * 3 + t] = v * sc;
   }
 }
 
@@ -536,8 +620,9 @@ using namespace ynb;
 
 extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const double *pos, const double *charges,
                               int32_t precision, double *gpos, int32_t accumulate, double *out, ynb_stream stream) {
-  if (!b || !ew || !pos || !charges || !out || b->npad % 128 != 0 || ew->nk_tot % 256 != 0 || ew->fslots < 0 ||
-      (ew->apl != 2 && ew->apl != 4 && ew->apl != 8) || (precision != YNB_FP64 && precision != YNB_MIXED)) {
+  if (!b || !ew || !pos || !charges || !out || b->npad % 256 != 0 || ew->nk_tot % 256 != 0 || ew->fslots < 0 ||
+      (ew->apl != 2 && ew->apl != 4 && ew->apl != 8) || (ew->fapl != 2 && ew->fapl != 4) ||
+      (precision != YNB_FP64 && precision != YNB_MIXED)) {
     set_error("ynb_ewald_reci: invalid arguments");
     return YNB_EINVAL;
   }
@@ -568,17 +653,27 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
     YNB_LAUNCHED();
   }
   if (ew->fslots > 0) {
-    const dim3 fgrid((unsigned)(npad / 128), (unsigned)ew->fslots);
-#define YNB_FORCE(PR, GR)                                                                                     \
-  YNB_LAUNCH("ewald_force", (k_ewald_force<PR, GR>), fgrid, dim3(256), 0, st, b->sys, b->wave_sys, ew->kvec,     \
-             ew->coef, ew->fk, ew->krow, ew->krow_k, ew->fblk, ew->frac, ew->step, ew->gpart, npad)
+    const int P = ew->fapl;
+    const dim3 fgrid((unsigned)(npad / (64 * P)), (unsigned)ew->fslots);
+#define YNB_FORCE(PR, PP)                                                                                     \
+  YNB_LAUNCH("ewald_force", (k_ewald_force<PR, PP>), fgrid, dim3(256), 0, st, b->sys, b->wave_sys, ew->frec,    \
+             ew->krow, ew->krow_f, ew->fblk, ew->frac, ew->step, ew->gpart, npad)
+#define YNB_FORCE_G(PR, PP)                                                                                   \
+  YNB_LAUNCH("ewald_force", (k_ewald_force_groups<PR, PP>), fgrid, dim3(256), 0, st, b->sys, b->wave_sys,       \
+             ew->kvec, ew->coef, ew->krow, ew->krow_k, ew->fblk, ew->frac, ew->step, ew->gpart, npad)
+#define YNB_FORCE_P(PR)                                            \
+  if (ew->groups) {                                                \
+    if (P == 2) YNB_FORCE_G(PR, 2); else YNB_FORCE_G(PR, 4);       \
+  } else {                                                         \
+    if (P == 2) YNB_FORCE(PR, 2); else YNB_FORCE(PR, 4);           \
+  }
     if (precision == YNB_FP64) {
-      if (ew->groups) YNB_FORCE(YNB_FP64, true);
-      else YNB_FORCE(YNB_FP64, false);
+      YNB_FORCE_P(YNB_FP64);
     } else {
-      if (ew->groups) YNB_FORCE(YNB_MIXED, true);
-      else YNB_FORCE(YNB_MIXED, false);
+      YNB_FORCE_P(YNB_MIXED);
     }
+#undef YNB_FORCE_P
+#undef YNB_FORCE_G
 #undef YNB_FORCE
     YNB_LAUNCHED();
   }

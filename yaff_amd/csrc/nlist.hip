@@ -225,6 +225,10 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *sys, co
   __syncthreads();
   const int T = l_pre[nnb];
   const double rc2 = S.rc * S.rc;
+  // per-system values used in the scan, held in registers: the scan stores
+  // rows through pointers the compiler cannot separate from sys, so reads of
+  // S inside it would be repeated memory loads
+  const int atom0 = S.atom0, nlow = S.nlow, nhigh = S.nhigh;
   const float lo32 = (float)(rc2 * (1.0 - 2e-4)), hi32 = (float)(rc2 * (1.0 + 2e-4));
   bool bad = false;
 
@@ -268,7 +272,7 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *sys, co
       wi = sw[p];
     }
     const int64_t gi = active ? wi.w : 0;
-    const int li = (int)(gi - S.atom0);
+    const int li = (int)(gi - atom0);
     const float xr = (float)(xi.x - X0), yr = (float)(xi.y - X1), zr = (float)(xi.z - X2);
     double pi[3] = {0, 0, 0};
     if (EXACT && active) {
@@ -303,9 +307,9 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *sys, co
             n[1] += wi.y;
             n[2] += wi.z;
             const bool self = (gj == gi);
-            const int lj = gj - S.atom0;
+            const int lj = gj - atom0;
             const int A = li > lj ? li : lj, B = li > lj ? lj : li;
-            acc = !(self && n[0] == 0 && n[1] == 0 && n[2] == 0) && nlow_nhigh_ok(A, B, S.nlow, S.nhigh);
+            acc = !(self && n[0] == 0 && n[1] == 0 && n[2] == 0) && nlow_nhigh_ok(A, B, nlow, nhigh);
             if (EXACT && acc && !(fast_ok && d2 < lo32)) {
               const bool i_is_a = (li == A);
               const int nab[3] = {i_is_a ? n[0] : -n[0], i_is_a ? n[1] : -n[1], i_is_a ? n[2] : -n[2]};

@@ -24,39 +24,42 @@ constexpr double TWO_DIV_SQRT_PI = 1.1283791670955125739;  // yaff/pes/constants
 
 // Transcendentals of the two precisions.
 //
-// FP64: exp from the device library; erfc(x) = exp(-x^2) erfcx(x) with erfcx
-// a Chebyshev series (25 terms, relative error < 3e-14 on [0, 27], table
-// generated by gen_erfcx_table.py) in t = 2/(2 + x), so that erfc and the EI
-// gradient share one exp(-x^2) and the evaluation needs few registers.
+// FP64: exp_fp64 (Cody-Waite, Taylor to r^13); erfc(x) = exp(-x^2) erfcx(x) with erfcx
+// a degree-24 polynomial in y = 4/(2 + x) - 1 (relative error < 3e-14 on
+// [0, 27], table generated by gen_erfcx_table.py), so that erfc and the EI
+// gradient share one exp(-x^2).
 //
 // Mixed: every argument reduction in FP64, only a well-conditioned
 // remainder in FP32:
 //   exp(a)   = 2^k * exp2f(f),  k = rint(a log2 e), f = a log2 e - k in FP64;
 //   erfc(x)  = exp(-x^2) * erfcxf(x), erfcx = exp(x^2) erfc(x) being smooth,
 //              so rounding x to FP32 barely moves it.
-__device__ inline double erfcx_cheb(double x) {  // x >= 0
-  const double t = 2.0 / (2.0 + x);
-  const double y = 2.0 * t - 1.0;
-  const double y2 = 2.0 * y;
-  double b1 = 0.0, b2 = 0.0;
+__device__ inline double erfcx_poly(double x) {  // x >= 0
+  // y = 4 / (2 + x) - 1; p(y) = E(y^2) + y O(y^2), the even and odd parts by
+  // Horner's rule in y^2 (two independent chains of 12 and 11 steps)
+  const double y = fma(4.0, rcp_fp64(2.0 + x), -1.0);
+  const double z = y * y;
+  double e = ERFCX_MONO[ERFCX_NCOEF - 1];
+  double o = ERFCX_MONO[ERFCX_NCOEF - 2];
 #pragma unroll
-  for (int k = ERFCX_NCOEF - 1; k >= 1; --k) {
-    const double b0 = fma(y2, b1, ERFCX_COEF[k] - b2);
-    b2 = b1;
-    b1 = b0;
+  for (int j = (ERFCX_NCOEF - 3) / 2; j >= 0; --j) {
+    e = fma3(e, z, ERFCX_MONO[2 * j]);
+    if (2 * j + 1 < ERFCX_NCOEF - 2) o = fma3(o, z, ERFCX_MONO[2 * j + 1]);
   }
-  return fma(y, b1, ERFCX_COEF[0] - b2);
+  return fma(y, o, e);
 }
 
 template <int PREC>
 struct Tr {
-  __device__ static double ex(double a) { return exp(a); }
+  __device__ static double ex(double a) { return exp_fp64(a); }
   // erfc(x) given e2 = exp(-x^2)
   __device__ static double erfc_e2(double x, double e2) {
-    return x >= 0.0 ? e2 * erfcx_cheb(x) : 2.0 - e2 * erfcx_cheb(-x);
+    return x >= 0.0 ? e2 * erfcx_poly(x) : 2.0 - e2 * erfcx_poly(-x);
   }
-  __device__ static double erfc_(double x) { return erfc_e2(x, exp(-x * x)); }
+  __device__ static double erfc_(double x) { return erfc_e2(x, exp_fp64(-x * x)); }
   __device__ static double erf_(double x) { return 1.0 - erfc_(x); }
+  // erfc(x) for x >= 0 given e2 = exp(-x^2)
+  __device__ static double erfc_pos(double x, double e2) { return e2 * erfcx_poly(x); }
 };
 template <>
 struct Tr<YNB_MIXED> {
@@ -68,6 +71,7 @@ struct Tr<YNB_MIXED> {
   __device__ static double erfc_(double x) { return ex(-x * x) * (double)erfcxf((float)x); }
   __device__ static double erf_(double x) { return 1.0 - erfc_(x); }
   __device__ static double erfc_e2(double x, double e2) { return e2 * (double)erfcxf((float)x); }
+  __device__ static double erfc_pos(double x, double e2) { return e2 * (double)erfcxf((float)x); }
 };
 
 template <int PREC>
@@ -186,11 +190,12 @@ struct PotFast {
     g = -e * b * inv;
     return e;
   }
-  // alpha > 0, point charges (the common case); qq = qa qb / dielectric
+  // alpha > 0, point charges (the common case); qq = qa qb / dielectric.
+  // x = alpha d > 0, so erfc needs only its non-negative branch.
   __device__ static double ei_point(double qq, double alpha, double d, double inv, double inv2, double &g) {
     const double x = alpha * d;
     const double e2 = R::ex(-x * x);
-    const double e = qq * R::erfc_e2(x, e2) * inv;
+    const double e = qq * R::erfc_pos(x, e2) * inv;
     g = (-TWO_DIV_SQRT_PI * alpha * e2 * qq - e) * inv2;
     return e;
   }
@@ -468,7 +473,11 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
         u[1] = fma(na, S.rvecs[3 * a + 1], u[1]);
         u[2] = fma(na, S.rvecs[3 * a + 2], u[2]);
       }
-      double dd = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      // 1/d from the device rsqrt (a few ulp) and d = r2 / d: every decision
+      // that depends on the last bits of d is made on the exact path below
+      const double r2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+      double inv = rsqrt(r2);
+      double dd = r2 * inv;
       bool near_cut = false;
       for (int p = 0; p < npot; ++p) near_cut |= fabs(dd - S.pot[p].rcut) <= 1e-12 * S.pot[p].rcut;
       if (near_cut) {
@@ -484,10 +493,10 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
         u[1] = i_is_a ? v[1] : -v[1];
         u[2] = i_is_a ? v[2] : -v[2];
         dd = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        inv = 1.0 / dd;
       }
       if (!(dd < rcmax)) continue;
       const double d = dd;
-      const double inv = 1.0 / d;
       const double inv2 = inv * inv;
       const double4 rj = *(const double4 *)(pp.rec + 4 * gj);
       double e_sum = 0.0, vg_sum = 0.0;

@@ -1,9 +1,9 @@
 """Batched device engine: lays out systems in HBM and drives libyaff_nb.so.
 
 A batch is any number of independent systems.  Their atoms are concatenated in
-padded ranges (each system starts at a multiple of 128 atoms, so neither a
-wavefront nor a two-wavefront block spans two systems); every launch covers
-the whole batch.  A single
+padded ranges (each system starts at a multiple of 256 atoms, so neither a
+wavefront nor an atom block of the Ewald force pass spans two systems); every
+launch covers the whole batch.  A single
 system -- the drop-in ForceField path -- is a batch of one.
 
 Device memory is held in torch tensors (plumbing only); every kernel is a
@@ -23,7 +23,7 @@ __all__ = ['PotSpec', 'EwaldSpec', 'NBSystem', 'Engine', 'ewald_kvectors']
 
 TWO_PI = 6.2831853071795864769
 FOUR_PI = 12.5663706143591729538
-ALIGN = 128          # systems start at multiples of 128 atoms (k_ewald_force takes 128-atom blocks)
+ALIGN = 256          # systems start at multiples of 256 atoms (k_ewald_force takes blocks of up to 256 atoms)
 KBLOCK = 256         # k vectors per reduction block
 
 
@@ -163,6 +163,13 @@ def _ewald_local_plan(s, apl, skb, fkb):
         np.zeros((0, 4), np.int32)
     skblk = _row_blocks(cnt, skb)
     fblk = _row_blocks(cnt, fkb)
+    # force records: rows padded to multiples of 4 records
+    rlen4 = (cnt + 3)//4*4
+    krow_f = (np.cumsum(rlen4) - rlen4).astype(np.int32)
+    kpos = np.full(nkp, -1, np.int32)
+    if nk:
+        rid = np.repeat(np.arange(len(st)), cnt)
+        kpos[:nk] = krow_f[rid] + (np.arange(nk) - st[rid])
     # atom chunks: below nlow (group 1), [nlow, nhigh) (group 0), above nhigh (group 2)
     nhigh = s.natom if ew.nhigh == -1 else ew.nhigh
     nlow = ew.nlow
@@ -176,7 +183,7 @@ def _ewald_local_plan(s, apl, skb, fkb):
     work = np.stack([np.zeros(wb.size, np.int64), wb.ravel(), wc.ravel(), np.zeros(wb.size, np.int64)],
                     axis=1).astype(np.int32)
     plan = dict(gmax=gmax, nk=nk, nkp=nkp, kpad=kpad, krow=krow, krow_k=st.astype(np.int32), nrow=len(st),
-                skblk=skblk, fblk=fblk, chunks=chunks, work=work,
+                skblk=skblk, fblk=fblk, chunks=chunks, work=work, krow_f=krow_f, kpos=kpos, nrec=int(rlen4.sum()),
                 grouped=bool(nlow > 0 or nhigh < s.natom),
                 nchunk_low=int((chunks[:, 3] == 1).sum()), nchunk_high=int((chunks[:, 3] == 2).sum()))
     cache[pkey] = plan
@@ -205,7 +212,7 @@ def _ceil_div(a, b):
 class Engine(object):
     """Device state and launches for a batch of NBSystem objects."""
 
-    def __init__(self, systems, precision='fp64', apl=4, skb=None, fkb=None, kcap=None, device='cuda',
+    def __init__(self, systems, precision='fp64', apl=4, skb=None, fkb=None, fapl=4, kcap=None, device='cuda',
                  pair_lanes=32, upload=True):
         self.lib = _lib.load()
         if not torch.cuda.is_available():
@@ -215,7 +222,7 @@ class Engine(object):
         self.precision = {'fp64': FP64, 'mixed': MIXED}[precision] if isinstance(precision, str) else int(precision)
         self.nsys = len(self.systems)
         self.pair_lanes = int(pair_lanes)
-        self.apl, self.skb, self.fkb = int(apl), skb, fkb
+        self.apl, self.skb, self.fkb, self.fapl = int(apl), skb, fkb, int(fapl)
         self._layout()
         self._plan_sys()
         self._alloc(kcap)
@@ -298,7 +305,7 @@ class Engine(object):
         if self.has_ewald:
             self._plan_ewald()
 
-    def _plan_ewald(self, apl=None, skb=None, fkb=None):
+    def _plan_ewald(self, apl=None, skb=None, fkb=None, fapl=None):
         """Rows, row blocks, atom chunks and work items of the Ewald passes:
         the cached per-system tables joined with their batch offsets.  Unset
         parameters get defaults that give both passes a few thousand
@@ -311,6 +318,10 @@ class Engine(object):
         if apl not in (2, 4, 8):
             raise ValueError('apl must be 2, 4 or 8')
         self.apl = apl
+        fapl = self.fapl if fapl is None else int(fapl)
+        if fapl not in (2, 4):
+            raise ValueError('fapl must be 2 or 4')
+        self.fapl = fapl
         nks = {i: len(ewald_kvectors_cached(self.systems[i])[1]) for i in ew_sys}
         if skb is None:
             skb = self.skb
@@ -323,14 +334,14 @@ class Engine(object):
         if fkb is None:
             fkb = self.fkb
         if fkb is None:
-            # force wavefronts: sum over systems of 128-atom blocks x row blocks
+            # force wavefronts: sum over systems of atom blocks x row blocks
             fkb = 512
-            while fkb > 16 and sum(max(1, nks[i]//fkb)*_ceil_div(self.systems[i].natom, 128)
+            while fkb > 16 and sum(max(1, nks[i]//fkb)*_ceil_div(self.systems[i].natom, 64*fapl)
                                    for i in ew_sys) < 4096:
                 fkb //= 2
         self.skb, self.fkb = int(skb), int(fkb)
-        kvec, krow, krow_k, skblk, fblk, chunk, work, kblk_sys = [], [], [], [], [], [], [], []
-        k0 = row0 = sb0 = fb0 = chunk0 = 0
+        kvec, krow, krow_k, skblk, fblk, chunk, work, kblk_sys, krow_f, kpos = [], [], [], [], [], [], [], [], [], []
+        k0 = row0 = sb0 = fb0 = chunk0 = rec0 = 0
         spart0 = 0
         fslots = 0
         self.gmax = [None]*self.nsys
@@ -352,6 +363,8 @@ class Engine(object):
             krow_k.append(p['krow_k'] + k0)
             skblk.append(p['skblk'] + row0)
             fblk.append(p['fblk'] + row0)
+            krow_f.append(p['krow_f'] + rec0)
+            kpos.append(np.where(p['kpos'] >= 0, p['kpos'] + rec0, -1).astype(np.int32))
             ch = p['chunks'].copy()
             ch[:, 0] = i
             chunk.append(ch)
@@ -378,6 +391,7 @@ class Engine(object):
             sb0 += len(p['skblk'])
             fb0 += len(p['fblk'])
             chunk0 += nch
+            rec0 += p['nrec']
 
         def cat(parts, empty):
             parts = [x for x in parts if len(x)]
@@ -391,7 +405,10 @@ class Engine(object):
             chunk=cat(chunk, np.zeros((1, 4), np.int32)),
             work=cat(work, np.zeros((1, 4), np.int32)),
             kblk_sys=cat(kblk_sys, np.zeros(1, np.int32)),
+            kpos=cat(kpos, np.full(KBLOCK, -1, np.int32)),
+            krow_f=cat(krow_f, np.zeros(1, np.int32)),
         )
+        self.nfrec = rec0 + 4  # 4 spare records: the force pass reads one group ahead
         self.nk_tot = k0
         self.nwork = int(sum(len(w) for w in work))
         self.spart_entries = max(spart0, 1)
@@ -515,24 +532,25 @@ class Engine(object):
         self.ew_step_d = torch.zeros(2*self.npad, dtype=torch.float64, device=dev)
         self.ew_spart_d = torch.zeros(2*self.spart_entries, dtype=torch.float64, device=dev)
         self.ew_coef_d = torch.zeros(6*max(self.nk_tot, KBLOCK), dtype=torch.float64, device=dev)
-        self.ew_fk_d = torch.zeros(8*max(self.nk_tot, KBLOCK), dtype=torch.float64, device=dev)
+        self.ew_frec_d = torch.zeros(2*self.nfrec, dtype=torch.float64, device=dev)
         self.ew_kpart_d = torch.zeros(8*max(self.nk_tot//32, 1), dtype=torch.float64, device=dev)
         self.ew_gpart_d = torch.zeros(max(self.fslots, 1)*self.npad*3, dtype=torch.float64, device=dev)
         t = self.ew_tabs
         self.ewald = YnbEwald(t['kvec'].data_ptr(), t['krow'].data_ptr(), t['krow_k'].data_ptr(),
                               t['skblk'].data_ptr(), t['fblk'].data_ptr(), t['chunk'].data_ptr(),
-                              t['work'].data_ptr(), t['kblk_sys'].data_ptr(), self.nwork,
-                              self.nk_tot, self.apl, self.fslots, self.groups, 0, self.ew_frac_d.data_ptr(),
+                              t['work'].data_ptr(), t['kblk_sys'].data_ptr(), t['kpos'].data_ptr(),
+                              t['krow_f'].data_ptr(), self.nwork,
+                              self.nk_tot, self.apl, self.fslots, self.groups, self.fapl, self.ew_frac_d.data_ptr(),
                               self.ew_step_d.data_ptr(), self.ew_spart_d.data_ptr(), self.ew_coef_d.data_ptr(),
-                              self.ew_fk_d.data_ptr(), self.ew_kpart_d.data_ptr(), self.ew_gpart_d.data_ptr())
+                              self.ew_frec_d.data_ptr(), self.ew_kpart_d.data_ptr(), self.ew_gpart_d.data_ptr())
 
     def tuning(self):
         """The launch parameters an autotuner may change."""
         e = self.has_ewald
         return {'pair_lanes': self.pair_lanes, 'apl': self.apl if e else None, 'skb': self.skb if e else None,
-                'fkb': self.fkb if e else None}
+                'fkb': self.fkb if e else None, 'fapl': self.fapl if e else None}
 
-    def set_tuning(self, pair_lanes=None, apl=None, skb=None, fkb=None):
+    def set_tuning(self, pair_lanes=None, apl=None, skb=None, fkb=None, fapl=None):
         """Changes launch parameters in place (results are unchanged up to
         summation order).  Ewald tables are re-planned when their parameters
         change."""
@@ -541,8 +559,8 @@ class Engine(object):
                 raise ValueError('pair_lanes must be 16, 32 or 64')
             self.pair_lanes = int(pair_lanes)
             self.pairpar.lanes = self.pair_lanes
-        if self.has_ewald and (apl is not None or skb is not None or fkb is not None):
-            self._plan_ewald(apl, skb, fkb)
+        if self.has_ewald and (apl is not None or skb is not None or fkb is not None or fapl is not None):
+            self._plan_ewald(apl, skb, fkb, fapl)
             self.sys_d = self._t(np.frombuffer(bytes(self.hsys), np.uint8))
             self.batch.sys = self.sys_d.data_ptr()
             self._alloc_ewald()

@@ -38,7 +38,7 @@ from .batch import nbsystem, timestep, maxwell_boltzmann
 __all__ = ['Request', 'Server', 'estimate_bytes', 'poisson_requests', 'latency_summary', 'SERVER_TUNING']
 
 # Fixed launch parameters of the server: results do not depend on the batch.
-SERVER_TUNING = dict(pair_lanes=32, apl=4, skb=64, fkb=64)
+SERVER_TUNING = dict(pair_lanes=32, apl=4, skb=64, fkb=64, fapl=4)
 
 
 class Request(object):
@@ -188,7 +188,7 @@ class Server(object):
         old_members = self.engine_members
         eng = Engine([r.nb for r in self.active], precision=self.precision, upload=False,
                      pair_lanes=self.tuning['pair_lanes'], apl=self.tuning['apl'], skb=self.tuning['skb'],
-                     fkb=self.tuning['fkb'])
+                     fkb=self.tuning['fkb'], fapl=self.tuning['fapl'])
         host_pos = np.zeros((eng.npad, 3))
         host_vel = np.zeros((eng.npad, 3))
         src, dst = [], []
