@@ -192,6 +192,30 @@ def cpu_baseline(s, evals):
     return n*evals/t, t, res
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch (read, written) of the FP64 instance of `kernel`
+    (bench name, e.g. 'ewald_force') from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of the 6912-atom rocksalt step (counters
+    scaled by a copy kernel of known size); None when absent."""
+    path = os.path.join(ROOT, 'profiles', 'rocksalt6912_hbm_traffic.md')
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    if 'Machine-readable' not in text:
+        return None
+    block = text.split('Machine-readable', 1)[1].split('```')[1]
+    for line in block.strip().splitlines():
+        parts = line.rsplit(' ', 2)
+        name = parts[0]
+        base = name.split('<')[0]
+        if base != 'k_' + kernel:
+            continue
+        if '<' in name and not name.split('<')[1].startswith('0'):
+            continue
+        return float(parts[1]), float(parts[2])
+    return None
+
+
 def progress(msg):
     print('bench: %s' % msg, file=sys.stderr, flush=True)
 
@@ -366,8 +390,11 @@ def main():
             v['frac_fp64_peak'] = v['achieved_tflops']/FP64_PEAK_TFLOPS
     if dom in algo:
         ach = algo[dom]/(kt[dom]['avg_ms']*1e-3)/1e12
+        tr = profiled_traffic(dom)
         roofline = {'kernel': dom, 'bound': 'valu', 'achieved': ach, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                    'frac': ach/FP64_PEAK_TFLOPS, 'traffic': None,
+                    'frac': ach/FP64_PEAK_TFLOPS, 'traffic': (tr[0] + tr[1]) if tr else None,
+                    'traffic_source': 'profiles/rocksalt6912_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE '
+                                      'per launch, scaled by a 512 MiB copy kernel)' if tr else None,
                     'algorithmic_flop_per_launch': algo[dom]}
     else:
         roofline = {'kernel': dom, 'bound': 'hbm', 'achieved': None, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
