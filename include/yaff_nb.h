@@ -132,6 +132,8 @@ typedef struct ynb_batch {
 #define YNB_BATCH_FORM_GENERIC 4 /* some system has pair_form YNB_FORM_GENERIC */
 #define YNB_BATCH_FORM_LJ 8      /* some system has pair_form YNB_FORM_LJ */
 #define YNB_BATCH_FORM_IONIC 16  /* some system has pair_form YNB_FORM_IONIC */
+#define YNB_BATCH_WIDE_SEARCH 32 /* some binned system searches more than 125 partner bins
+                                    ((2 range0 + 1)(2 range1 + 1)(2 range2 + 1) > 125) */
 
 /* Full neighbour list, one row of kcap entries per atom: entry k of atom i is
  * at i*kcap + k.

@@ -17,6 +17,7 @@ TR_NONE, TR_SWITCH3, TR_HAMMER = 0, 1, 2
 FP64, MIXED = 0, 1
 BATCH_ALLPAIRS, BATCH_EXACT_BUILD = 1, 2
 BATCH_FORM_GENERIC, BATCH_FORM_LJ, BATCH_FORM_IONIC = 4, 8, 16
+BATCH_WIDE_SEARCH = 32
 FORM_GENERIC, FORM_LJ, FORM_IONIC = 0, 1, 2
 
 ROW_DTYPE = np.dtype([('a', np.int64), ('b', np.int64), ('d', np.float64), ('dx', np.float64),

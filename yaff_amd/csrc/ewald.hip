@@ -628,7 +628,7 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
   }
   hipStream_t st = (hipStream_t)stream;
   const int64_t npad = b->npad;
-  hipLaunchKernelGGL(k_ewald_prep, dim3((unsigned)ceil_div(npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys, npad,
+  YNB_LAUNCH("ewald_prep", k_ewald_prep, dim3((unsigned)ceil_div(npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys, npad,
                      pos, charges, ew->frac, ew->step);
   YNB_LAUNCHED();
   if (ew->nwork > 0) {
@@ -678,11 +678,11 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
     YNB_LAUNCHED();
   }
   if (gpos) {
-    hipLaunchKernelGGL(k_ewald_gsum, dim3((unsigned)ceil_div(3 * npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys,
+    YNB_LAUNCH("ewald_gsum", k_ewald_gsum, dim3((unsigned)ceil_div(3 * npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys,
                        ew->gpart, npad, gpos, accumulate);
     YNB_LAUNCHED();
   }
-  hipLaunchKernelGGL(k_ewald_sysreduce, dim3((unsigned)b->nsys), dim3(256), 0, st, b->sys, b->nsys, *ew,
+  YNB_LAUNCH("ewald_sysreduce", k_ewald_sysreduce, dim3((unsigned)b->nsys), dim3(256), 0, st, b->sys, b->nsys, *ew,
                      out);
   YNB_LAUNCHED();
   return YNB_OK;

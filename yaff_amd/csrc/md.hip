@@ -172,7 +172,7 @@ int ynb_verlet_kick(const ynb_batch *b, double *vel, const double *gpos, const d
   YNB_LAUNCH("verlet_kick", k_kick, dim3((unsigned)ceil_div(b->npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys,
              b->npad, vel, gpos, minv, masses, dt, partial);
   YNB_LAUNCHED();
-  hipLaunchKernelGGL(k_wave_to_sys, dim3((unsigned)b->nsys), dim3(256), 0, st, b->sys, partial, ekin2);
+  YNB_LAUNCH("verlet_ekin", k_wave_to_sys, dim3((unsigned)b->nsys), dim3(256), 0, st, b->sys, partial, ekin2);
   YNB_LAUNCHED();
   return YNB_OK;
 }
@@ -183,7 +183,7 @@ int ynb_total(const ynb_batch *b, const double *pair_out, const double *ewald_ou
     set_error("ynb_total: invalid arguments");
     return YNB_EINVAL;
   }
-  hipLaunchKernelGGL(k_total, dim3((unsigned)ceil_div(b->nsys, 64)), dim3(64), 0, (hipStream_t)stream, b->sys,
+  YNB_LAUNCH("total", k_total, dim3((unsigned)ceil_div(b->nsys, 64)), dim3(64), 0, (hipStream_t)stream, b->sys,
                      b->nsys, pair_out, ewald_out, total);
   YNB_LAUNCHED();
   return YNB_OK;

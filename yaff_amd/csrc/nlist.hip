@@ -135,6 +135,7 @@ __device__ inline void store_entry(const ynb_nlist &nl, int64_t npad, int64_t gi
 constexpr int BW = 8;         // atoms (wavefronts) per block of the binned build
 constexpr int CAP = 1024;     // partner candidates staged in LDS at once
 constexpr int MAXNB = 512;    // partner bins per home bin (ynb_sys_plan keeps (2R+1)^3 below)
+constexpr int NB_SMALL = 128; // partner-bin tables of the common search box (5 x 5 x 5 bins)
 
 // Lanes of the wave below this one that hold a set bit of mask.
 __device__ inline int lanes_below(uint64_t mask) {
@@ -158,20 +159,24 @@ __device__ inline int lanes_below(uint64_t mask) {
 // ynb_sys_plan) and the exact test is made by whoever needs it: the pair
 // kernel re-tests every entry within 1e-12 of a cutoff exactly and skips
 // d >= rc, and the row materialisation drops entries with exact d >= rc.
-template <bool EXACT>
-__global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *sys, const int32_t *bin_sys, int64_t npad,
-                                                          const double *pos, const int32_t *bin_start,
-                                                          const int32_t *bin_end, const double4 *sxw,
-                                                          const int4 *sw, ynb_nlist nl) {
+// NB: capacity of the partner-bin tables (NB_SMALL or MAXNB).
+template <bool EXACT, int NB>
+__global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restrict__ sys,
+                                                          const int32_t *__restrict__ bin_sys, int64_t npad,
+                                                          const double *__restrict__ pos,
+                                                          const int32_t *__restrict__ bin_start,
+                                                          const int32_t *__restrict__ bin_end,
+                                                          const double4 *__restrict__ sxw,
+                                                          const int4 *__restrict__ sw, ynb_nlist nl) {
   __shared__ float l_x[CAP], l_y[CAP], l_z[CAP];
   __shared__ int32_t l_m[CAP], l_g[CAP];
-  __shared__ int32_t l_pre[MAXNB + 1];
-  __shared__ int32_t l_bs[MAXNB];       // slot start of each partner bin
-  __shared__ double l_sh[3][MAXNB];     // image shift of each partner bin minus the home-bin centre
-  __shared__ int32_t l_s[MAXNB];        // packed shift (s0, s1, s2)
+  __shared__ int32_t l_pre[NB + 1];
+  __shared__ int32_t l_bs[NB];       // slot start of each partner bin
+  __shared__ double l_sh[3][NB];     // image shift of each partner bin minus the home-bin centre
+  __shared__ int32_t l_s[NB];        // packed shift (s0, s1, s2)
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6;
+  const int lane = tid & 63, wv = uniform_int(tid >> 6);
   const int sid = bin_sys[b];
   if (nl.rebuild && !nl.rebuild[sid]) return;
   const ynb_sys &S = sys[sid];
@@ -232,11 +237,13 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *sys, co
   const float lo32 = (float)(rc2 * (1.0 - 2e-4)), hi32 = (float)(rc2 * (1.0 + 2e-4));
   bool bad = false;
 
+  // Stage the candidates [base, base + CAP): candidate f belongs to the
+  // largest partner bin k with l_pre[k] <= f (binary search).
   auto stage = [&](int base) {
     const int lim = min(T - base, CAP);
     for (int q = tid; q < lim; q += BW * 64) {
       const int f = base + q;
-      int lo = 0, hi = nnb;  // largest k with l_pre[k] <= f
+      int lo = 0, hi = nnb;
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (l_pre[mid] <= f) lo = mid; else hi = mid;
@@ -294,45 +301,39 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *sys, co
       const int lim = min(T - base, CAP);
       for (int q0 = 0; q0 < lim; q0 += 64) {
         const int q = q0 + lane;
-        bool acc = false;
-        int32_t gj = 0, img = 0;
-        if (q < lim) {
-          const float dx = l_x[q] - xr, dy = l_y[q] - yr, dz = l_z[q] - zr;
-          const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          if (d2 < hi32) {
-            gj = l_g[q];
-            int n[3];
-            unpack_img(l_m[q], n);
-            n[0] += wi.x;
-            n[1] += wi.y;
-            n[2] += wi.z;
-            const bool self = (gj == gi);
-            const int lj = gj - atom0;
-            const int A = li > lj ? li : lj, B = li > lj ? lj : li;
-            acc = !(self && n[0] == 0 && n[1] == 0 && n[2] == 0) && nlow_nhigh_ok(A, B, nlow, nhigh);
-            if (EXACT && acc && !(fast_ok && d2 < lo32)) {
-              const bool i_is_a = (li == A);
-              const int nab[3] = {i_is_a ? n[0] : -n[0], i_is_a ? n[1] : -n[1], i_is_a ? n[2] : -n[2]};
-              const double pj[3] = {pos[3 * (int64_t)gj], pos[3 * (int64_t)gj + 1], pos[3 * (int64_t)gj + 2]};
-              double v[3];
-              int r[3];
-              acc = yaff_row_vector(i_is_a ? pi : pj, i_is_a ? pj : pi, self, nab, S.rvecs, S.gvecs, 3, S.rmax, v,
-                                    r) &&
-                    (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] < rc2);
-            }
-            if (acc && (n[0] < -511 || n[0] > 511 || n[1] < -511 || n[1] > 511 || n[2] < -511 || n[2] > 511)) {
-              bad = true;
-              acc = false;
-            }
-            if (acc) img = pack_img(n[0], n[1], n[2]);
-          }
+        const int qc = q < lim ? q : lim - 1;
+        // branch-free test of the candidate (all lanes read a valid slot)
+        const float dx = l_x[qc] - xr, dy = l_y[qc] - yr, dz = l_z[qc] - zr;
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const int32_t gj = l_g[qc];
+        int n[3];
+        unpack_img(l_m[qc], n);
+        n[0] += wi.x;
+        n[1] += wi.y;
+        n[2] += wi.z;
+        const bool self = (gj == gi);
+        const int lj = gj - atom0;
+        const int A = li > lj ? li : lj, B = li > lj ? lj : li;
+        bool acc = (q < lim) && (d2 < hi32) && !(self && n[0] == 0 && n[1] == 0 && n[2] == 0) &&
+                   nlow_nhigh_ok(A, B, nlow, nhigh);
+        if (EXACT && acc && !(fast_ok && d2 < lo32)) {
+          const bool i_is_a = (li == A);
+          const int nab[3] = {i_is_a ? n[0] : -n[0], i_is_a ? n[1] : -n[1], i_is_a ? n[2] : -n[2]};
+          const double pj[3] = {pos[3 * (int64_t)gj], pos[3 * (int64_t)gj + 1], pos[3 * (int64_t)gj + 2]};
+          double v[3];
+          int r[3];
+          acc = yaff_row_vector(i_is_a ? pi : pj, i_is_a ? pj : pi, self, nab, S.rvecs, S.gvecs, 3, S.rmax, v, r) &&
+                (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] < rc2);
         }
+        const bool far = (n[0] < -511) | (n[0] > 511) | (n[1] < -511) | (n[1] > 511) | (n[2] < -511) | (n[2] > 511);
+        bad |= acc && far;
+        acc = acc && !far;
         const uint64_t mask = __ballot(acc);
         if (acc) {
           const int o = cnt + lanes_below(mask);
           if (o < nl.kcap) {
             rowj[o] = gj;
-            rowi[o] = img;
+            rowi[o] = pack_img(n[0], n[1], n[2]);
           }
         }
         cnt += __popcll(mask);
@@ -705,19 +706,23 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
   const int tb = 256;
   const int nb = (int)ceil_div(npad, tb);
   if (b->nbin > 0) {
-    hipLaunchKernelGGL(k_fill_i32, dim3((int)ceil_div(b->nbin, tb)), dim3(tb), 0, st, w.bin_count,
+    YNB_LAUNCH("nlist_clear", k_fill_i32, dim3((int)ceil_div(b->nbin, tb)), dim3(tb), 0, st, w.bin_count,
                        (int64_t)b->nbin, 0);
     YNB_LAUNCH("nlist_bin", k_bin, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, cb, w.bin_count,
                w.slot_idx, w.xw, w.wr, nl->status, nl->rebuild);
     YNB_LAUNCH("nlist_bin_order", k_bin_order, dim3(b->nbin), dim3(256), 0, st, cb, w.bin_count, w.slot_idx, w.xw,
                w.wr, w.sxw, w.sw, w.bin_start, w.bin_end, b->bin_sys, nl->rebuild);
+#define YNB_BUILD(EX, NBB)                                                                                    \
+  YNB_LAUNCH("nlist_build_binned", (k_build_binned<EX, NBB>), dim3(b->nbin, ychunks), dim3(BW * 64), 0, st,        \
+             b->sys, b->bin_sys, npad, pos, w.bin_start, w.bin_end, w.sxw, w.sw, *nl)
     const int ychunks = (int)ceil_div(cb, BW) < 64 ? (int)ceil_div(cb, BW) : 64;
-    if (b->flags & YNB_BATCH_EXACT_BUILD)
-      YNB_LAUNCH("nlist_build_binned", k_build_binned<true>, dim3(b->nbin, ychunks), dim3(BW * 64), 0, st, b->sys,
-                 b->bin_sys, npad, pos, w.bin_start, w.bin_end, w.sxw, w.sw, *nl);
-    else
-      YNB_LAUNCH("nlist_build_binned", k_build_binned<false>, dim3(b->nbin, ychunks), dim3(BW * 64), 0, st, b->sys,
-                 b->bin_sys, npad, pos, w.bin_start, w.bin_end, w.sxw, w.sw, *nl);
+    const bool wide = (b->flags & YNB_BATCH_WIDE_SEARCH) != 0;
+    if (b->flags & YNB_BATCH_EXACT_BUILD) {
+      if (wide) YNB_BUILD(true, MAXNB); else YNB_BUILD(true, NB_SMALL);
+    } else {
+      if (wide) YNB_BUILD(false, MAXNB); else YNB_BUILD(false, NB_SMALL);
+    }
+#undef YNB_BUILD
     YNB_LAUNCHED();
   }
   if (b->flags & YNB_BATCH_ALLPAIRS) {

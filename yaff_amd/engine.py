@@ -499,8 +499,9 @@ class Engine(object):
         return rec
 
     def _batch_flags(self):
-        """Build paths the batch needs: all-pairs (systems without bins) and the
-        exact in-build re-test (binned systems that are not box_safe)."""
+        """Build paths the batch needs: all-pairs (systems without bins), the
+        exact in-build re-test (binned systems that are not box_safe) and the
+        large partner-bin tables (search boxes of more than 125 bins)."""
         f = 0
         for i in range(self.nsys):
             h = self.hsys[i]
@@ -508,6 +509,8 @@ class Engine(object):
                 f |= _lib.BATCH_ALLPAIRS
             if h.nbin > 0 and not h.box_safe:
                 f |= _lib.BATCH_EXACT_BUILD
+            if h.nbin > 0 and (2*h.range[0] + 1)*(2*h.range[1] + 1)*(2*h.range[2] + 1) > 125:
+                f |= _lib.BATCH_WIDE_SEARCH
             f |= {_lib.FORM_GENERIC: _lib.BATCH_FORM_GENERIC, _lib.FORM_LJ: _lib.BATCH_FORM_LJ,
                   _lib.FORM_IONIC: _lib.BATCH_FORM_IONIC}[h.pair_form]
         return f
