@@ -165,22 +165,28 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
         zr[p] = q[p] * cs;
         zi[p] = q[p] * sn;
       }
-      for (int j = 0; j < row.w; ++j) {
-        double sr = zr[0], si = zi[0];
+      for (int j = 0; j < row.w;) {
+        // k up to the end of the row or of the tile: a loop without branches
+        const int n = min(row.w - j, T - m);
+#pragma unroll 2
+        for (int i = 0; i < n; ++i) {
+          double sr = zr[0], si = zi[0];
 #pragma unroll
-        for (int p = 1; p < P; ++p) {
-          sr += zr[p];
-          si += zi[p];
-        }
-        l_t[m * RS + lane] = make_double2(sr, si);
+          for (int p = 1; p < P; ++p) {
+            sr += zr[p];
+            si += zi[p];
+          }
+          l_t[(m + i) * RS + lane] = make_double2(sr, si);
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-          const double nr = zr[p] * cr[p] - zi[p] * ci[p];
-          zi[p] = zr[p] * ci[p] + zi[p] * cr[p];
-          zr[p] = nr;
+          for (int p = 0; p < P; ++p) {
+            const double nr = zr[p] * cr[p] - zi[p] * ci[p];
+            zi[p] = zr[p] * ci[p] + zi[p] * cr[p];
+            zr[p] = nr;
+          }
         }
-        ++m;
-        if (m == T || (r + 1 == re && j + 1 == row.w)) {  // flush the tile
+        j += n;
+        m += n;
+        if (m == T || (r + 1 == re && j == row.w)) {  // flush the tile
           __syncthreads();
           double ar = 0.0, ai = 0.0;
           if (m_r < m) {
