@@ -300,8 +300,16 @@ def batched(nsys, steps, warmup):
     md.warm(warmup)
     md.capture(1)
     dt = md.run(steps)
+    from yaff_amd import _lib
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    for _ in range(10):
+        md.engine.md_step()
+    kt = kernel_table(_lib.profile_read(), 10)
+    _lib.profile_enable(False)
     out['batched_mixed_skin1A'] = {'value': natom*steps/dt, 'ms_per_step': dt/steps*1e3, 'steps': steps,
                                    'skin_angstrom': 1.0,
+                                   'kernels_ms_per_step': {k: v['ms_per_step'] for k, v in kt.items()},
                                    'note': 'same systems and tuning; Verlet skin, per-system rebuild decided on '
                                            'the device inside the graph'}
     progress('batched mixed skin 1A %.3g atom-evals/s' % (natom*steps/dt))

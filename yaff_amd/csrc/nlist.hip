@@ -142,7 +142,8 @@ __device__ inline int lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
 }
 
-// Block (bin b, chunk of BW atoms of the bin), one atom per wavefront.  All
+// Block (bin b, part y of its atoms), one atom per wavefront, the atoms
+// BW gridDim.y apart.  All
 // partner bins of the home bin are staged into LDS at once (CAP candidates
 // per stage) as FP32 coordinates relative to the home-bin centre with the
 // image shift applied.  The wave scans 64 candidates per step; accepted
@@ -715,7 +716,13 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
 #define YNB_BUILD(EX, NBB)                                                                                    \
   YNB_LAUNCH("nlist_build_binned", (k_build_binned<EX, NBB>), dim3(b->nbin, ychunks), dim3(BW * 64), 0, st,        \
              b->sys, b->bin_sys, npad, pos, w.bin_start, w.bin_end, w.sxw, w.sw, *nl)
-    const int ychunks = (int)ceil_div(cb, BW) < 64 ? (int)ceil_div(cb, BW) : 64;
+    // Bins hold a few atoms on average; the block's wavefronts loop over the
+    // atoms beyond BW.  Large batches take one block per bin (a block per BW
+    // atoms of the largest bin would mostly launch empty blocks); small ones
+    // add blocks over the bin's atoms while there are fewer than 8192 blocks.
+    const int yfull = (int)ceil_div(cb, BW);
+    const int ywant = (int)(8192 / (b->nbin > 0 ? b->nbin : 1));
+    const int ychunks = max(1, min(min(yfull, ywant), 64));
     const bool wide = (b->flags & YNB_BATCH_WIDE_SEARCH) != 0;
     if (b->flags & YNB_BATCH_EXACT_BUILD) {
       if (wide) YNB_BUILD(true, MAXNB); else YNB_BUILD(true, NB_SMALL);

@@ -452,6 +452,17 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
   const int nf = FORM == YNB_FORM_IONIC ? S.pot[0].nffatype : 0;
   const double *amp = pp.cross + (FORM == YNB_FORM_IONIC ? S.pot[0].cross0 : 0);
   const double *bex = amp + (int64_t)nf * nf;
+  // ExpRep cross tables of up to 16 atom types go to LDS (a block's atoms
+  // belong to one system, so the condition is the same for the block)
+  __shared__ double l_amp[256], l_bex[256];
+  const bool lds_cross = FORM == YNB_FORM_IONIC && nf <= 16;
+  if (lds_cross) {
+    for (int q = threadIdx.x; q < nf * nf; q += blockDim.x) {
+      l_amp[q] = amp[q];
+      l_bex[q] = bex[q];
+    }
+    __syncthreads();
+  }
 
   double acc_e = 0.0, acc_v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, acc_g[3] = {0.0, 0.0, 0.0};
   if (active) {
@@ -461,11 +472,39 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
     const int n = min(nl.cnt[gi], nl.kcap);
     const int32_t *rowj = nl.j + gi * nl.kcap;
     const int32_t *rowi = nl.img + gi * nl.kcap;
+    // Software pipeline over the lane's entries k = s, s + G, ...: the
+    // partner's position and record are requested one entry ahead and its
+    // index two entries ahead, so each entry's loads overlap the previous
+    // entry's arithmetic.  Past the row end the atom itself stands in (a
+    // valid address; its data are never used).
+    const int32_t gi32 = (int32_t)gi;
+    int32_t jA = s < n ? rowj[s] : gi32;
+    int32_t iA = s < n ? rowi[s] : 0;
+    int32_t jB = s + G < n ? rowj[s + G] : gi32;
+    int32_t iB = s + G < n ? rowi[s + G] : 0;
+    double pA[3] = {pos[3 * (int64_t)jA], pos[3 * (int64_t)jA + 1], pos[3 * (int64_t)jA + 2]};
+    double4 rA = *(const double4 *)(pp.rec + 4 * (int64_t)jA);
     for (int k = s; k < n; k += G) {
-      const int64_t gj = rowj[k];
+      const int64_t gj = jA;
+      const int32_t imgk = iA;
+      const double pj[3] = {pA[0], pA[1], pA[2]};
+      const double4 rj = rA;
+      jA = jB;
+      iA = iB;
+      pA[0] = pos[3 * (int64_t)jB];
+      pA[1] = pos[3 * (int64_t)jB + 1];
+      pA[2] = pos[3 * (int64_t)jB + 2];
+      rA = *(const double4 *)(pp.rec + 4 * (int64_t)jB);
+      // the index two entries ahead: an unconditional load of a valid entry
+      // (k itself past the row end), selected afterwards, so no branch makes
+      // the loads in flight wait
+      const int k2 = k + 2 * G;
+      const int kk = k2 < n ? k2 : k;
+      const int32_t jt = rowj[kk], it = rowi[kk];
+      jB = k2 < n ? jt : gi32;
+      iB = k2 < n ? it : 0;
       int nn[3];
-      unpack_img(rowi[k], nn);
-      const double pj[3] = {pos[3 * gj], pos[3 * gj + 1], pos[3 * gj + 2]};
+      unpack_img(imgk, nn);
       double u[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
       for (int a = 0; a < 3; ++a) {
         const double na = (double)nn[a];
@@ -498,7 +537,6 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
       if (!(dd < rcmax)) continue;
       const double d = dd;
       const double inv2 = inv * inv;
-      const double4 rj = *(const double4 *)(pp.rec + 4 * gj);
       double e_sum = 0.0, vg_sum = 0.0;
       if (FORM == YNB_FORM_LJ) {
         if (d < rc0) {
@@ -517,7 +555,7 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
         if (d < rc0) {  // ExpRep
           const int ci = ti * nf + (int)rj.w;
           double g;
-          e_sum += PF::exprep(amp[ci], bex[ci], d, inv, g);
+          e_sum += PF::exprep(lds_cross ? l_amp[ci] : amp[ci], lds_cross ? l_bex[ci] : bex[ci], d, inv, g);
           vg_sum += g;
         }
         if (d < rc1) {  // LJ
