@@ -133,15 +133,22 @@ def timed(eng, K, profile=True):
     return t1 - t0, prof
 
 
+def graph_steps(K, most=10):
+    """Steps per captured graph: the largest divisor of K up to `most`."""
+    return max(d for d in range(1, min(K, most) + 1) if K % d == 0)
+
+
 def timed_graph(eng, K, W):
-    """K replays of a HIP graph holding one MD step (after W warm replays)."""
+    """K MD steps as K/S replays of a HIP graph holding S steps
+    (S = graph_steps(K)), after at least W warm-up steps."""
     import torch
-    g = eng.capture_md(1)
-    for _ in range(W):
+    S = graph_steps(K)
+    g = eng.capture_md(S)
+    for _ in range(max(1, -(-W//S))):
         g.replay()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(K):
+    for _ in range(K//S):
         g.replay()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -276,8 +283,8 @@ def batched(nsys, steps, warmup):
         tun = autotune.tune(md.engine, cache=None)
         t_tune = time.perf_counter() - t0
         md.warm(warmup)
-        md.capture(1)
         nst = steps if prec == 'mixed' else max(steps//4, 1)
+        md.capture(graph_steps(nst))
         dt = md.run(nst)
         # per-kernel durations of a few eager steps of the same batch
         from yaff_amd import _lib
@@ -298,7 +305,7 @@ def batched(nsys, steps, warmup):
     md = BatchMD(systems, precision='mixed', skin=1.0*angstrom)
     md.engine.set_tuning(**out['batched_mixed']['tuning'])
     md.warm(warmup)
-    md.capture(1)
+    md.capture(graph_steps(steps))
     dt = md.run(steps)
     from yaff_amd import _lib
     torch.cuda.synchronize()
@@ -417,8 +424,8 @@ def main():
                       'natom': n, 'half_pairs': half_pairs, 'kvectors': nk_real, 'parallelism': 'single GPU',
                       'tuning': tuning},
            'roofline': roofline, 'kernels': kt,
-           'timing': 'value: K replays of a HIP graph holding one MD step; kernels: HIP events around each launch '
-                     'of K eager steps on the launch stream',
+           'timing': 'value: K MD steps as K/S replays of a HIP graph holding S = %d steps; kernels: HIP events '
+                     'around each launch of K eager steps on the launch stream' % graph_steps(K),
            'eager': {'value': n*K/dt_eager, 'ms_per_step': dt_eager/K*1e3}}
 
     # mixed precision on the same system: rate and error against FP64

@@ -143,11 +143,12 @@ typedef struct ynb_nlist {
   int32_t *j;
   int32_t *img;
   int32_t *cnt;        /* [npad] entries per atom (also counted past kcap) */
-  int32_t *status;     /* [4]: [0] flags: 1 row > kcap, 2 image shift > 511 cells, 4 bin > max_bin_atoms.
+  int32_t *status;     /* [4]: [0] flags: 1 row > kcap, 2 image shift > 511 cells, 4 bin > max_bin_atoms;
+                          [1] after flag 4, the fullest bin's atom count.
                           Flags are sticky (builds OR into them, the caller clears them), so
                           a check after many graph-replayed steps sees every overflow.  On a
-                          flag the caller reads cnt (rows) or the bin counts
-                          (ynb_nlist_bin_counts) to size the retry. */
+                          flag the caller reads cnt (rows) or status[1] (bins) to size the
+                          retry. */
   const int32_t *rebuild; /* [nsys] per-system rebuild flags (ynb_nlist_skin_check) or NULL: rebuild all */
   int32_t kcap;
   int32_t pad_;
@@ -160,6 +161,7 @@ typedef struct ynb_pairpar {
   const double *cross; /* ExpRep amp/b tables, row-major nffatype^2 each */
   const ynb_scaling_row *stab; /* scaling rows (a > b, local indices), sorted per system */
   const double *rec;   /* [npad][4] per-atom record of the specialised forms (YNB_FORM_*) */
+  int32_t *ctr;        /* [nsys] block counters of ynb_pair_compute (zero-initialised; left at zero) */
   int32_t lanes;       /* lanes per atom in ynb_pair_compute: 16, 32 or 64 (0: 16) */
   int32_t pad_;
 } ynb_pairpar;
@@ -184,6 +186,7 @@ typedef struct ynb_ewald {
   const int32_t *kblk_sys;/* [nk_tot/256]: system of each block of 256 k vectors */
   const int32_t *kpos;    /* [nk_tot]: force record of each k (-1: padding k) */
   const int32_t *krow_f;  /* [nrow_tot]: first force record of each row */
+  int32_t *fctr;          /* [npad/128] block counters of the force pass (zero-initialised; left at zero) */
   int32_t nwork;
   int32_t nk_tot;
   int32_t apl;            /* atoms per lane of the structure-factor pass: 2, 4 or 8 */
@@ -218,7 +221,8 @@ int ynb_profile_read(int i, const char **name, float *ms);
 int ynb_sys_plan(ynb_sys *sys);
 
 /* Workspace of ynb_nlist_build in bytes, for bins of bin_cap atoms
- * (ynb_batch.max_bin_atoms). */
+ * (ynb_batch.max_bin_atoms).  The caller zero-initialises it once; the build
+ * leaves its bin counters at zero. */
 size_t ynb_nlist_workspace_bytes(int64_t npad, int32_t nbin, int32_t bin_cap);
 
 /* Workspace of ynb_nlist_rows for a system of natom atoms and nrows rows. */
@@ -232,11 +236,6 @@ size_t ynb_rows_workspace_bytes(int64_t natom, int64_t nrows);
  * library sort and no synchronisation, so it can be captured in a graph. */
 int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t work_bytes,
                     const ynb_nlist *nl, ynb_stream stream);
-
-/* Copies the atom count of every bin of the last build (nbin int32) from the
- * workspace to `counts` (device); used to size max_bin_atoms after a bin
- * overflow. */
-int ynb_nlist_bin_counts(const ynb_batch *b, const void *work, int32_t *counts, ynb_stream stream);
 
 /* Verlet skin: rebuild[s] = 1 when skin[s] <= 0 or some atom of system s
  * moved by skin[s]/2 or more since pos_ref (then pos_ref of that system is
@@ -264,7 +263,8 @@ int ynb_nlist_recompute(const ynb_sys *h_sys, const double *pos, const double *p
  * gpos (nullable): [npad][3] gradient summed over the slots, written or, with
  * accumulate != 0, added.  gpos_slots (nullable): [YNB_MAX_POT][npad][3].
  * out: [nsys][YNB_MAX_POT][10] energy followed by the 3x3 virial.
- * partial: work of npad*lanes/256 * YNB_MAX_POT * 8 doubles. */
+ * partial: work of npad*lanes/256 * YNB_MAX_POT * 8 doubles; pp->ctr: the
+ * per-system block counters (the last block of a system reduces its sums). */
 int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const double *pos,
                      const ynb_nlist *nl, uint32_t slot_mask, int32_t precision,
                      double *gpos, int32_t accumulate, double *gpos_slots,
@@ -279,7 +279,11 @@ int ynb_pair_rows(const ynb_neigh_row *rows, int64_t nrow, const ynb_scaling_row
                   const double *cross, double *gpos, double *partial, double *out, ynb_stream stream);
 
 /* Ewald reciprocal term of every system with nk > 0.
- * out: [nsys][10] energy followed by the virial; gpos as in ynb_pair_compute. */
+ * out: [nsys][10] energy followed by the virial (zeros for systems without k
+ * vectors); gpos as in ynb_pair_compute.  The partial gradients of each
+ * system's force slots are left in ew->gpart ([slot][npad][3], ynb_sys.nslot
+ * slots); with gpos non-NULL the last force block of each atom block adds
+ * them into gpos (counters ew->fctr). */
 int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const double *pos, const double *charges,
                    int32_t precision, double *gpos, int32_t accumulate, double *out,
                    ynb_stream stream);
@@ -293,12 +297,19 @@ int ynb_ewald_corr(const ynb_sys *h_sys, const ynb_sys *d_sys, const double *pos
 /* Velocity Verlet halves (yaff/sampling/verlet.py:345-357): kick_drift does
  * vel += dt/2 * (-gpos/m); pos += dt * vel; kick does vel += dt/2 * (-gpos/m)
  * and writes 2*Ekin of each system to ekin2[nsys] via `partial`
- * (npad/64 doubles). */
+ * (npad/64 doubles; the last block of each system adds them, counters
+ * ctr[nsys] zero-initialised and left at zero).  gpart (NULL or the ynb_ewald
+ * gpart of a ynb_ewald_reci call made with gpos = NULL): kick first adds each
+ * system's nslot Ewald partial gradients to gpos, in slot order, and stores
+ * the sum in gpos -- the Ewald gradient sum folded into the kick.  total
+ * (NULL or as in ynb_total, with pair_out/ewald_out nullable): kick also
+ * totals each system's energy and virial, in place of a ynb_total call. */
 int ynb_verlet_kick_drift(const ynb_batch *b, double *pos, double *vel, const double *gpos,
                           const double *minv, const double *dt, ynb_stream stream);
-int ynb_verlet_kick(const ynb_batch *b, double *vel, const double *gpos, const double *minv,
-                    const double *masses, const double *dt, double *partial, double *ekin2,
-                    ynb_stream stream);
+int ynb_verlet_kick(const ynb_batch *b, double *vel, double *gpos, const double *gpart,
+                    const double *minv, const double *masses, const double *dt, double *partial,
+                    int32_t *ctr, double *ekin2, const double *pair_out, const double *ewald_out,
+                    double *total, ynb_stream stream);
 
 /* out[s][k] = sum over slots of in[s][slot][k] for k < 10, plus optional
  * extra [nsys][10] terms; used to total a force-field evaluation on device. */

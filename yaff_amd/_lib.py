@@ -56,13 +56,14 @@ class YnbNlist(C.Structure):
 
 class YnbPairpar(C.Structure):
     _fields_ = [('par', C.c_void_p), ('cross', C.c_void_p), ('stab', C.c_void_p), ('rec', C.c_void_p),
-                ('lanes', C.c_int32), ('pad_', C.c_int32)]
+                ('ctr', C.c_void_p), ('lanes', C.c_int32), ('pad_', C.c_int32)]
 
 
 class YnbEwald(C.Structure):
     _fields_ = [('kvec', C.c_void_p), ('krow', C.c_void_p), ('krow_k', C.c_void_p), ('skblk', C.c_void_p),
                 ('fblk', C.c_void_p), ('chunk', C.c_void_p), ('work', C.c_void_p), ('kblk_sys', C.c_void_p),
-                ('kpos', C.c_void_p), ('krow_f', C.c_void_p), ('nwork', C.c_int32), ('nk_tot', C.c_int32),
+                ('kpos', C.c_void_p), ('krow_f', C.c_void_p), ('fctr', C.c_void_p),
+                ('nwork', C.c_int32), ('nk_tot', C.c_int32),
                 ('apl', C.c_int32), ('fslots', C.c_int32),
                 ('groups', C.c_int32), ('fapl', C.c_int32), ('frac', C.c_void_p), ('step', C.c_void_p),
                 ('spart', C.c_void_p), ('coef', C.c_void_p), ('frec', C.c_void_p), ('kpart', C.c_void_p),
@@ -72,7 +73,7 @@ class YnbEwald(C.Structure):
 EXPORTS = ['ynb_version', 'ynb_last_error', 'ynb_sys_plan', 'ynb_nlist_workspace_bytes',
            'ynb_rows_workspace_bytes', 'ynb_nlist_build', 'ynb_nlist_rows', 'ynb_nlist_recompute',
            'ynb_pair_compute', 'ynb_pair_rows', 'ynb_ewald_reci', 'ynb_ewald_corr', 'ynb_verlet_kick_drift', 'ynb_verlet_kick',
-           'ynb_total', 'ynb_profile_enable', 'ynb_profile_count', 'ynb_profile_read', 'ynb_nlist_bin_counts',
+           'ynb_total', 'ynb_profile_enable', 'ynb_profile_count', 'ynb_profile_read',
            'ynb_nlist_skin_check']
 
 
@@ -100,7 +101,6 @@ def load(path=LIB_PATH):
     L.ynb_rows_workspace_bytes.restype = sz
     L.ynb_rows_workspace_bytes.argtypes = [i64, i64]
     L.ynb_nlist_build.argtypes = [P(YnbBatch), vp, vp, sz, P(YnbNlist), vp]
-    L.ynb_nlist_bin_counts.argtypes = [P(YnbBatch), vp, vp, vp]
     L.ynb_nlist_skin_check.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp]
     L.ynb_nlist_rows.argtypes = [P(YnbBatch), i32, vp, P(YnbNlist), vp, sz, vp, i64, P(i64), vp]
     L.ynb_nlist_recompute.argtypes = [P(YnbSys), vp, vp, vp, i64, vp]
@@ -110,7 +110,7 @@ def load(path=LIB_PATH):
     L.ynb_ewald_reci.argtypes = [P(YnbBatch), P(YnbEwald), vp, vp, i32, vp, i32, vp, vp]
     L.ynb_ewald_corr.argtypes = [P(YnbSys), vp, vp, vp, vp, i64, vp, i32, vp, vp]
     L.ynb_verlet_kick_drift.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp, vp]
-    L.ynb_verlet_kick.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp, vp, vp, vp]
+    L.ynb_verlet_kick.argtypes = [P(YnbBatch)] + [vp]*13
     L.ynb_total.argtypes = [P(YnbBatch), vp, vp, vp, vp]
     L.ynb_profile_enable.argtypes = [C.c_int]
     L.ynb_profile_read.argtypes = [C.c_int, P(C.c_char_p), P(C.c_float)]

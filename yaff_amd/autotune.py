@@ -19,11 +19,11 @@ from . import _lib
 __all__ = ['tune', 'cache_key']
 
 CANDIDATES = [
-    ('pair_lanes', (16, 32, 64), ('pair', 'pair_reduce')),
+    ('pair_lanes', (16, 32, 64), ('pair',)),
     ('apl', (2, 4, 8), ('ewald_sk', 'ewald_kreduce')),
     ('skb', None, ('ewald_sk', 'ewald_kreduce')),
-    ('fapl', (2, 4), ('ewald_force', 'ewald_gsum')),
-    ('fkb', None, ('ewald_force', 'ewald_gsum')),
+    ('fapl', (2, 4), ('ewald_force',)),
+    ('fkb', None, ('ewald_force',)),
 ]
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'yaff_amd', 'autotune.json')

@@ -153,6 +153,47 @@ __device__ inline void wave_status_max(int32_t *addr, int v) {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Partial results handed from one block to another within a kernel are
+// written with publish() and read with gather(): relaxed atomic store and
+// load at device scope, which go through the XCD's L2 to the device-coherent
+// level (the sc1 bit), so no L2 write-back or invalidate is needed to pass
+// them between XCDs.
+__device__ inline void publish(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double gather(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Completion of a group of nblocks blocks (the blocks of one system, say):
+// each block calls this, uniformly in the block, after publishing its
+// partial results; exactly one block -- the last to arrive -- gets true, and
+// it may then gather every partial of the group, reduce them in a fixed order
+// and write the group's result, so a separate reduction kernel is not needed.
+// The counter is reset by the last block, ready for the next launch (and for
+// graph replays).  Counters start at zero.
+//
+// Ordering: each wave waits until its publish() stores are acknowledged
+// (device-coherent), the block meets at a barrier, then one thread counts the
+// block in with a device-scope atomic; the last block's gathers are issued
+// after it has read the final count.  Data written with plain stores is not
+// covered (that would need a device-scope release fence, an L2 write-back
+// per block; __threadfence in every wave also invalidates the L2 the kernel
+// still reads from, which cost 30-50% of the fused kernels' time).
+__device__ inline bool last_block(int32_t *ctr, int nblocks) {
+  __shared__ int l_last;
+  __builtin_amdgcn_s_waitcnt(0);  // vmcnt = expcnt = lgkmcnt = 0
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int prev = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int last = (prev == nblocks - 1);
+    if (last) __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    l_last = last;
+  }
+  __syncthreads();
+  return l_last != 0;
+}
+
 // a * b + c as the three-operand v_fma_f64 with the addend c a scalar
 // register (c: a constant coefficient).  For an addend that stays live (a
 // polynomial coefficient held in a vector register across a loop) the

@@ -20,7 +20,10 @@
 //   coefficients are uniform and come through scalar loads; the k vector
 //   weighting is carried by two running sums per row; the 4 wavefronts of a
 //   block walk 4 row blocks of the system and are summed in LDS.
-// Pass 4 (k_ewald_gsum, k_ewald_sysreduce): fixed-order sums.
+// Pass 4: one extra block of the force launch per system sums its energy
+//   and virial from the k-block sums (ewald_system_reduce); the last force
+//   block of an atom block sums its partial gradients (ewald_gsum_block), or
+//   the caller leaves them to the Verlet kick.  Fixed-order sums.
 // No floating-point atomics: results do not depend on scheduling or on the
 // other systems of a batch.
 #include "common.h"
@@ -29,13 +32,15 @@ namespace ynb {
 namespace {
 
 __global__ void k_ewald_prep(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, const double *pos,
-                             const double *charges, double *frac, double *step) {
+                             const double *charges, double *frac, double *step, double *out) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (g >= npad) return;
   const int sid = wave_sys[g >> 6];
   double f[3] = {0.0, 0.0, 0.0}, q = 0.0, c = 1.0, s = 0.0;
   if (sid >= 0) {
     const ynb_sys &S = sys[sid];
+    if (g == S.atom0 && S.nk == 0)  // no k vectors: the reduction never writes this system's row
+      for (int i = 0; i < 10; ++i) out[10 * (int64_t)sid + i] = 0.0;
     if (g - S.atom0 < S.natom && S.nk > 0) {
       const double p0 = pos[3 * g], p1 = pos[3 * g + 1], p2 = pos[3 * g + 2];
       for (int k = 0; k < 3; ++k) {
@@ -215,6 +220,38 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
   }
 }
 
+// out[s] = {E, V (3x3)}; ewald.c:128-149 (virial diagonal -= E, then 1/eps_r),
+// from the system's k-block sums, by one extra block of the force launch
+// (256 threads): strided sums over the blocks, wavefront trees, then the 4
+// wavefronts in order.
+__device__ void ewald_system_reduce(const ynb_sys &S, int s, const double *kpart, double *out) {
+  const int t = threadIdx.x;
+  __shared__ double l_red[4][8];
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  const int b0 = S.k0 / 32, b1 = (S.k0 + S.nk) / 32;
+  for (int b = b0 + t; b < b1; b += 256)
+#pragma unroll
+    for (int c = 0; c < 7; ++c) acc[c] += kpart[8 * (int64_t)b + c];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) acc[c] = wave_sum(acc[c]);
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < 7; ++c) l_red[t >> 6][c] = acc[c];
+  __syncthreads();
+  if (t == 0) {
+    double r[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) r[c] = ((l_red[0][c] + l_red[1][c]) + l_red[2][c]) + l_red[3][c];
+    const double e = r[0];
+    const double f = 1.0 / S.dielectric;
+    double *o = out + 10 * (int64_t)s;
+    o[0] = e * f;
+    o[1] = (r[1] - e) * f; o[2] = r[4] * f; o[3] = r[5] * f;
+    o[4] = r[4] * f; o[5] = (r[2] - e) * f; o[6] = r[6] * f;
+    o[7] = r[5] * f; o[8] = r[6] * f; o[9] = (r[3] - e) * f;
+  }
+}
+
 // Block of 32 k vectors x 8 chunk parts: part p sums the chunks p, p+8, ...
 // of each group (4 independent accumulators), then part sums are added in a
 // fixed order; thread (k, part 0) forms the energy/virial terms of k and the
@@ -308,6 +345,35 @@ __global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_e
   if (t < 7) ew.kpart[8 * (int64_t)blockIdx.x + t] = red[t][0];
 }
 
+// Gradient of one atom block (na atoms from g0): the sum of the system's
+// nslot partial gradients, one thread per (atom, component), four
+// interleaved accumulators combined in a fixed order; written or, with
+// accumulate, added to gpos.  Run by the atom block's last force block.
+__device__ void ewald_gsum_block(const ynb_sys &S, int64_t g0, int na, const double *gpart, int64_t npad,
+                                 double *gpos, int accumulate) {
+  const int ns = S.nslot;
+  const int64_t stride = 3 * npad;
+  for (int t = threadIdx.x; t < 3 * na; t += blockDim.x) {
+    const int64_t e = 3 * g0 + t;  // gpart[(slot * npad + g) * 3 + c] = src[slot * 3 npad]
+    if (g0 + t / 3 - S.atom0 >= S.natom) continue;
+    const double *src = gpart + e;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int sl = 0;
+    for (; sl + 4 <= ns; sl += 4) {
+      a0 += gather(src + (int64_t)sl * stride);
+      a1 += gather(src + (int64_t)(sl + 1) * stride);
+      a2 += gather(src + (int64_t)(sl + 2) * stride);
+      a3 += gather(src + (int64_t)(sl + 3) * stride);
+    }
+    for (; sl < ns; ++sl) a0 += gather(src + (int64_t)sl * stride);
+    const double G = (a0 + a1) + (a2 + a3);
+    if (accumulate)
+      gpos[e] += G;
+    else
+      gpos[e] = G;
+  }
+}
+
 // Forces (ewald.c:104-111), no nlow/nhigh groups.  Block (atom block of 64 P
 // atoms, slot y) of 4 wavefronts: wavefront w walks row block 4 y + w of the
 // system with P atoms per lane (atoms g + 64 p).
@@ -330,13 +396,20 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
                                                      const int32_t *__restrict__ krow_f,
                                                      const int32_t *__restrict__ fblk,
                                                      const double *__restrict__ frac, const double *__restrict__ step,
-                                                     double *__restrict__ gpart, int64_t npad) {
+                                                     double *__restrict__ gpart, int64_t npad, int32_t *fctr,
+                                                     double *gpos, int accumulate, const double *__restrict__ kpart,
+    double *__restrict__ out) {
 #pragma clang fp contract(fast)
   constexpr int NA = 64 * P;
   constexpr double TWO_PI = 6.283185307179586477;
   __shared__ double l_g[3][4][NA];
   const int wv = uniform_int(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int64_t blk = blockIdx.x;
+  if (blk >= npad / NA) {  // one block per system: energy and virial from the k-block sums
+    const int s = (int)(blk - npad / NA);
+    if (blockIdx.y == 0 && sys[s].nk > 0) ewald_system_reduce(sys[s], s, kpart, out);
+    return;
+  }
   const int sid = wave_sys[P * blk];
   if (sid < 0) return;
   const ynb_sys &S = sys[sid];
@@ -419,8 +492,9 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
     const int64_t g = blk * NA + a;
     const double v = ((l_g[c][0][a] + l_g[c][1][a]) + l_g[c][2][a]) + l_g[c][3][a];
     const double sc = (g - S.atom0 < S.natom) ? frac[3 * npad + g] / S.dielectric : 0.0;
-    gpart[((int64_t)blockIdx.y * npad + blk * NA) * 3 + t] = v * sc;
+    publish(gpart + ((int64_t)blockIdx.y * npad + blk * NA) * 3 + t, v * sc);
   }
+  if (gpos && last_block(fctr + blk, S.nslot)) ewald_gsum_block(S, blk * NA, NA, gpart, npad, gpos, accumulate);
 }
 
 // Forces with the nlow/nhigh groups (ewald.c:104-111 with the per-atom
@@ -436,12 +510,19 @@ __global__ __launch_bounds__(256) void k_ewald_force_groups(const ynb_sys *__res
                                                             const int32_t *__restrict__ fblk,
                                                             const double *__restrict__ frac,
                                                             const double *__restrict__ step,
-                                                            double *__restrict__ gpart, int64_t npad) {
+                                                            double *__restrict__ gpart, int64_t npad,
+                                                            int32_t *fctr, double *gpos, int accumulate, const double *__restrict__ kpart,
+    double *__restrict__ out) {
 #pragma clang fp contract(fast)
   constexpr int NA = 64 * P;
   __shared__ double l_g[3][4][NA];
   const int wv = uniform_int(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int64_t blk = blockIdx.x;
+  if (blk >= npad / NA) {  // one block per system: energy and virial from the k-block sums
+    const int s = (int)(blk - npad / NA);
+    if (blockIdx.y == 0 && sys[s].nk > 0) ewald_system_reduce(sys[s], s, kpart, out);
+    return;
+  }
   const int sid = wave_sys[P * blk];
   if (sid < 0) return;
   const ynb_sys &S = sys[sid];
@@ -501,71 +582,9 @@ __global__ __launch_bounds__(256) void k_ewald_force_groups(const ynb_sys *__res
     const int64_t g = blk * NA + a;
     const double v = ((l_g[c][0][a] + l_g[c][1][a]) + l_g[c][2][a]) + l_g[c][3][a];
     const double sc = (g - S.atom0 < S.natom) ? frac[3 * npad + g] / S.dielectric : 0.0;
-    gpart[((int64_t)blockIdx.y * npad + blk * NA) * 3 + t] = v * sc;
+    publish(gpart + ((int64_t)blockIdx.y * npad + blk * NA) * 3 + t, v * sc);
   }
-}
-
-// Gradient: sum of the system's nslot partial gradients, one thread per
-// (atom, component), four interleaved accumulators combined in a fixed order.
-__global__ void k_ewald_gsum(const ynb_sys *sys, const int32_t *wave_sys, double *gpart_, int64_t npad, double *gpos,
-                             int accumulate) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= 3 * npad) return;
-  const int64_t g = t / 3;
-  const int sid = wave_sys[g >> 6];
-  if (sid < 0) return;
-  const ynb_sys &S = sys[sid];
-  if (S.nk == 0 || g - S.atom0 >= S.natom) return;
-  const int ns = S.nslot;
-  const double *src = gpart_ + t;  // gpart[(slot * npad + g) * 3 + c] = src[slot * 3 * npad]
-  const int64_t stride = 3 * npad;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int sl = 0;
-  for (; sl + 4 <= ns; sl += 4) {
-    a0 += src[(int64_t)sl * stride];
-    a1 += src[(int64_t)(sl + 1) * stride];
-    a2 += src[(int64_t)(sl + 2) * stride];
-    a3 += src[(int64_t)(sl + 3) * stride];
-  }
-  for (; sl < ns; ++sl) a0 += src[(int64_t)sl * stride];
-  const double G = (a0 + a1) + (a2 + a3);
-  if (accumulate)
-    gpos[t] += G;
-  else
-    gpos[t] = G;
-}
-
-// out[s] = {E, V (3x3)}; ewald.c:128-149 (virial diagonal -= E, then 1/eps_r).
-// One block per system: 256 threads take the k blocks t, t + 256, ... and a
-// fixed LDS tree adds them.
-__global__ __launch_bounds__(256) void k_ewald_sysreduce(const ynb_sys *sys, int nsys, ynb_ewald ew, double *out) {
-  const int s = blockIdx.x;
-  const int t = threadIdx.x;
-  const ynb_sys &S = sys[s];
-  __shared__ double red[7][256];
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  const int b0 = S.k0 / 32, b1 = (S.k0 + S.nk) / 32;
-  for (int b = b0 + t; b < b1; b += 256)
-#pragma unroll
-    for (int c = 0; c < 7; ++c) acc[c] += ew.kpart[8 * (int64_t)b + c];
-#pragma unroll
-  for (int c = 0; c < 7; ++c) red[c][t] = acc[c];
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (t < off)
-#pragma unroll
-      for (int c = 0; c < 7; ++c) red[c][t] += red[c][t + off];
-    __syncthreads();
-  }
-  if (t == 0) {
-    const double e = red[0][0];
-    const double f = (S.nk > 0) ? 1.0 / S.dielectric : 0.0;
-    double *o = out + 10 * (int64_t)s;
-    o[0] = e * f;
-    o[1] = (red[1][0] - e) * f; o[2] = red[4][0] * f; o[3] = red[5][0] * f;
-    o[4] = red[4][0] * f; o[5] = (red[2][0] - e) * f; o[6] = red[6][0] * f;
-    o[7] = red[5][0] * f; o[8] = red[6][0] * f; o[9] = (red[3][0] - e) * f;
-  }
+  if (gpos && last_block(fctr + blk, S.nslot)) ewald_gsum_block(S, blk * NA, NA, gpart, npad, gpos, accumulate);
 }
 
 // ewald.c:278-349 for one system, rows processed in order by lane 0 of one
@@ -626,7 +645,8 @@ using namespace ynb;
 
 extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const double *pos, const double *charges,
                               int32_t precision, double *gpos, int32_t accumulate, double *out, ynb_stream stream) {
-  if (!b || !ew || !pos || !charges || !out || b->npad % 256 != 0 || ew->nk_tot % 256 != 0 || ew->fslots < 0 ||
+  if (!b || !ew || !pos || !charges || !out || !ew->fctr || b->npad % 256 != 0 || ew->nk_tot % 256 != 0 ||
+      ew->fslots < 0 ||
       (ew->apl != 2 && ew->apl != 4 && ew->apl != 8) || (ew->fapl != 2 && ew->fapl != 4) ||
       (precision != YNB_FP64 && precision != YNB_MIXED)) {
     set_error("ynb_ewald_reci: invalid arguments");
@@ -635,7 +655,7 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
   hipStream_t st = (hipStream_t)stream;
   const int64_t npad = b->npad;
   YNB_LAUNCH("ewald_prep", k_ewald_prep, dim3((unsigned)ceil_div(npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys, npad,
-                     pos, charges, ew->frac, ew->step);
+             pos, charges, ew->frac, ew->step, out);
   YNB_LAUNCHED();
   if (ew->nwork > 0) {
     const dim3 grid((unsigned)(ew->nwork < (1 << 30) ? ew->nwork : (1 << 30)));
@@ -660,13 +680,16 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
   }
   if (ew->fslots > 0) {
     const int P = ew->fapl;
-    const dim3 fgrid((unsigned)(npad / (64 * P)), (unsigned)ew->fslots);
+    // npad / (64 P) atom blocks, then one block per system for its energy and virial
+    const dim3 fgrid((unsigned)(npad / (64 * P) + b->nsys), (unsigned)ew->fslots);
 #define YNB_FORCE(PR, PP)                                                                                     \
   YNB_LAUNCH("ewald_force", (k_ewald_force<PR, PP>), fgrid, dim3(256), 0, st, b->sys, b->wave_sys, ew->frec,    \
-             ew->krow, ew->krow_f, ew->fblk, ew->frac, ew->step, ew->gpart, npad)
+             ew->krow, ew->krow_f, ew->fblk, ew->frac, ew->step, ew->gpart, npad, ew->fctr, gpos, accumulate,      \
+             ew->kpart, out)
 #define YNB_FORCE_G(PR, PP)                                                                                   \
   YNB_LAUNCH("ewald_force", (k_ewald_force_groups<PR, PP>), fgrid, dim3(256), 0, st, b->sys, b->wave_sys,       \
-             ew->kvec, ew->coef, ew->krow, ew->krow_k, ew->fblk, ew->frac, ew->step, ew->gpart, npad)
+             ew->kvec, ew->coef, ew->krow, ew->krow_k, ew->fblk, ew->frac, ew->step, ew->gpart, npad, ew->fctr, gpos,    \
+             accumulate, ew->kpart, out)
 #define YNB_FORCE_P(PR)                                            \
   if (ew->groups) {                                                \
     if (P == 2) YNB_FORCE_G(PR, 2); else YNB_FORCE_G(PR, 4);       \
@@ -683,14 +706,6 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
 #undef YNB_FORCE
     YNB_LAUNCHED();
   }
-  if (gpos) {
-    YNB_LAUNCH("ewald_gsum", k_ewald_gsum, dim3((unsigned)ceil_div(3 * npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys,
-                       ew->gpart, npad, gpos, accumulate);
-    YNB_LAUNCHED();
-  }
-  YNB_LAUNCH("ewald_sysreduce", k_ewald_sysreduce, dim3((unsigned)b->nsys), dim3(256), 0, st, b->sys, b->nsys, *ew,
-                     out);
-  YNB_LAUNCHED();
   return YNB_OK;
 }
 

@@ -62,58 +62,86 @@ __global__ void k_kick_drift(const ynb_sys *sys, const int32_t *wave_sys, int64_
   }
 }
 
-// vel += dt/2 acc (verlet.py:354-355) and per-wave sums of m v^2.
-__global__ void k_kick(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, double *vel, const double *gpos,
-                       const double *minv, const double *masses, const double *dt, double *partial) {
-  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= npad) return;  // npad is a multiple of 64: whole waves leave
-  const int sid = uniform_int(wave_sys[g >> 6]);
-  if (sid < 0) return;
-  const ynb_sys &S = sys[sid];
-  double e2 = 0.0;
-  if (g - S.atom0 < S.natom) {
-    const double h = dt[sid];
-    const double mi = minv[g];
-    for (int c = 0; c < 3; ++c) {
-      const double a = -gpos[3 * g + c] * mi;
-      const double v = vel[3 * g + c] + 0.5 * a * h;
-      vel[3 * g + c] = v;
-      e2 += masses[g] * v * v;
-    }
-  }
-  e2 = wave_sum(e2);
-  if ((threadIdx.x & 63) == 0) partial[g >> 6] = e2;
+// Term c of system s's total: its potentials' pair terms in order, then the
+// Ewald reciprocal term.
+__device__ inline void system_total(const ynb_sys &S, int s, int c, const double *pair_out, const double *ewald_out,
+                                    double *total) {
+  double acc = 0.0;
+  if (pair_out)
+    for (int p = 0; p < S.npot; ++p) acc += pair_out[((int64_t)s * YNB_MAX_POT + p) * 10 + c];
+  if (ewald_out) acc += ewald_out[(int64_t)s * 10 + c];
+  total[(int64_t)s * 10 + c] = acc;
 }
 
-// One block per system: fixed strided sums over the system's waves, then a
-// fixed LDS tree.
-__global__ __launch_bounds__(256) void k_wave_to_sys(const ynb_sys *sys, const double *partial, double *out) {
-  const ynb_sys &S = sys[blockIdx.x];
+// vel += dt/2 acc (verlet.py:354-355) and 2 Ekin of each system.  Blocks of
+// 64 atoms x 3 components (one thread per component; systems start at
+// multiples of 64 atoms, so a block belongs to the system of its wave entry);
+// each block's sum of m v^2 goes to `partial` and the system's last block
+// adds them in a fixed order.  With gpart, the gradient is gpos + the
+// system's nslot Ewald partial gradients (slots of 3 npad doubles, eight
+// loads in flight, four interleaved sums in a fixed order) and is written
+// back to gpos.  With total, the last block also totals the system's energy
+// and virial terms (system_total).
+__global__ __launch_bounds__(192) void k_kick(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
+                                              double *vel, double *gpos, const double *gpart,
+                                              const double *minv, const double *masses, const double *dt,
+                                              double *partial, int32_t *ctr, double *ekin2,
+                                              const double *pair_out, const double *ewald_out, double *total) {
+  const int64_t blk = blockIdx.x;
+  const int sid = wave_sys[blk];
+  if (sid < 0) return;  // the same for the whole block
+  const ynb_sys &S = sys[sid];
   const int t = threadIdx.x;
-  __shared__ double red[256];
-  double acc = 0.0;
-  for (int64_t w = S.atom0 / 64 + t; w < (S.atom0 + S.natom + 63) / 64; w += 256) acc += partial[w];
-  red[t] = acc;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (t < off) red[t] += red[t + off];
-    __syncthreads();
+  const int64_t g = blk * 64 + t / 3;
+  const int64_t e = blk * 192 + t;  // = 3 g + c
+  double e2 = 0.0;
+  if (g - S.atom0 < S.natom) {
+    double gr = gpos[e];
+    const int ns = S.nslot;
+    if (gpart && ns > 0) {
+      const double *src = gpart + e;
+      const int64_t stride = 3 * npad;
+      double a[4] = {0.0, 0.0, 0.0, 0.0};
+      int sl = 0;
+      for (; sl + 8 <= ns; sl += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = src[(int64_t)(sl + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u & 3] += v[u];
+      }
+      for (; sl < ns; ++sl) a[sl & 3] += src[(int64_t)sl * stride];
+      gr += (a[0] + a[1]) + (a[2] + a[3]);
+      gpos[e] = gr;
+    }
+    const double v = vel[e] + 0.5 * (-gr * minv[g]) * dt[sid];
+    vel[e] = v;
+    e2 = masses[g] * v * v;
   }
-  if (t == 0) out[blockIdx.x] = red[0];
+  __shared__ double l_w[3];
+  e2 = wave_sum(e2);
+  if ((t & 63) == 0) l_w[t >> 6] = e2;
+  __syncthreads();
+  if (t == 0) publish(partial + blk, (l_w[0] + l_w[1]) + l_w[2]);
+  const int nb = S.natom > 0 ? (S.natom + 63) / 64 : 1;
+  if (last_block(ctr + sid, nb)) {
+    const int64_t b0 = S.atom0 / 64;
+    double acc = 0.0;
+    for (int w = t; w < nb; w += 192) acc += gather(partial + b0 + w);
+    acc = wave_sum(acc);
+    __syncthreads();
+    if ((t & 63) == 0) l_w[t >> 6] = acc;
+    __syncthreads();
+    if (t == 0) ekin2[sid] = (l_w[0] + l_w[1]) + l_w[2];
+    if (total && t < 10) system_total(S, sid, t, pair_out, ewald_out, total);
+  }
 }
 
 __global__ void k_total(const ynb_sys *sys, int nsys, const double *pair_out, const double *ewald_out,
                         double *total) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsys) return;
-  const ynb_sys &S = sys[s];
-  for (int c = 0; c < 10; ++c) {
-    double acc = 0.0;
-    if (pair_out)
-      for (int p = 0; p < S.npot; ++p) acc += pair_out[((int64_t)s * YNB_MAX_POT + p) * 10 + c];
-    if (ewald_out) acc += ewald_out[(int64_t)s * 10 + c];
-    total[(int64_t)s * 10 + c] = acc;
-  }
+  for (int c = 0; c < 10; ++c) system_total(sys[s], s, c, pair_out, ewald_out, total);
 }
 
 }  // namespace
@@ -162,17 +190,16 @@ int ynb_verlet_kick_drift(const ynb_batch *b, double *pos, double *vel, const do
   return YNB_OK;
 }
 
-int ynb_verlet_kick(const ynb_batch *b, double *vel, const double *gpos, const double *minv, const double *masses,
-                    const double *dt, double *partial, double *ekin2, ynb_stream stream) {
-  if (!b || !vel || !gpos || !minv || !masses || !dt || !partial || !ekin2 || b->npad % 64 != 0) {
+int ynb_verlet_kick(const ynb_batch *b, double *vel, double *gpos, const double *gpart, const double *minv,
+                    const double *masses, const double *dt, double *partial, int32_t *ctr, double *ekin2,
+                    const double *pair_out, const double *ewald_out, double *total, ynb_stream stream) {
+  if (!b || !vel || !gpos || !minv || !masses || !dt || !partial || !ctr || !ekin2 || b->npad % 256 != 0) {
     set_error("ynb_verlet_kick: invalid arguments");
     return YNB_EINVAL;
   }
   hipStream_t st = (hipStream_t)stream;
-  YNB_LAUNCH("verlet_kick", k_kick, dim3((unsigned)ceil_div(b->npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys,
-             b->npad, vel, gpos, minv, masses, dt, partial);
-  YNB_LAUNCHED();
-  YNB_LAUNCH("verlet_ekin", k_wave_to_sys, dim3((unsigned)b->nsys), dim3(256), 0, st, b->sys, partial, ekin2);
+  YNB_LAUNCH("verlet_kick", k_kick, dim3((unsigned)(b->npad / 64)), dim3(192), 0, st, b->sys, b->wave_sys, b->npad,
+             vel, gpos, gpart, minv, masses, dt, partial, ctr, ekin2, pair_out, ewald_out, total);
   YNB_LAUNCHED();
   return YNB_OK;
 }

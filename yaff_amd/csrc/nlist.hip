@@ -94,13 +94,15 @@ __global__ void k_bin(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
       atomicOr(status, 4);
     fill = slot + 1;
   }
-  (void)fill;
+  // the fullest bin, recorded only on overflow (the caller sizes the retry
+  // from it; bin counts are cleared by k_bin_order)
+  wave_status_max(status + 1, fill > cb ? fill : 0);
 }
 
 // One block per bin: orders the bin's atoms by index (rank among the bin's
 // atoms) and writes their wrapped data to the bin's slots, so the layout
 // does not depend on the order of the atomics in k_bin.
-__global__ __launch_bounds__(256) void k_bin_order(int32_t cb, const int32_t *bin_count, const int32_t *slot_idx,
+__global__ __launch_bounds__(256) void k_bin_order(int32_t cb, int32_t *bin_count, const int32_t *slot_idx,
                                                    const double4 *xw, const int4 *wr, double4 *sxw, int4 *sw,
                                                    int32_t *bin_start, int32_t *bin_end, const int32_t *bin_sys,
                                                    const int32_t *rebuild) {
@@ -121,6 +123,7 @@ __global__ __launch_bounds__(256) void k_bin_order(int32_t cb, const int32_t *bi
   if (threadIdx.x == 0) {
     bin_start[b] = b * cb;
     bin_end[b] = b * cb + n;
+    bin_count[b] = 0;  // every thread has read the count (barrier above): ready for the next build
   }
 }
 
@@ -707,8 +710,8 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
   const int tb = 256;
   const int nb = (int)ceil_div(npad, tb);
   if (b->nbin > 0) {
-    YNB_LAUNCH("nlist_clear", k_fill_i32, dim3((int)ceil_div(b->nbin, tb)), dim3(tb), 0, st, w.bin_count,
-                       (int64_t)b->nbin, 0);
+    // bin counts are zero here: the caller zero-initialises the workspace
+    // and k_bin_order resets the counts after each build
     YNB_LAUNCH("nlist_bin", k_bin, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, cb, w.bin_count,
                w.slot_idx, w.xw, w.wr, nl->status, nl->rebuild);
     YNB_LAUNCH("nlist_bin_order", k_bin_order, dim3(b->nbin), dim3(256), 0, st, cb, w.bin_count, w.slot_idx, w.xw,
@@ -751,17 +754,6 @@ int ynb_nlist_skin_check(const ynb_batch *b, const double *pos, double *pos_ref,
   YNB_LAUNCH("nlist_skin_check", k_skin_check, dim3((unsigned)b->nsys), dim3(256), 0, (hipStream_t)stream, b->sys,
              pos, pos_ref, skin, rebuild);
   YNB_LAUNCHED();
-  return YNB_OK;
-}
-
-int ynb_nlist_bin_counts(const ynb_batch *b, const void *work, int32_t *counts, ynb_stream stream) {
-  if (!b || !work || !counts) {
-    set_error("ynb_nlist_bin_counts: invalid arguments");
-    return YNB_EINVAL;
-  }
-  if (b->nbin == 0) return YNB_OK;
-  Work w = carve(const_cast<void *>(work), b->npad, b->nbin, b->max_bin_atoms);
-  YNB_HIP(hipMemcpyAsync(counts, w.bin_count, 4 * (size_t)b->nbin, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return YNB_OK;
 }
 

@@ -4,10 +4,10 @@
 //
 // One lane per atom i walks its ELL entries; both directions of a pair are in
 // the list, so the lane accumulates its own gradient in registers without
-// atomics and energy/virial get a factor 1/2.  Per-wave sums of energy and
-// virial go to `partial`; k_pair_reduce adds the waves of each system in a
-// fixed order, so results are deterministic and independent of the other
-// systems of the batch.
+// atomics and energy/virial get a factor 1/2.  Per-block sums of energy and
+// virial go to `partial`; the last block of each system to finish adds the
+// system's block sums in a fixed order (pair_system_reduce), so results are
+// deterministic and independent of the other systems of the batch.
 //
 // FP64 mode rebuilds every row vector exactly as nlist_build_low does
 // (yaff_row_vector), so d, the cutoff test d < rcut and the scaling lookup on
@@ -247,11 +247,64 @@ __device__ inline bool form_applies(const ynb_sys &S, uint32_t mask) {
 // the potentials of a pair are summed first, so that only totals are
 // accumulated (one gradient/virial update per pair); the totals go to
 // potential slot 0 and the other slots get zeros.
+// The system's block partials summed by its last block (256 threads): nval/8
+// value groups (nval = 8: totals in slot 0; nval = 8 npot: per-potential
+// slots) x 256/nval strided parts, four interleaved accumulators, then a
+// fixed LDS tree over the parts.  out[s][p] = {E, V00 V01 V02 V10 V11 V12 V20
+// V21 V22} with the 1/2 of the full list applied; slots past nval/8 get
+// zeros.  apb: atoms per block of the pair kernel.
+__device__ void pair_system_reduce(const ynb_sys &S, int s, const double *partial, double *out, int apb, int nval) {
+  __shared__ double l_red[256];
+  const int t = threadIdx.x;
+  const int nval_p2 = nval <= 8 ? 8 : (nval <= 16 ? 16 : 32);
+  const int nparts = 256 / nval_p2;
+  const int q = t % nval_p2, part = t / nval_p2;
+  const int64_t b0 = S.atom0 / apb, b1 = (S.atom0 + S.natom + apb - 1) / apb;
+  double a[4] = {0, 0, 0, 0};
+  if (q < nval) {
+    // 16 loads in flight per thread (clamped addresses, masked adds): the
+    // whole system's partials arrive in one or two memory round trips
+    for (int64_t b = b0 + part; b < b1; b += 16 * nparts) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int64_t bu = min(b + (int64_t)nparts * u, b1 - 1);
+        v[u] = gather(partial + bu * (YNB_MAX_POT * 8) + q);
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (b + (int64_t)nparts * u < b1) a[u & 3] += v[u];
+    }
+  }
+  l_red[t] = (a[0] + a[1]) + (a[2] + a[3]);
+  __syncthreads();
+  for (int off = nparts / 2; off > 0; off >>= 1) {
+    if (part < off) l_red[t] += l_red[t + off * nval_p2];
+    __syncthreads();
+  }
+  if (t < S.npot) {
+    double r[8];
+    for (int c = 0; c < 8; ++c) r[c] = (t * 8 < nval) ? 0.5 * l_red[t * 8 + c] : 0.0;
+    double *o = out + ((int64_t)s * YNB_MAX_POT + t) * 10;
+    o[0] = r[0];
+    o[1] = r[1]; o[2] = r[4]; o[3] = r[5];
+    o[4] = r[4]; o[5] = r[2]; o[6] = r[6];
+    o[7] = r[5]; o[8] = r[6]; o[9] = r[3];
+  }
+}
+
+// Blocks of a system in the pair kernels: the system's waves (at least one)
+// in blocks of apb atoms.
+__device__ inline int pair_system_blocks(const ynb_sys &S, int apb) {
+  const int n = S.natom > 0 ? S.natom : 1;
+  return ((n + 63) / 64) * 64 / apb;
+}
+
 template <int PREC, bool SLOTS, int G>
 __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
                                               ynb_pairpar pp, const double *pos, ynb_nlist nl, uint32_t mask,
                                               double *gpos, int accumulate, double *gpos_slots,
-                                              double *partial) {
+                                              double *partial, double *out) {
   using PE = PotEval<PREC>;
   using PF = PotFast<PREC>;
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -406,13 +459,16 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
     }
   }
   // Block sums (the block's atoms belong to one system; lanes past natom
-  // contribute zeros): the 4 waves of the block in order.
+  // contribute zeros): the 4 waves of the block in order; the system's last
+  // block adds the system's block sums.
   __syncthreads();
   if (threadIdx.x < npot * 8) {
     const int q = threadIdx.x;
-    partial[(int64_t)blockIdx.x * (YNB_MAX_POT * 8) + q] =
-        (q < ngroup * 8) ? ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q] : 0.0;
+    publish(partial + (int64_t)blockIdx.x * (YNB_MAX_POT * 8) + q,
+            (q < ngroup * 8) ? ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q] : 0.0);
   }
+  if (last_block(pp.ctr + sid, pair_system_blocks(S, 256 / G)))
+    pair_system_reduce(S, sid, partial, out, 256 / G, SLOTS ? 8 * npot : 8);
 }
 
 
@@ -426,7 +482,7 @@ __global__ __launch_bounds__(256) void k_pair(const ynb_sys *sys, const int32_t 
 template <int PREC, int FORM, int G>
 __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
                                                    ynb_pairpar pp, const double *pos, ynb_nlist nl, uint32_t mask,
-                                                   double *gpos, int accumulate, double *partial) {
+                                                   double *gpos, int accumulate, double *partial, double *out) {
   using PE = PotEval<PREC>;
   using PF = PotFast<PREC>;
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -606,53 +662,10 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
   __syncthreads();
   if (threadIdx.x < npot * 8) {
     const int q = threadIdx.x;
-    partial[(int64_t)blockIdx.x * (YNB_MAX_POT * 8) + q] =
-        (q < 8) ? ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q] : 0.0;
+    publish(partial + (int64_t)blockIdx.x * (YNB_MAX_POT * 8) + q,
+            (q < 8) ? ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q] : 0.0);
   }
-}
-
-// One block per system: the block partials of the system's atoms are summed
-// by nval/8 value groups x (1024 / nval) strided parts (nval = 8: totals in
-// slot 0; nval = 8 npot: per-potential slots), then a fixed LDS tree adds
-// the parts.  out[s][p] = {E, V00 V01 V02 V10 V11 V12 V20 V21 V22} with the
-// 1/2 of the full list applied; slots past nval/8 are written as zeros.
-__global__ __launch_bounds__(1024) void k_pair_reduce(const ynb_sys *sys, const double *partial, double *out,
-                                                      int apb, int totals) {
-  const int s = blockIdx.x;
-  const ynb_sys &S = sys[s];
-  const int t = threadIdx.x;
-  const int nval = totals ? 8 : 8 * S.npot;
-  const int nval_p2 = nval <= 8 ? 8 : (nval <= 16 ? 16 : 32);
-  const int nparts = 1024 / nval_p2;
-  const int q = t % nval_p2, part = t / nval_p2;
-  // blocks of apb atoms covering the system (atom0 is a multiple of 128)
-  const int64_t b0 = S.atom0 / apb, b1 = (S.atom0 + S.natom + apb - 1) / apb;
-  __shared__ double red[1024];
-  double a[4] = {0, 0, 0, 0};
-  if (q < nval) {
-    for (int64_t b = b0 + part; b < b1; b += 4 * nparts) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int64_t bu = b + (int64_t)nparts * u;
-        if (bu < b1) a[u] += partial[bu * (YNB_MAX_POT * 8) + q];
-      }
-    }
-  }
-  red[t] = (a[0] + a[1]) + (a[2] + a[3]);
-  __syncthreads();
-  for (int off = nparts / 2; off > 0; off >>= 1) {
-    if (part < off) red[t] += red[t + off * nval_p2];
-    __syncthreads();
-  }
-  if (t < S.npot) {
-    double r[8];
-    for (int c = 0; c < 8; ++c) r[c] = (t * 8 < nval) ? 0.5 * red[t * 8 + c] : 0.0;
-    double *o = out + ((int64_t)s * YNB_MAX_POT + t) * 10;
-    o[0] = r[0];
-    o[1] = r[1]; o[2] = r[4]; o[3] = r[5];
-    o[4] = r[4]; o[5] = r[2]; o[6] = r[6];
-    o[7] = r[5]; o[8] = r[6]; o[9] = r[3];
-  }
+  if (last_block(pp.ctr + sid, pair_system_blocks(S, 256 / G))) pair_system_reduce(S, sid, partial, out, 256 / G, 8);
 }
 
 // pair_pot_compute over explicit Yaff rows (yaff/pes/pair_pot.c:84-176):
@@ -754,7 +767,7 @@ using namespace ynb;
 extern "C" int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const double *pos, const ynb_nlist *nl,
                                 uint32_t slot_mask, int32_t precision, double *gpos, int32_t accumulate,
                                 double *gpos_slots, double *partial, double *out, ynb_stream stream) {
-  if (!b || !pp || !nl || !pos || !partial || !out || b->npad % 64 != 0 ||
+  if (!b || !pp || !pp->ctr || !nl || !pos || !partial || !out || b->npad % 64 != 0 ||
       (precision != YNB_FP64 && precision != YNB_MIXED)) {
     set_error("ynb_pair_compute: invalid arguments");
     return YNB_EINVAL;
@@ -769,7 +782,7 @@ extern "C" int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const
   const bool slots = gpos_slots != nullptr;
 #define YNB_PAIR(P, SL, GG)                                                                                     \
   YNB_LAUNCH("pair", (k_pair<P, SL, GG>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,      \
-             slot_mask, gpos, accumulate, gpos_slots, partial)
+             slot_mask, gpos, accumulate, gpos_slots, partial, out)
 #define YNB_PAIR_G(P, SL)        \
   if (G == 16) YNB_PAIR(P, SL, 16); \
   else if (G == 32) YNB_PAIR(P, SL, 32); \
@@ -782,7 +795,7 @@ extern "C" int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const
   if (!slots) {
 #define YNB_FORMK(P, F, GG)                                                                                    \
   YNB_LAUNCH("pair", (k_pair_form<P, F, GG>), grid, block, 0, st, b->sys, b->wave_sys, b->npad, *pp, pos, *nl,  \
-             slot_mask, gpos, accumulate, partial)
+             slot_mask, gpos, accumulate, partial, out)
 #define YNB_FORM_G(P, F)                 \
   if (G == 16) YNB_FORMK(P, F, 16);      \
   else if (G == 32) YNB_FORMK(P, F, 32); \
@@ -798,9 +811,6 @@ extern "C" int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const
   }
 #undef YNB_PAIR_G
 #undef YNB_PAIR
-  YNB_LAUNCHED();
-  YNB_LAUNCH("pair_reduce", k_pair_reduce, dim3(b->nsys), dim3(1024), 0, st, b->sys, partial, out, 256 / G,
-             slots ? 0 : 1);
   YNB_LAUNCHED();
   return YNB_OK;
 }

@@ -435,8 +435,11 @@ class Engine(object):
         self.cross_d = self._t(self.cross_h)
         self.stab_d = self._t(np.frombuffer(self.stab_h.tobytes(), np.uint8))
         self.rec_d = self._t(self.rec_h)
+        # block counters of the last-block reductions (zero; kernels leave them at zero):
+        # [nsys] pair, [nsys] unused, [nsys] kick, [npad/128] Ewald force blocks
+        self.ctr_d = torch.zeros(3*self.nsys + self.npad//128 + 1, dtype=torch.int32, device=dev)
         self.pairpar = YnbPairpar(self.par_d.data_ptr(), self.cross_d.data_ptr(), self.stab_d.data_ptr(),
-                                  self.rec_d.data_ptr(), self.pair_lanes, 0)
+                                  self.rec_d.data_ptr(), self._ctr(0), self.pair_lanes, 0)
         # neighbour list
         self._alloc_nlist_work()
         self.cnt_d = torch.zeros(self.npad, dtype=torch.int32, device=dev)
@@ -523,9 +526,14 @@ class Engine(object):
                 m = max(m, int(2*s.natom/self.hsys[i].nbin) + 16)
         return min(_round_up(m, 16), 1024)
 
+    def _ctr(self, off):
+        """Address of counter `off` of ctr_d."""
+        return self.ctr_d.data_ptr() + 4*off
+
     def _alloc_nlist_work(self):
         self.nl_work_bytes = self.lib.ynb_nlist_workspace_bytes(self.npad, self.nbin, self.bin_cap)
-        self.nl_work_d = torch.empty(self.nl_work_bytes, dtype=torch.uint8, device=self.device)
+        # zeroed once: the build keeps its bin counters at zero between builds
+        self.nl_work_d = torch.zeros(self.nl_work_bytes, dtype=torch.uint8, device=self.device)
 
     def _alloc_ewald(self):
         dev = self.device
@@ -542,7 +550,7 @@ class Engine(object):
         self.ewald = YnbEwald(t['kvec'].data_ptr(), t['krow'].data_ptr(), t['krow_k'].data_ptr(),
                               t['skblk'].data_ptr(), t['fblk'].data_ptr(), t['chunk'].data_ptr(),
                               t['work'].data_ptr(), t['kblk_sys'].data_ptr(), t['kpos'].data_ptr(),
-                              t['krow_f'].data_ptr(), self.nwork,
+                              t['krow_f'].data_ptr(), self._ctr(3*self.nsys), self.nwork,
                               self.nk_tot, self.apl, self.fslots, self.groups, self.fapl, self.ew_frac_d.data_ptr(),
                               self.ew_step_d.data_ptr(), self.ew_spart_d.data_ptr(), self.ew_coef_d.data_ptr(),
                               self.ew_frec_d.data_ptr(), self.ew_kpart_d.data_ptr(), self.ew_gpart_d.data_ptr())
@@ -633,16 +641,13 @@ class Engine(object):
         image shift exceeded 511 cells, 4 a bin exceeded its capacity.  The
         longest row is read only after a flag."""
         flags = int(self.status_d[0].item())
-        if flags:
-            self.status_d.zero_()
         longest = 0
         if flags & 1:
             longest = int(self.cnt_d.max().item())
         if flags & 4:
-            counts = torch.zeros(max(self.nbin, 1), dtype=torch.int32, device=self.device)
-            check(self.lib.ynb_nlist_bin_counts(self.batch, self.nl_work_d.data_ptr(), counts.data_ptr(),
-                                                self.stream()))
-            self._fullest_bin = int(counts.max().item())
+            self._fullest_bin = int(self.status_d[1].item())
+        if flags:
+            self.status_d.zero_()
         return flags, longest
 
     def grow(self, flags, longest):
@@ -720,10 +725,19 @@ class Engine(object):
         check(self.lib.ynb_verlet_kick_drift(self.batch, self.pos_d.data_ptr(), self.vel_d.data_ptr(),
                                              self.gpos_d.data_ptr(), self.minv_d.data_ptr(), self.dt_d.data_ptr(),
                                              st))
-        self.compute(build=True)
+        # forces: the Ewald partial gradients stay in their slots and are
+        # summed by the kick, which leaves the total gradient in gpos_d
+        self.build_nlist(force=False)
+        self.pair(gpos=True, accumulate=False)
+        self.ewald_reci(gpos=False)
+        # ... and the kick also totals energy and virial (in place of total())
         check(self.lib.ynb_verlet_kick(self.batch, self.vel_d.data_ptr(), self.gpos_d.data_ptr(),
+                                       self.ew_gpart_d.data_ptr() if self.has_ewald else None,
                                        self.minv_d.data_ptr(), self.masses_d.data_ptr(), self.dt_d.data_ptr(),
-                                       self.md_partial_d.data_ptr(), self.ekin2_d.data_ptr(), st))
+                                       self.md_partial_d.data_ptr(), self._ctr(2*self.nsys), self.ekin2_d.data_ptr(),
+                                       self.pair_out_d.data_ptr(),
+                                       self.ewald_out_d.data_ptr() if self.has_ewald else None,
+                                       self.total_d.data_ptr(), st))
 
     def capture_md(self, nsteps=1):
         """Captures nsteps Verlet steps of every system in a HIP graph (all
