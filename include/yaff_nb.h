@@ -151,8 +151,11 @@ typedef struct ynb_nlist {
                           retry. */
   const int32_t *rebuild; /* [nsys] per-system rebuild flags (ynb_nlist_skin_check) or NULL: rebuild all */
   int32_t kcap;
-  int32_t pad_;
+  int32_t flags;       /* YNB_NLIST_*: 0, or PREBINNED when a ynb_verlet_kick_drift given this list and
+                          workspace has already binned the atoms at the positions of this build */
 } ynb_nlist;
+
+#define YNB_NLIST_PREBINNED 1  /* (requires rebuild == NULL) */
 
 /* Per-atom parameters of the pair potentials. */
 typedef struct ynb_pairpar {
@@ -193,6 +196,9 @@ typedef struct ynb_ewald {
   int32_t fslots;         /* partial-gradient slots: the largest ynb_sys.nslot */
   int32_t groups;         /* nonzero when some system has nlow > 0 or nhigh < natom */
   int32_t fapl;           /* atoms per lane of the force pass: 2 or 4 */
+  int32_t flags;          /* YNB_EWALD_*: 0, or PREPARED when a ynb_verlet_kick_drift given this
+                             struct has already filled frac and step for the current positions */
+  int32_t pad_;
   double *frac;           /* work [4][npad]: wrapped fractional coordinates and charge */
   double *step;           /* work [2][npad]: cos, sin of 2 pi f0 */
   double *spart;          /* work: partial structure factors, 2 doubles per (chunk, k) */
@@ -202,6 +208,8 @@ typedef struct ynb_ewald {
   double *kpart;          /* work [nk_tot/32][8] */
   double *gpart;          /* work [fslots][npad][3] */
 } ynb_ewald;
+
+#define YNB_EWALD_PREPARED 1
 
 /* ------------------------------------------------------------------ */
 const char *ynb_version(void);
@@ -303,9 +311,17 @@ int ynb_ewald_corr(const ynb_sys *h_sys, const ynb_sys *d_sys, const double *pos
  * system's nslot Ewald partial gradients to gpos, in slot order, and stores
  * the sum in gpos -- the Ewald gradient sum folded into the kick.  total
  * (NULL or as in ynb_total, with pair_out/ewald_out nullable): kick also
- * totals each system's energy and virial, in place of a ynb_total call. */
+ * totals each system's energy and virial, in place of a ynb_total call.
+ * kick_drift with ew (nullable; charges as in ynb_ewald_reci) also fills
+ * ew->frac and ew->step for the new positions, after which ynb_ewald_reci
+ * with ew->flags = YNB_EWALD_PREPARED skips that pass; with nl_work (the
+ * ynb_nlist_build workspace, nullable) and nl (rebuild == NULL) it also bins
+ * the atoms at the new positions, after which ynb_nlist_build with
+ * nl->flags = YNB_NLIST_PREBINNED skips its binning pass. */
 int ynb_verlet_kick_drift(const ynb_batch *b, double *pos, double *vel, const double *gpos,
-                          const double *minv, const double *dt, ynb_stream stream);
+                          const double *minv, const double *dt, const ynb_ewald *ew,
+                          const double *charges, void *nl_work, size_t nl_work_bytes,
+                          const ynb_nlist *nl, ynb_stream stream);
 int ynb_verlet_kick(const ynb_batch *b, double *vel, double *gpos, const double *gpart,
                     const double *minv, const double *masses, const double *dt, double *partial,
                     int32_t *ctr, double *ekin2, const double *pair_out, const double *ewald_out,

@@ -18,6 +18,8 @@ FP64, MIXED = 0, 1
 BATCH_ALLPAIRS, BATCH_EXACT_BUILD = 1, 2
 BATCH_FORM_GENERIC, BATCH_FORM_LJ, BATCH_FORM_IONIC = 4, 8, 16
 BATCH_WIDE_SEARCH = 32
+NLIST_PREBINNED = 1
+EWALD_PREPARED = 1
 FORM_GENERIC, FORM_LJ, FORM_IONIC = 0, 1, 2
 
 ROW_DTYPE = np.dtype([('a', np.int64), ('b', np.int64), ('d', np.float64), ('dx', np.float64),
@@ -51,7 +53,7 @@ class YnbBatch(C.Structure):
 
 class YnbNlist(C.Structure):
     _fields_ = [('j', C.c_void_p), ('img', C.c_void_p), ('cnt', C.c_void_p), ('status', C.c_void_p),
-                ('rebuild', C.c_void_p), ('kcap', C.c_int32), ('pad_', C.c_int32)]
+                ('rebuild', C.c_void_p), ('kcap', C.c_int32), ('flags', C.c_int32)]
 
 
 class YnbPairpar(C.Structure):
@@ -65,7 +67,8 @@ class YnbEwald(C.Structure):
                 ('kpos', C.c_void_p), ('krow_f', C.c_void_p), ('fctr', C.c_void_p),
                 ('nwork', C.c_int32), ('nk_tot', C.c_int32),
                 ('apl', C.c_int32), ('fslots', C.c_int32),
-                ('groups', C.c_int32), ('fapl', C.c_int32), ('frac', C.c_void_p), ('step', C.c_void_p),
+                ('groups', C.c_int32), ('fapl', C.c_int32), ('flags', C.c_int32), ('pad_', C.c_int32),
+                ('frac', C.c_void_p), ('step', C.c_void_p),
                 ('spart', C.c_void_p), ('coef', C.c_void_p), ('frec', C.c_void_p), ('kpart', C.c_void_p),
                 ('gpart', C.c_void_p)]
 
@@ -109,7 +112,7 @@ def load(path=LIB_PATH):
     L.ynb_pair_rows.argtypes = [vp, i64, vp, i64, P(YnbPot), vp, vp, vp, vp, vp, vp, vp, vp]
     L.ynb_ewald_reci.argtypes = [P(YnbBatch), P(YnbEwald), vp, vp, i32, vp, i32, vp, vp]
     L.ynb_ewald_corr.argtypes = [P(YnbSys), vp, vp, vp, vp, i64, vp, i32, vp, vp]
-    L.ynb_verlet_kick_drift.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp, vp]
+    L.ynb_verlet_kick_drift.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp, P(YnbEwald), vp, vp, sz, P(YnbNlist), vp]
     L.ynb_verlet_kick.argtypes = [P(YnbBatch)] + [vp]*13
     L.ynb_total.argtypes = [P(YnbBatch), vp, vp, vp, vp]
     L.ynb_profile_enable.argtypes = [C.c_int]

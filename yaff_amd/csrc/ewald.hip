@@ -27,36 +27,20 @@
 // No floating-point atomics: results do not depend on scheduling or on the
 // other systems of a batch.
 #include "common.h"
+#include "ewald_prep.h"
 
 namespace ynb {
 namespace {
 
 __global__ void k_ewald_prep(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, const double *pos,
-                             const double *charges, double *frac, double *step, double *out) {
+                             const double *charges, double *frac, double *step) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (g >= npad) return;
   const int sid = wave_sys[g >> 6];
-  double f[3] = {0.0, 0.0, 0.0}, q = 0.0, c = 1.0, s = 0.0;
-  if (sid >= 0) {
-    const ynb_sys &S = sys[sid];
-    if (g == S.atom0 && S.nk == 0)  // no k vectors: the reduction never writes this system's row
-      for (int i = 0; i < 10; ++i) out[10 * (int64_t)sid + i] = 0.0;
-    if (g - S.atom0 < S.natom && S.nk > 0) {
-      const double p0 = pos[3 * g], p1 = pos[3 * g + 1], p2 = pos[3 * g + 2];
-      for (int k = 0; k < 3; ++k) {
-        const double x = fma(S.gvecs[3 * k + 2], p2, fma(S.gvecs[3 * k + 1], p1, S.gvecs[3 * k] * p0));
-        f[k] = x - floor(x);
-      }
-      q = charges[g];
-      sincospi(2.0 * f[0], &s, &c);
-    }
-  }
-  frac[g] = f[0];
-  frac[npad + g] = f[1];
-  frac[2 * npad + g] = f[2];
-  frac[3 * npad + g] = q;
-  step[g] = c;
-  step[npad + g] = s;
+  const ynb_sys *S = sid >= 0 ? &sys[sid] : nullptr;
+  const bool active = S && g - S->atom0 < S->natom;
+  ewald_prep_atom(S, active, g, npad, active ? pos[3 * g] : 0.0, active ? pos[3 * g + 1] : 0.0,
+                  active ? pos[3 * g + 2] : 0.0, charges, frac, step);
 }
 
 // sin and cos of 2 pi t for |t| <= 1/2 (a phase in turns), FP64: quadrant
@@ -407,7 +391,12 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
   const int64_t blk = blockIdx.x;
   if (blk >= npad / NA) {  // one block per system: energy and virial from the k-block sums
     const int s = (int)(blk - npad / NA);
-    if (blockIdx.y == 0 && sys[s].nk > 0) ewald_system_reduce(sys[s], s, kpart, out);
+    if (blockIdx.y == 0) {
+      if (sys[s].nk > 0)
+        ewald_system_reduce(sys[s], s, kpart, out);
+      else if (threadIdx.x < 10)  // no k vectors: a zero term
+        out[10 * (int64_t)s + threadIdx.x] = 0.0;
+    }
     return;
   }
   const int sid = wave_sys[P * blk];
@@ -520,7 +509,12 @@ __global__ __launch_bounds__(256) void k_ewald_force_groups(const ynb_sys *__res
   const int64_t blk = blockIdx.x;
   if (blk >= npad / NA) {  // one block per system: energy and virial from the k-block sums
     const int s = (int)(blk - npad / NA);
-    if (blockIdx.y == 0 && sys[s].nk > 0) ewald_system_reduce(sys[s], s, kpart, out);
+    if (blockIdx.y == 0) {
+      if (sys[s].nk > 0)
+        ewald_system_reduce(sys[s], s, kpart, out);
+      else if (threadIdx.x < 10)  // no k vectors: a zero term
+        out[10 * (int64_t)s + threadIdx.x] = 0.0;
+    }
     return;
   }
   const int sid = wave_sys[P * blk];
@@ -654,9 +648,11 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
   }
   hipStream_t st = (hipStream_t)stream;
   const int64_t npad = b->npad;
-  YNB_LAUNCH("ewald_prep", k_ewald_prep, dim3((unsigned)ceil_div(npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys, npad,
-             pos, charges, ew->frac, ew->step, out);
-  YNB_LAUNCHED();
+  if (!(ew->flags & YNB_EWALD_PREPARED)) {
+    YNB_LAUNCH("ewald_prep", k_ewald_prep, dim3((unsigned)ceil_div(npad, 256)), dim3(256), 0, st, b->sys, b->wave_sys,
+               npad, pos, charges, ew->frac, ew->step);
+    YNB_LAUNCHED();
+  }
   if (ew->nwork > 0) {
     const dim3 grid((unsigned)(ew->nwork < (1 << 30) ? ew->nwork : (1 << 30)));
 #define YNB_SK(PR, PP)                                                                                        \

@@ -4,6 +4,8 @@
 #include <string.h>
 
 #include "common.h"
+#include "ewald_prep.h"
+#include "nlist_work.h"
 
 namespace ynb {
 
@@ -43,23 +45,33 @@ void prof_after(hipStream_t st) {
 
 namespace {
 
-// acc = -gpos/m; vel += dt/2 acc; pos += dt vel  (verlet.py:344-347)
-__global__ void k_kick_drift(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, double *pos, double *vel,
-                             const double *gpos, const double *minv, const double *dt) {
+// acc = -gpos/m; vel += dt/2 acc; pos += dt vel  (verlet.py:344-347); then,
+// with frac, the Ewald set-up of the atom at its new position, and with
+// bin_count its cell binning for the next neighbour-list build.
+__global__ __launch_bounds__(256) void k_kick_drift(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
+                                                    double *pos, double *vel, const double *gpos, const double *minv,
+                                                    const double *dt, const double *charges, double *frac,
+                                                    double *step, int32_t cb, int32_t *bin_count, int32_t *slot_idx,
+                                                    double4 *xw, int4 *wr, int32_t *status) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= npad) return;
-  const int sid = wave_sys[g >> 6];
-  if (sid < 0) return;
-  const ynb_sys &S = sys[sid];
-  if (g - S.atom0 >= S.natom) return;
-  const double h = dt[sid];
-  const double mi = minv[g];
-  for (int c = 0; c < 3; ++c) {
-    const double a = -gpos[3 * g + c] * mi;
-    const double v = vel[3 * g + c] + 0.5 * a * h;
-    vel[3 * g + c] = v;
-    pos[3 * g + c] += h * v;
+  if (g >= npad) return;  // whole waves: npad is a multiple of 64
+  const int sid = uniform_int(wave_sys[g >> 6]);
+  const ynb_sys *S = sid >= 0 ? &sys[sid] : nullptr;
+  const bool active = S && g - S->atom0 < S->natom;
+  double p[3] = {0.0, 0.0, 0.0};
+  if (active) {
+    const double h = dt[sid];
+    const double mi = minv[g];
+    for (int c = 0; c < 3; ++c) {
+      const double a = -gpos[3 * g + c] * mi;
+      const double v = vel[3 * g + c] + 0.5 * a * h;
+      vel[3 * g + c] = v;
+      p[c] = pos[3 * g + c] + h * v;
+      pos[3 * g + c] = p[c];
+    }
   }
+  if (frac) ewald_prep_atom(S, active, g, npad, p[0], p[1], p[2], charges, frac, step);
+  if (bin_count && S) bin_atom(*S, active, g, p[0], p[1], p[2], cb, bin_count, slot_idx, xw, wr, status);
 }
 
 // Term c of system s's total: its potentials' pair terms in order, then the
@@ -179,13 +191,26 @@ int ynb_profile_read(int i, const char **name, float *ms) {
 }
 
 int ynb_verlet_kick_drift(const ynb_batch *b, double *pos, double *vel, const double *gpos, const double *minv,
-                          const double *dt, ynb_stream stream) {
-  if (!b || !pos || !vel || !gpos || !minv || !dt) {
+                          const double *dt, const ynb_ewald *ew, const double *charges, void *nl_work,
+                          size_t nl_work_bytes, const ynb_nlist *nl, ynb_stream stream) {
+  if (!b || !pos || !vel || !gpos || !minv || !dt || b->npad % 64 != 0 || (ew && (!charges || !ew->frac || !ew->step)) ||
+      (nl_work && (!nl || nl->rebuild || b->max_bin_atoms < 1 || b->max_bin_atoms > 1024))) {
     set_error("ynb_verlet_kick_drift: invalid arguments");
     return YNB_EINVAL;
   }
+  Work w{};
+  const bool bin = nl_work && b->nbin > 0;
+  if (bin) {
+    w = carve(nl_work, b->npad, b->nbin, b->max_bin_atoms);
+    if (nl_work_bytes < w.total) {
+      set_error("ynb_verlet_kick_drift: workspace of %zu bytes, need %zu", nl_work_bytes, w.total);
+      return YNB_EWORKSPACE;
+    }
+  }
   YNB_LAUNCH("verlet_kick_drift", k_kick_drift, dim3((unsigned)ceil_div(b->npad, 256)), dim3(256), 0,
-             (hipStream_t)stream, b->sys, b->wave_sys, b->npad, pos, vel, gpos, minv, dt);
+             (hipStream_t)stream, b->sys, b->wave_sys, b->npad, pos, vel, gpos, minv, dt, charges,
+             ew ? ew->frac : nullptr, ew ? ew->step : nullptr, b->max_bin_atoms, bin ? w.bin_count : nullptr,
+             w.slot_idx, w.xw, w.wr, bin ? nl->status : nullptr);
   YNB_LAUNCHED();
   return YNB_OK;
 }

@@ -12,44 +12,11 @@
 #include <hipcub/hipcub.hpp>
 
 #include "common.h"
+#include "nlist_work.h"
 
 namespace ynb {
 
 namespace {
-
-struct Work {
-  int32_t *slot_idx;   // [nbin][cb] atoms of each bin in arrival order
-  int32_t *bin_count;  // [nbin]
-  double4 *xw, *sxw;   // wrapped coordinates: per atom, and per bin slot
-  int4 *wr, *sw;       // wrap counts (+ atom index in .w)
-  int32_t *bin_start, *bin_end;
-  size_t total;
-};
-
-inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
-
-Work carve(void *base, int64_t npad, int32_t nbin, int32_t cb) {
-  Work w;
-  char *p = (char *)base;
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    char *q = p ? p + off : nullptr;
-    off += align256(bytes);
-    return q;
-  };
-  const size_t nb = (size_t)(nbin > 0 ? nbin : 1);
-  const size_t slots = nb * (size_t)(cb > 0 ? cb : 1);
-  w.slot_idx = (int32_t *)take(4 * slots);
-  w.bin_count = (int32_t *)take(4 * nb);
-  w.xw = (double4 *)take(32 * npad);
-  w.wr = (int4 *)take(16 * npad);
-  w.sxw = (double4 *)take(32 * slots);
-  w.sw = (int4 *)take(16 * slots);
-  w.bin_start = (int32_t *)take(4 * nb);
-  w.bin_end = (int32_t *)take(4 * nb);
-  w.total = off;
-  return w;
-}
 
 __device__ inline int floor_div(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 
@@ -68,35 +35,9 @@ __global__ void k_bin(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
   const int sid = uniform_int(wave_sys[g >> 6]);
   if (sid < 0 || (rebuild && !rebuild[sid])) return;
   const ynb_sys &S = sys[sid];
-  int fill = 0;
-  if (g - S.atom0 < S.natom && S.nbin != 0) {
-    const double p0 = pos[3 * g], p1 = pos[3 * g + 1], p2 = pos[3 * g + 2];
-    int c[3], w[3];
-    double x = p0, y = p1, z = p2;
-    for (int k = 0; k < 3; ++k) {
-      const double f = fma(S.gvecs[3 * k + 2], p2, fma(S.gvecs[3 * k + 1], p1, S.gvecs[3 * k] * p0));
-      const double fl = floor(f);
-      int ci = (int)((f - fl) * S.grid[k]);
-      ci = ci < 0 ? 0 : (ci >= S.grid[k] ? S.grid[k] - 1 : ci);
-      c[k] = ci;
-      w[k] = (int)fl;
-      x = fma(-fl, S.rvecs[3 * k], x);
-      y = fma(-fl, S.rvecs[3 * k + 1], y);
-      z = fma(-fl, S.rvecs[3 * k + 2], z);
-    }
-    const int key = S.bin0 + (c[0] * S.grid[1] + c[1]) * S.grid[2] + c[2];
-    xw[g] = make_double4(x, y, z, 0.0);
-    wr[g] = make_int4(w[0], w[1], w[2], (int)g);
-    const int slot = atomicAdd(bin_count + key, 1);
-    if (slot < cb)
-      slot_idx[(int64_t)key * cb + slot] = (int32_t)g;
-    else
-      atomicOr(status, 4);
-    fill = slot + 1;
-  }
-  // the fullest bin, recorded only on overflow (the caller sizes the retry
-  // from it; bin counts are cleared by k_bin_order)
-  wave_status_max(status + 1, fill > cb ? fill : 0);
+  const bool active = g - S.atom0 < S.natom;
+  bin_atom(S, active, g, active ? pos[3 * g] : 0.0, active ? pos[3 * g + 1] : 0.0, active ? pos[3 * g + 2] : 0.0,
+           cb, bin_count, slot_idx, xw, wr, status);
 }
 
 // One block per bin: orders the bin's atoms by index (rank among the bin's
@@ -695,7 +636,8 @@ size_t ynb_nlist_workspace_bytes(int64_t npad, int32_t nbin, int32_t bin_cap) {
 int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t work_bytes,
                     const ynb_nlist *nl, ynb_stream stream) {
   if (!b || !nl || !pos || b->npad % 64 != 0 || nl->kcap < 1 ||
-      (b->nbin > 0 && (b->max_bin_atoms < 1 || b->max_bin_atoms > 1024))) {
+      (b->nbin > 0 && (b->max_bin_atoms < 1 || b->max_bin_atoms > 1024)) ||
+      ((nl->flags & YNB_NLIST_PREBINNED) && nl->rebuild)) {
     set_error("ynb_nlist_build: invalid arguments");
     return YNB_EINVAL;
   }
@@ -712,8 +654,9 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
   if (b->nbin > 0) {
     // bin counts are zero here: the caller zero-initialises the workspace
     // and k_bin_order resets the counts after each build
-    YNB_LAUNCH("nlist_bin", k_bin, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, cb, w.bin_count,
-               w.slot_idx, w.xw, w.wr, nl->status, nl->rebuild);
+    if (!(nl->flags & YNB_NLIST_PREBINNED))
+      YNB_LAUNCH("nlist_bin", k_bin, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, cb, w.bin_count,
+                 w.slot_idx, w.xw, w.wr, nl->status, nl->rebuild);
     YNB_LAUNCH("nlist_bin_order", k_bin_order, dim3(b->nbin), dim3(256), 0, st, cb, w.bin_count, w.slot_idx, w.xw,
                w.wr, w.sxw, w.sw, w.bin_start, w.bin_end, b->bin_sys, nl->rebuild);
 #define YNB_BUILD(EX, NBB)                                                                                    \

@@ -551,7 +551,8 @@ class Engine(object):
                               t['skblk'].data_ptr(), t['fblk'].data_ptr(), t['chunk'].data_ptr(),
                               t['work'].data_ptr(), t['kblk_sys'].data_ptr(), t['kpos'].data_ptr(),
                               t['krow_f'].data_ptr(), self._ctr(3*self.nsys), self.nwork,
-                              self.nk_tot, self.apl, self.fslots, self.groups, self.fapl, self.ew_frac_d.data_ptr(),
+                              self.nk_tot, self.apl, self.fslots, self.groups, self.fapl, 0, 0,
+                              self.ew_frac_d.data_ptr(),
                               self.ew_step_d.data_ptr(), self.ew_spart_d.data_ptr(), self.ew_coef_d.data_ptr(),
                               self.ew_frec_d.data_ptr(), self.ew_kpart_d.data_ptr(), self.ew_gpart_d.data_ptr())
 
@@ -722,14 +723,33 @@ class Engine(object):
         """kick, drift, neighbour list, forces, kick: one Verlet step of every
         system, on device, without synchronisation."""
         st = self.stream()
+        # the drift also prepares the Ewald pass and bins the atoms for the
+        # list build (not with a skin: the skin test decides which systems
+        # are rebuilt, after the drift)
+        prebin = not self.use_skin and self.batch.nbin > 0
+        ew = self.ewald if self.has_ewald else None
         check(self.lib.ynb_verlet_kick_drift(self.batch, self.pos_d.data_ptr(), self.vel_d.data_ptr(),
                                              self.gpos_d.data_ptr(), self.minv_d.data_ptr(), self.dt_d.data_ptr(),
-                                             st))
-        # forces: the Ewald partial gradients stay in their slots and are
-        # summed by the kick, which leaves the total gradient in gpos_d
-        self.build_nlist(force=False)
+                                             ew, self.charges_d.data_ptr(),
+                                             self.nl_work_d.data_ptr() if prebin else None, self.nl_work_bytes,
+                                             self.nlist if prebin else None, st))
+        if prebin:
+            self.nlist.flags = _lib.NLIST_PREBINNED
+            try:
+                self.build_nlist(force=False)
+            finally:
+                self.nlist.flags = 0
+        else:
+            self.build_nlist(force=False)
         self.pair(gpos=True, accumulate=False)
-        self.ewald_reci(gpos=False)
+        # the Ewald partial gradients stay in their slots and are summed by
+        # the kick, which leaves the total gradient in gpos_d ...
+        if ew is not None:
+            ew.flags = _lib.EWALD_PREPARED
+            try:
+                self.ewald_reci(gpos=False)
+            finally:
+                ew.flags = 0
         # ... and the kick also totals energy and virial (in place of total())
         check(self.lib.ynb_verlet_kick(self.batch, self.vel_d.data_ptr(), self.gpos_d.data_ptr(),
                                        self.ew_gpart_d.data_ptr() if self.has_ewald else None,
