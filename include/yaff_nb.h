@@ -200,7 +200,7 @@ typedef struct ynb_ewald {
                              struct has already filled frac and step for the current positions */
   int32_t pad_;
   double *frac;           /* work [4][npad]: wrapped fractional coordinates and charge */
-  double *step;           /* work [2][npad]: cos, sin of 2 pi f0 */
+  double *step;           /* work [4][npad]: cos, sin of 2 pi f0, then of 2 pi f1 */
   double *spart;          /* work: partial structure factors, 2 doubles per (chunk, k) */
   double *coef;           /* work [nk_tot][6]: 2 c_k S(k) of the mid, low and high groups (cos, sin) */
   double *frec;           /* work [nfrec][2]: 2 c_k S_sin, 2 c_k S_cos of the mid group per k; each row

@@ -100,6 +100,34 @@ __device__ inline double reduced_phase(int g0, int g1, int g2, double f0, double
   return t - rint(t);
 }
 
+// Row seeds by continuation.  The next row of a block usually lies in the
+// same g2 plane one step up in g1 and starts within two steps of the previous
+// row's first g0: then its seed is the previous seed times exp(2 pi i f1)
+// and exp(+-2 pi i f0) once or twice (complex products, a few ulp each),
+// instead of a sine and cosine from scratch.  Other rows are seeded from the
+// reduced phase.  The test depends only on the row records (uniform).
+__device__ inline int row_step(int4 prev, int4 row, bool have_prev) {
+  const int dx = row.x - prev.x;
+  return (have_prev && row.z == prev.z && row.y == prev.y + 1 && dx >= -2 && dx <= 2) ? dx : 3;
+}
+
+// z *= (br + i bi)
+__device__ inline void cmul(double &zr, double &zi, double br, double bi) {
+  const double nr = zr * br - zi * bi;
+  zi = zr * bi + zi * br;
+  zr = nr;
+}
+
+// z *= exp(2 pi i (f1 + dx f0)) for |dx| <= 2 (dx uniform).
+__device__ inline void continue_seed(double &zr, double &zi, double c1, double s1, double c0, double s0, int dx) {
+  cmul(zr, zi, c1, s1);
+  if (dx != 0) {
+    const double sd = dx > 0 ? s0 : -s0;
+    cmul(zr, zi, c0, sd);
+    if (dx == 2 || dx == -2) cmul(zr, zi, c0, sd);
+  }
+}
+
 // Structure factors.  One wavefront per work item (row block, atom chunk):
 // lane l holds atoms l, l + 64, ... of the chunk (P per lane).  Per k the
 // lane adds its atoms' q exp(i k.r) and stores the sum in row m of an LDS
@@ -126,7 +154,7 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
     const ynb_sys &S = sys[wk.x];
     const int4 ch = *(const int4 *)(chunk + 4 * wk.z);
     const int rb = skblk[2 * wk.y], re = skblk[2 * wk.y + 1];
-    double f0[P], f1[P], f2[P], q[P], cr[P], ci[P], zr[P], zi[P];
+    double f0[P], f1[P], f2[P], q[P], cr[P], ci[P], c1[P], s1[P], zr[P], zi[P], sr0[P], si0[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       const int la = lane + 64 * p;
@@ -138,8 +166,11 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
       q[p] = valid ? frac[3 * npad + g] : 0.0;
       cr[p] = step[g];
       ci[p] = step[npad + g];
-      zr[p] = zi[p] = 0.0;
+      c1[p] = step[2 * npad + g];
+      s1[p] = step[3 * npad + g];
+      zr[p] = zi[p] = sr0[p] = si0[p] = 0.0;
     }
+    int4 prev = make_int4(0, 0, 0, 0);
     double *out = spart + 2 * (S.spart0 + (int64_t)(wk.z - S.chunk0) * S.nk - S.k0);
     int kt = krow_k[rb];  // first k of the current tile
     int m = 0;            // k stored in the tile
@@ -147,12 +178,24 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
     for (int r = rb; r < re; ++r) {
       const int4 row = rown;
       if (r + 1 < re) rown = *(const int4 *)(krow + 4 * (r + 1));
+      const int dx = row_step(prev, row, r > rb);
+      prev = row;
+      if (dx != 3) {
 #pragma unroll
-      for (int p = 0; p < P; ++p) {  // row start: seed q exp(2 pi i g.f)
-        double sn, cs;
-        seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, f0[p], f1[p], f2[p]), &sn, &cs);
-        zr[p] = q[p] * cs;
-        zi[p] = q[p] * sn;
+        for (int p = 0; p < P; ++p) continue_seed(sr0[p], si0[p], c1[p], s1[p], cr[p], ci[p], dx);
+      } else {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {  // row start: seed q exp(2 pi i g.f)
+          double sn, cs;
+          seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, f0[p], f1[p], f2[p]), &sn, &cs);
+          sr0[p] = q[p] * cs;
+          si0[p] = q[p] * sn;
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        zr[p] = sr0[p];
+        zi[p] = si0[p];
       }
       for (int j = 0; j < row.w;) {
         // k up to the end of the row or of the tile: a loop without branches
@@ -382,7 +425,7 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
                                                      const double *__restrict__ frac, const double *__restrict__ step,
                                                      double *__restrict__ gpart, int64_t npad, int32_t *fctr,
                                                      double *gpos, int accumulate, const double *__restrict__ kpart,
-    double *__restrict__ out) {
+                                                     double *__restrict__ out) {
 #pragma clang fp contract(fast)
   constexpr int NA = 64 * P;
   constexpr double TWO_PI = 6.283185307179586477;
@@ -403,7 +446,8 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
   if (sid < 0) return;
   const ynb_sys &S = sys[sid];
   if (S.nk == 0 || (int)blockIdx.y >= S.nslot) return;  // the same for the whole block
-  double f0[P], f1[P], f2[P], cr[P], ci[P], zr[P], zi[P], T[P], U[P], G0[P], G1[P], G2[P];
+  double f0[P], f1[P], f2[P], cr[P], ci[P], c1[P], s1[P], zr[P], zi[P], sr0[P], si0[P], T[P], U[P], G0[P], G1[P],
+      G2[P];
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     const int64_t g = blk * NA + lane + 64 * p;
@@ -412,7 +456,10 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
     f2[p] = frac[2 * npad + g];
     cr[p] = step[g];
     ci[p] = step[npad + g];
+    c1[p] = step[2 * npad + g];
+    s1[p] = step[3 * npad + g];
     G0[p] = G1[p] = G2[p] = 0.0;
+    sr0[p] = si0[p] = 0.0;
   }
   const int fb = 4 * (int)blockIdx.y + wv;
   if (fb < S.nfrb) {
@@ -421,12 +468,24 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
     const double *rp = frec + 2 * (int64_t)krow_f[rb];
     double4 na = *(const double4 *)rp, nb = *(const double4 *)(rp + 4);  // next 4 records
     int4 rown = *(const int4 *)(krow + 4 * rb);
+    int4 prev = make_int4(0, 0, 0, 0);
     for (int r = rb; r < re; ++r) {
       const int4 row = rown;
       if (r + 1 < re) rown = *(const int4 *)(krow + 4 * (r + 1));
+      const int dx = row_step(prev, row, r > rb);
+      prev = row;
+      if (dx != 3) {
 #pragma unroll
-      for (int p = 0; p < P; ++p) {  // row start: seed exp(2 pi i g.f)
-        seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, f0[p], f1[p], f2[p]), &zi[p], &zr[p]);
+        for (int p = 0; p < P; ++p) continue_seed(sr0[p], si0[p], c1[p], s1[p], cr[p], ci[p], dx);
+      } else {
+#pragma unroll
+        for (int p = 0; p < P; ++p)  // row start: seed exp(2 pi i g.f)
+          seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, f0[p], f1[p], f2[p]), &si0[p], &sr0[p]);
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        zr[p] = sr0[p];
+        zi[p] = si0[p];
         T[p] = U[p] = 0.0;
       }
       const int ng = (row.w + 3) >> 2;

@@ -540,7 +540,7 @@ class Engine(object):
         e = self.ewald_h
         self.ew_tabs = {k: self._t(v) for k, v in e.items()}
         self.ew_frac_d = torch.zeros(4*self.npad, dtype=torch.float64, device=dev)
-        self.ew_step_d = torch.zeros(2*self.npad, dtype=torch.float64, device=dev)
+        self.ew_step_d = torch.zeros(4*self.npad, dtype=torch.float64, device=dev)
         self.ew_spart_d = torch.zeros(2*self.spart_entries, dtype=torch.float64, device=dev)
         self.ew_coef_d = torch.zeros(6*max(self.nk_tot, KBLOCK), dtype=torch.float64, device=dev)
         self.ew_frec_d = torch.zeros(2*self.nfrec, dtype=torch.float64, device=dev)

@@ -113,7 +113,7 @@ def estimate_bytes(nbs, tuning=SERVER_TUNING):
         # greedy row blocks: two consecutive blocks hold more than fkb k vectors
         nslot = max(nslot, _ceil_div(2*_ceil_div(nk, fkb) + 1, 4))
     if nslot:
-        total += npad*(nslot*24 + 48)
+        total += npad*(nslot*24 + 64)
     return int(total)
 
 
