@@ -20,11 +20,6 @@ namespace {
 
 __device__ inline int floor_div(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 
-__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
 // Fractional wrap and bin of every atom of the binned systems; each atom
 // takes a slot of its bin (arrival order, made deterministic by k_bin_order).
 __global__ void k_bin(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, const double *pos, int32_t cb,
