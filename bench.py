@@ -55,6 +55,8 @@ def parse():
     ap.add_argument('--batch-steps', type=int, default=200)
     ap.add_argument('--batch-systems', type=int, default=64)
     ap.add_argument('--no-served', action='store_true')
+    ap.add_argument('--no-precision', action='store_true',
+                    help='skip the 55,296-atom triclinic FP64-versus-mixed comparison')
     ap.add_argument('--serve-requests', type=int, default=150)
     ap.add_argument('--serve-rates', type=str, default='25,100,400')
     ap.add_argument('--tuning', type=str, default=None,
@@ -221,6 +223,56 @@ def profiled_traffic(kernel):
             continue
         return float(parts[1]), float(parts[2])
     return None
+
+
+def precision_large(steps, warmup):
+    """The precision workload: rocksalt 24x24x12 conventional cells (55,296
+    atoms), triclinic with angles drawn uniformly in 85-95 degrees (seed 2),
+    ExpRep + LJ + EI + Ewald as the headline system.  One FP64 and one mixed
+    evaluation (errors of the mixed one: energy, RMS and max force, virial;
+    the FP64 GPU evaluation is the reference, itself checked against the
+    oracle to 1e-10 by the test suite), then MD steps of each from a graph."""
+    import torch
+    from yaff_amd.engine import Engine
+    from yaff_amd.units import femtosecond
+    from yaff_amd import autotune
+    rng = np.random.default_rng(2)
+    angles = tuple(float(a) for a in rng.uniform(85.0, 95.0, 3))
+    s, nb = rocksalt_system(reps=(24, 24, 12), seed=2, angles=angles)
+    n = s.natom
+    res, out = {}, {'workload': 'rocksalt 24x24x12 cells (%d atoms), triclinic angles %s deg, ExpRep+LJ+EI rcut 11 A '
+                                '+ Ewald reci gcut 0.32/A' % (n, ', '.join('%.2f' % a for a in angles)),
+                    'natom': n}
+    for prec in ('fp64', 'mixed'):
+        eng = Engine([nb], precision=prec)
+        eng.ensure_nlist()
+        tun = autotune.tune(eng, cache=None)
+        eng.compute(build=False)
+        res[prec] = eng.results(0)
+        eng.set_velocities([maxwell_boltzmann(s.masses, 300.0, 1)])
+        eng.set_timestep(1.0*femtosecond)
+        run_steps(eng, warmup)
+        torch.cuda.synchronize()
+        flags, _ = eng.nlist_status()
+        if flags:
+            eng.ensure_nlist()
+        dt = timed_graph(eng, steps, warmup)
+        out[prec] = {'value': n*steps/dt, 'ms_per_step': dt/steps*1e3, 'steps': steps, 'tuning': tun}
+        if prec == 'fp64':
+            out['kvectors'] = int((eng.ewald_h['kvec'][:, 3] != 0).sum())
+        del eng
+        torch.cuda.empty_cache()
+    e64, g64, v64 = res['fp64']
+    em, gm, vm = res['mixed']
+    fn = np.sqrt((g64**2).sum(axis=1))
+    out['mixed_error'] = {
+        'energy_rel_err': float(abs(em - e64)/abs(e64)),
+        'force_rms_rel_err': float(np.sqrt(((gm - g64)**2).sum(axis=1).mean())/np.sqrt((fn**2).mean())),
+        'force_max_abs_err': float(np.abs(gm - g64).max()),
+        'force_max_rel_err': float(np.sqrt(((gm - g64)**2).sum(axis=1)).max()/fn.max()),
+        'virial_rel_err': float(np.abs(vm - v64).max()/np.abs(v64).max()),
+        'reference': 'FP64 GPU evaluation of the same positions'}
+    return out
 
 
 def progress(msg):
@@ -462,6 +514,10 @@ def main():
                              'kernel': 'torch elementwise mul (out=)'}
         del src, dst
     progress('single-system FP64/mixed done')
+    if not args.no_precision:
+        out['precision'] = precision_large(max(K//2, 1), 2)
+        progress('precision (55,296 atoms) done: mixed RMS force error %.2e'
+                 % out['precision']['mixed_error']['force_rms_rel_err'])
     if not args.no_batched:
         out['batched'] = batched(args.batch_systems, args.batch_steps, 2)
         progress('batched done')
