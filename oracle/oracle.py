@@ -203,6 +203,8 @@ class Oracle(object):
             L.trunc_scheme_fn.argtypes = [vp, d, d, vp]
             L.pair_pot_compute.restype = d
             L.pair_pot_compute.argtypes = [vp, l, vp, l, vp, vp, vp]
+            L.pair_pot_tailcorr_cut.argtypes = [vp, l, l, l, vp]
+            L.pair_pot_tailcorr_switch3.argtypes = [vp, l, l, l, vp]
             L.compute_ewald_reci.restype = d
             L.compute_ewald_reci.argtypes = [vp, l, l, l, vp, vp, d, vp, d, d, vp, vp, vp]
             L.compute_ewald_corr.restype = d
@@ -352,6 +354,36 @@ class Oracle(object):
         return e, gpos, vtens
 
     # -- Ewald --------------------------------------------------------------
+    def tailcorr(self, pot, natom, nlow=0, nhigh=-1):
+        """(C0, C1) of pair_pot_tailcorr_cut / _switch3 (pair_pot.c:178-254)
+        for an LJ, MM3 or ExpRep potential; the compiled reference only."""
+        if not self.reference:
+            raise NotImplementedError('tail corrections: compiled reference only')
+        if nhigh == -1:
+            nhigh = natom
+        L = self.lib
+        pp = L.pair_pot_new()
+        L.pair_pot_set_rcut(pp, pot.rcut)
+        tr = L.switch3_new(pot.trunc_par) if pot.trunc == TR_SWITCH3 else None
+        L.pair_pot_set_trunc_scheme(pp, tr)
+        if pot.kind == LJ:
+            L.pair_data_lj_init(pp, _ptr(pot.p0), _ptr(pot.p1))
+        elif pot.kind == MM3:
+            L.pair_data_mm3_init(pp, _ptr(pot.p0), _ptr(pot.p1), _ptr(pot.onlypauli))
+        elif pot.kind == EXPREP:
+            L.pair_data_exprep_init(pp, pot.amp_cross.shape[0], _ptr(pot.ffatype_ids),
+                                    _ptr(pot.amp_cross), _ptr(pot.b_cross))
+        else:
+            raise ValueError('no tail correction for this potential')
+        corrs = np.zeros(2)
+        fn = L.pair_pot_tailcorr_switch3 if tr is not None else L.pair_pot_tailcorr_cut
+        fn(_ptr(corrs), natom, nlow, nhigh, pp)
+        L.pair_data_free(pp)
+        L.pair_pot_free(pp)
+        if tr is not None:
+            L.trunc_scheme_free(tr)
+        return float(corrs[0]), float(corrs[1])
+
     def ewald_reci(self, pos, charges, cell, alpha, gmax, gcut, dielectric=1.0, nlow=0, nhigh=-1,
                    want_gpos=True, want_vtens=True):
         pos = np.ascontiguousarray(pos, dtype=float)

@@ -12,7 +12,7 @@ import numpy as np
 from .device import SystemDevice
 
 __all__ = ['ForcePart', 'ForceField', 'ForcePartPair', 'ForcePartEwaldReciprocal', 'ForcePartEwaldCorrection',
-           'ForcePartEwaldNeutralizing']
+           'ForcePartEwaldNeutralizing', 'ForcePartTailCorrection']
 
 
 def check_nlow_nhigh(system, nlow, nhigh):
@@ -240,3 +240,27 @@ class ForcePartEwaldNeutralizing(ForcePart):
         if vtens is not None:
             vtens.ravel()[::4] -= fac
         return fac
+
+
+class ForcePartTailCorrection(ForcePart):
+    """ff.py:1012-1057: long-range corrections of a pair part's energy and
+    virial for a homogeneous system (host, O(1) per evaluation after set-up;
+    the integrals are made once by PairPot.prepare_tailcorrections)."""
+
+    def __init__(self, system, part_pair, nlow=0, nhigh=-1):
+        if system.cell.nvec != 3:
+            raise ValueError('Tail corrections can only be applied to 3D periodic systems')
+        if part_pair.name in ['pair_ei', 'pair_eidip']:
+            raise ValueError('Tail corrections are divergent for %s' % part_pair.name)
+        ForcePart.__init__(self, 'tailcorr_%s' % part_pair.name, system)
+        self.nlow, self.nhigh = check_nlow_nhigh(system, nlow, nhigh)
+        self.ecorr, self.wcorr = part_pair.pair_pot.prepare_tailcorrections(system.natom, self.nlow, self.nhigh)
+        self.system = system
+
+    def _internal_compute(self, gpos, vtens):
+        if vtens is not None:
+            w = 2.0*np.pi*self.wcorr/self.system.cell.volume
+            vtens[0, 0] += w
+            vtens[1, 1] += w
+            vtens[2, 2] += w
+        return 2.0*np.pi*self.ecorr/self.system.cell.volume

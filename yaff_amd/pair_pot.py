@@ -85,6 +85,13 @@ class PairPot(object):
     def get_truncation(self):
         return self.tr
 
+    def prepare_tailcorrections(self, natom, nlow=0, nhigh=-1):
+        """(C0, C1): the tail-correction sums of ext.pyx:665-687 (energy and
+        virial integrals over the ordered atom pairs); LJ, MM3 and ExpRep
+        with a plain cutoff or a Switch3."""
+        from .tailcorr import prepare_tailcorrections
+        return prepare_tailcorrections(self, natom, nlow, nhigh)
+
     def log(self):
         pass
 
