@@ -1,0 +1,148 @@
+"""Parameter files and the nonbonded generators (CPU: what the generators
+build, before any device work), mirroring yaff/pes/test/test_generator.py
+(water32 LJ/MM3/EXPREP/FIXQ cases, 389-531 and 761-851) on the reference's
+own parameter files (data, copied under tests/golden/params), plus the
+parameter-file syntax errors of yaff/pes/parameters.py and the .chk reader."""
+import os
+import sys
+import warnings
+
+import numpy as np
+import pytest
+
+import yaff_amd as ya
+from yaff_amd.units import angstrom, kcalmol
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PAR = os.path.join(HERE, 'golden', 'params')
+sys.path.insert(0, os.path.join(HERE, 'golden'))
+import cases  # noqa: E402
+
+
+def water32():
+    f = cases.fixture_arrays('water32')
+    return ya.System(f['numbers'], f['pos'], ffatypes=['O', 'H', 'H']*32, bonds=f['bonds'], rvecs=f['rvecs'])
+
+
+def gen(fn, system=None, **kw):
+    return ya.ForceField.generate(system or water32(), os.path.join(PAR, fn), **kw)
+
+
+def test_water32_lj():
+    ff = gen('parameters_water_lj.txt')
+    assert len(ff.parts) == 1
+    pp = ff.part_pair_lj.pair_pot
+    assert abs(pp.sigmas[0] - 3.15*angstrom) < 1e-10 and abs(pp.sigmas[1] - 0.4*angstrom) < 1e-10
+    assert abs(pp.epsilons[0] - 0.1521*kcalmol) < 1e-10 and abs(pp.epsilons[1] - 0.046*kcalmol) < 1e-10
+    # default truncation and cutoff of FFArgs (generator.py:90-91)
+    assert abs(pp.rcut - 18.89726133921252) < 1e-12 and abs(pp.tr.width - 7.558904535685008) < 1e-12
+    # scalings: 1-2 and 1-3 pairs excluded, 1-4 kept
+    st = ff.part_pair_lj.scalings.stab if hasattr(ff.part_pair_lj, 'scalings') else None
+    if st is not None:
+        assert set(np.unique(st['scale'])) <= {0.0}
+
+
+def test_water32_mm3_and_tail_corrections():
+    ff = gen('parameters_water_mm3.txt')
+    assert len(ff.parts) == 1
+    pp = ff.part_pair_mm3.pair_pot
+    assert abs(pp.sigmas[0] - 1.7*angstrom) < 1e-10 and abs(pp.sigmas[1] - 0.2*angstrom) < 1e-10
+    assert abs(pp.epsilons[0] - 0.12*kcalmol) < 1e-10 and abs(pp.epsilons[1] - 0.04*kcalmol) < 1e-10
+    assert pp.onlypaulis[0] == 1 and pp.onlypaulis[1] == 0
+    ff = gen('parameters_water_mm3.txt', tailcorrections=True)
+    assert len(ff.parts) == 2 and ff.parts[1].name == 'tailcorr_pair_mm3'
+
+
+@pytest.mark.parametrize('fn', ['parameters_water_exprep1.txt', 'parameters_water_exprep2.txt'])
+def test_water32_exprep_mixed(fn):
+    ff = gen(fn)
+    assert len(ff.parts) == 1
+    a, b = ff.part_pair_exprep.pair_pot.amp_cross, ff.part_pair_exprep.pair_pot.b_cross
+    assert (a > 0).all() and (a == a.T).all() and (b > 0).all() and (b == b.T).all()
+    assert abs(a[0, 0] - 4.2117588157e+02) < 1e-10 and abs(a[1, 1] - 2.3514195495e+00) < 1e-10
+    assert abs(b[0, 0] - 4.4661933834e+00/angstrom) < 1e-10 and abs(b[1, 1] - 4.4107388814e+00/angstrom) < 1e-10
+    if fn.endswith('1.txt'):  # plain geometric / arithmetic mixing (ext.pyx:1146-1172)
+        assert abs(a[0, 1] - np.sqrt(a[0, 0]*a[1, 1])) < 1e-10
+        assert abs(b[0, 1] - 0.5*(b[0, 0] + b[1, 1])) < 1e-10
+
+
+def test_water32_exprep_cross():
+    ff = gen('parameters_water_exprep3.txt')
+    a, b = ff.part_pair_exprep.pair_pot.amp_cross, ff.part_pair_exprep.pair_pot.b_cross
+    assert abs(a[0, 1] - 1.4360351514e+01) < 1e-10 and abs(b[0, 1] - 4.0518324069e+00/angstrom) < 1e-10
+    assert abs(a[1, 0] - a[0, 1]) == 0 and abs(b[1, 0] - b[0, 1]) == 0
+
+
+def test_water32_fixq():
+    system = water32()
+    ff = gen('parameters_water_fixq.txt', system)
+    assert [p.name for p in ff.parts] == ['pair_ei', 'ewald_reci', 'ewald_cor', 'ewald_neut']
+    ei = ff.part_pair_ei.pair_pot
+    assert ei.alpha > 0 and ei.alpha == ff.part_ewald_reci.alpha == ff.part_ewald_cor.alpha
+    assert ei.alpha == ff.part_ewald_neut.alpha
+    assert abs(ei.alpha - 3.5/18.89726133921252) < 1e-15
+    for i in range(system.natom):
+        if system.numbers[i] == 1:
+            assert abs(system.charges[i] - 0.417) < 1e-5 and abs(system.radii[i] - 1.2*angstrom) < 1e-5
+        else:
+            assert abs(system.charges[i] + 2*0.417) < 1e-5 and abs(system.radii[i] - 1.5*angstrom) < 1e-5
+
+
+def test_water32_fixq_dielectric():
+    ff = gen('parameters_water_fixq_dielectric.txt')
+    assert ff.part_pair_ei.pair_pot.dielectric == 1.44
+    assert ff.part_ewald_reci.dielectric == ff.part_ewald_cor.dielectric == ff.part_ewald_neut.dielectric == 1.44
+
+
+def test_mil53_nonbonded_sections():
+    system = ya.System.from_file(os.path.join(PAR, 'system_mil53.chk'))
+    assert system.natom == 152 and system.cell.nvec == 3 and system.nffatype == 8
+    with pytest.raises(NotImplementedError):
+        ya.ForceField.generate(system, os.path.join(PAR, 'parameters_mil53.txt'))
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter('always')
+        ff = ya.ForceField.generate(system, os.path.join(PAR, 'parameters_mil53.txt'), skip_other_sections=True)
+    left_out = sorted(str(x.message).split()[1] for x in w)
+    assert left_out == sorted(['BONDHARM', 'BENDAHARM', 'BENDCHARM', 'BENDCOS', 'TORSION', 'OOPDIST', 'CROSS'])
+    assert sorted(p.name for p in ff.parts) == sorted(['pair_mm3', 'pair_ei', 'ewald_reci', 'ewald_cor', 'ewald_neut'])
+    assert abs(system.charges.sum()) < 1e-12  # bond charge transfers conserve charge
+    assert (system.radii > 0).all()
+
+
+def test_parameter_file_errors(tmp_path):
+    from yaff_amd.parameters import Parameters
+    bad = {'no colon': 'LJ UNIT SIGMA angstrom\n', 'no data': 'LJ:UNIT\n', 'space prefix': 'L J:UNIT SIGMA a\n'}
+    for name, text in bad.items():
+        p = tmp_path / ('bad_%s.txt' % name.replace(' ', '_'))
+        p.write_text(text)
+        with pytest.raises(IOError):
+            Parameters.from_file(str(p))
+    p = tmp_path / 'wrong_nargs.txt'
+    p.write_text('LJ:UNIT SIGMA angstrom\nLJ:UNIT EPSILON kcalmol\nLJ:SCALE 1 0.0\nLJ:SCALE 2 0.0\n'
+                 'LJ:SCALE 3 1.0\nLJ:PARS O 3.15\n')
+    with pytest.raises(IOError):
+        ya.ForceField.generate(water32(), str(p))
+    p = tmp_path / 'comments.txt'
+    p.write_text('# a comment\n\nLJ:UNIT SIGMA angstrom # trailing\nlj:unit EPSILON kcalmol\nLJ:SCALE 1 0.0\n'
+                 'LJ:SCALE 2 0.0\nLJ:SCALE 3 1.0\nLJ:PARS O 3.15 0.1521\nLJ:PARS H 0.4 0.046\n')
+    ff = ya.ForceField.generate(water32(), str(p))
+    assert abs(ff.part_pair_lj.pair_pot.sigmas[0] - 3.15*angstrom) < 1e-10
+
+
+def test_units_and_chk_round_trip(tmp_path):
+    from yaff_amd.units import parse_unit
+    from yaff_amd.chk import load_chk, dump_chk
+    assert parse_unit('1/angstrom') == 1.0/angstrom
+    assert abs(parse_unit('electronvolt*angstrom**6') - ya.units.electronvolt*angstrom**6) < 1e-15
+    with pytest.raises(ValueError):
+        parse_unit('__import__("os")')
+    with pytest.raises(ValueError):
+        parse_unit('furlong')
+    d = load_chk(os.path.join(PAR, 'system_mil53.chk'))
+    assert d['pos'].shape == (152, 3) and d['charges'] is None and list(d['ffatypes'])[:2] == ['O_HY', 'O_CA']
+    out = tmp_path / 'x.chk'
+    dump_chk(str(out), {'pos': d['pos'], 'numbers': d['numbers'], 'ffatypes': d['ffatypes'], 'energy': -1.5,
+                        'n': 3, 'charges': None})
+    e = load_chk(str(out))
+    assert np.array_equal(e['pos'], d['pos']) and np.array_equal(e['numbers'], d['numbers'])
+    assert list(e['ffatypes']) == list(d['ffatypes']) and e['energy'] == -1.5 and e['n'] == 3 and e['charges'] is None

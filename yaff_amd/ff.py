@@ -87,6 +87,24 @@ class ForceField(ForcePart):
         for part in parts:
             self.add_part(part)
 
+    @classmethod
+    def generate(cls, system, parameters, skip_other_sections=False, **kwargs):
+        """ff.py:217-251: a force field from a Yaff parameter file (a
+        filename, a list of them or a Parameters instance) with the
+        nonbonded generators LJ, MM3, EXPREP and FIXQ; keywords as
+        generator.FFArgs (rcut, tr, alpha_scale, gcut_scale, skin, smooth_ei,
+        reci_ei, nlow, nhigh, tailcorrections).  Sections of Yaff's other
+        generators raise NotImplementedError unless skip_other_sections."""
+        from .generator import FFArgs, apply_generators
+        from .parameters import Parameters
+        if system.ffatype_ids is None:
+            raise ValueError('The generators needs ffatype_ids in the system object.')
+        if not isinstance(parameters, Parameters):
+            parameters = Parameters.from_file(parameters)
+        ff_args = FFArgs(**kwargs)
+        apply_generators(system, parameters, ff_args, skip_other_sections)
+        return cls(system, ff_args.parts, ff_args.nlist)
+
     def add_part(self, part):
         self.parts.append(part)
         name = 'part_%s' % part.name

@@ -41,6 +41,40 @@ class System(object):
     def natom(self):
         return len(self.pos)
 
+    @property
+    def nffatype(self):
+        return 0 if self.ffatypes is None else len(self.ffatypes)
+
+    def get_ffatype(self, index):
+        """yaff/system.py:333-345: the force-field type name of an atom."""
+        if self.ffatypes is None or self.ffatype_ids is None:
+            return None
+        return self.ffatypes[self.ffatype_ids[index]]
+
+    def iter_bonds(self):
+        """yaff/system.py:349-354"""
+        if self.bonds is not None:
+            for i0, i1 in self.bonds:
+                yield int(i0), int(i1)
+
+    @classmethod
+    def from_file(cls, *fns, **user_kwargs):
+        """yaff/system.py:405-470 for the .chk checkpoint format (later files
+        override earlier ones, keywords override files).  .h5 files need
+        h5py and .xyz/.psf files molmod, neither of which this package
+        uses: they raise IOError."""
+        from .chk import load_chk
+        allowed = ('numbers', 'pos', 'ffatypes', 'ffatype_ids', 'bonds', 'rvecs', 'charges', 'radii', 'masses')
+        kwargs = {}
+        for fn in fns:
+            if not fn.endswith('.chk'):
+                raise IOError('Can not read from file \'%s\' (only .chk is supported).' % fn)
+            for key, value in load_chk(fn).items():
+                if key in allowed:
+                    kwargs[key] = value
+        kwargs.update(user_kwargs)
+        return cls(**kwargs)
+
     def _init_derived_bonds(self):
         """yaff/system.py:199-234"""
         n = self.natom
