@@ -154,7 +154,7 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
     const ynb_sys &S = sys[wk.x];
     const int4 ch = *(const int4 *)(chunk + 4 * wk.z);
     const int rb = skblk[2 * wk.y], re = skblk[2 * wk.y + 1];
-    double f0[P], f1[P], f2[P], q[P], cr[P], ci[P], c1[P], s1[P], zr[P], zi[P], sr0[P], si0[P];
+    double f0[P], f1[P], f2[P], q[P], cr[P], ci[P], zr[P], zi[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       const int la = lane + 64 * p;
@@ -166,11 +166,8 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
       q[p] = valid ? frac[3 * npad + g] : 0.0;
       cr[p] = step[g];
       ci[p] = step[npad + g];
-      c1[p] = step[2 * npad + g];
-      s1[p] = step[3 * npad + g];
-      zr[p] = zi[p] = sr0[p] = si0[p] = 0.0;
+      zr[p] = zi[p] = 0.0;
     }
-    int4 prev = make_int4(0, 0, 0, 0);
     double *out = spart + 2 * (S.spart0 + (int64_t)(wk.z - S.chunk0) * S.nk - S.k0);
     int kt = krow_k[rb];  // first k of the current tile
     int m = 0;            // k stored in the tile
@@ -178,24 +175,14 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
     for (int r = rb; r < re; ++r) {
       const int4 row = rown;
       if (r + 1 < re) rown = *(const int4 *)(krow + 4 * (r + 1));
-      const int dx = row_step(prev, row, r > rb);
-      prev = row;
-      if (dx != 3) {
+      // Row blocks of this pass hold few rows (many work items keep the
+      // chip busy), so every row is seeded from its reduced phase.
 #pragma unroll
-        for (int p = 0; p < P; ++p) continue_seed(sr0[p], si0[p], c1[p], s1[p], cr[p], ci[p], dx);
-      } else {
-#pragma unroll
-        for (int p = 0; p < P; ++p) {  // row start: seed q exp(2 pi i g.f)
-          double sn, cs;
-          seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, f0[p], f1[p], f2[p]), &sn, &cs);
-          sr0[p] = q[p] * cs;
-          si0[p] = q[p] * sn;
-        }
-      }
-#pragma unroll
-      for (int p = 0; p < P; ++p) {
-        zr[p] = sr0[p];
-        zi[p] = si0[p];
+      for (int p = 0; p < P; ++p) {  // row start: seed q exp(2 pi i g.f)
+        double sn, cs;
+        seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, f0[p], f1[p], f2[p]), &sn, &cs);
+        zr[p] = q[p] * cs;
+        zi[p] = q[p] * sn;
       }
       for (int j = 0; j < row.w;) {
         // k up to the end of the row or of the tile: a loop without branches
