@@ -176,6 +176,7 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
   const int atom0 = S.atom0, nlow = S.nlow, nhigh = S.nhigh;
   const float lo32 = (float)(rc2 * (1.0 - 2e-4)), hi32 = (float)(rc2 * (1.0 + 2e-4));
   bool bad = false;
+  bool wide = false;  // a staged image shift outside [-255, 255] (this thread)
 
   // Stage the candidates [base, base + CAP): candidate f belongs to the
   // largest partner bin k with l_pre[k] <= f (binary search).
@@ -199,16 +200,24 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
       unpack_img(sp, s3);
       const int m0 = s3[0] - w.x, m1 = s3[1] - w.y, m2 = s3[2] - w.z;
       if (sp == INT32_MIN || m0 < -511 || m0 > 511 || m1 < -511 || m1 > 511 || m2 < -511 || m2 > 511) bad = true;
+      wide |= m0 < -255 || m0 > 255 || m1 < -255 || m1 > 255 || m2 < -255 || m2 > 255;
       l_m[q] = pack_img(m0, m1, m2);
       l_g[q] = w.w;
     }
   };
 
   const bool one_stage = T <= CAP;
+  int any_wide = 0;  // uniform: some staged shift needs the unpacked image test
   if (one_stage) {
     stage(0);
-    __syncthreads();
+    any_wide = __syncthreads_or(wide);
   }
+  // Packed images add field by field while every field stays in range:
+  // with all staged shifts and the atom's wrap counts within [-255, 255],
+  // code(m + w) = code(m) + w0 + w1 2^10 + w2 2^20 and no sum leaves
+  // [-511, 511].  The unpacked test covers the rest.
+  const bool no_groups = nlow <= 0 && nhigh >= S.natom;
+  const int32_t zero_code = pack_img(0, 0, 0);
   for (int ib = start + (int)blockIdx.y * BW; ib < end; ib += (int)gridDim.y * BW) {
     const int p = ib + wv;
     const bool active = p < end;  // uniform in the wave
@@ -235,10 +244,37 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
       if (!one_stage) {
         __syncthreads();
         stage(base);
-        __syncthreads();
+        any_wide = __syncthreads_or(wide);
       }
       if (!active) continue;
       const int lim = min(T - base, CAP);
+      const bool wi_small = wi.x >= -255 && wi.x <= 255 && wi.y >= -255 && wi.y <= 255 && wi.z >= -255 && wi.z <= 255;
+      if (!EXACT && !any_wide && wi_small) {  // uniform in the wave
+        const int32_t wlin = wi.x + (wi.y << 10) + (wi.z << 20);
+        for (int q0 = 0; q0 < lim; q0 += 64) {
+          const int q = q0 + lane;
+          const int qc = q < lim ? q : lim - 1;
+          const float dx = l_x[qc] - xr, dy = l_y[qc] - yr, dz = l_z[qc] - zr;
+          const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          const int32_t gj = l_g[qc];
+          const int32_t code = l_m[qc] + wlin;
+          bool acc = (q < lim) && (d2 < hi32) && !(gj == gi && code == zero_code);
+          if (!no_groups) {
+            const int lj = gj - atom0;
+            acc = acc && nlow_nhigh_ok(li > lj ? li : lj, li > lj ? lj : li, nlow, nhigh);
+          }
+          const uint64_t mask = __ballot(acc);
+          if (acc) {
+            const int o = cnt + lanes_below(mask);
+            if (o < nl.kcap) {
+              rowj[o] = gj;
+              rowi[o] = code;
+            }
+          }
+          cnt += __popcll(mask);
+        }
+        continue;
+      }
       for (int q0 = 0; q0 < lim; q0 += 64) {
         const int q = q0 + lane;
         const int qc = q < lim ? q : lim - 1;
