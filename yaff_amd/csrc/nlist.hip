@@ -114,6 +114,7 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
   __shared__ int32_t l_bs[NB];       // slot start of each partner bin
   __shared__ double l_sh[3][NB];     // image shift of each partner bin minus the home-bin centre
   __shared__ int32_t l_s[NB];        // packed shift (s0, s1, s2)
+  __shared__ int16_t l_bin[CAP];     // partner bin of each staged candidate
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = uniform_int(tid >> 6);
@@ -179,16 +180,19 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
   bool wide = false;  // a staged image shift outside [-255, 255] (this thread)
 
   // Stage the candidates [base, base + CAP): candidate f belongs to the
-  // largest partner bin k with l_pre[k] <= f (binary search).
+  // partner bin k with l_pre[k] <= f < l_pre[k + 1] (a table written bin by
+  // bin, then one lookup per candidate).
   auto stage = [&](int base) {
     const int lim = min(T - base, CAP);
+    // partner bin of each staged slot: thread k writes bin k over its range
+    for (int k = tid; k < nnb; k += BW * 64) {
+      const int f1 = min(l_pre[k + 1] - base, lim);
+      for (int f = max(l_pre[k] - base, 0); f < f1; ++f) l_bin[f] = (int16_t)k;
+    }
+    __syncthreads();
     for (int q = tid; q < lim; q += BW * 64) {
       const int f = base + q;
-      int lo = 0, hi = nnb;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (l_pre[mid] <= f) lo = mid; else hi = mid;
-      }
+      const int lo = l_bin[q];
       const int j = l_bs[lo] + (f - l_pre[lo]);
       const double4 x = sxw[j];
       const int4 w = sw[j];
