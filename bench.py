@@ -50,7 +50,7 @@ def parse():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-mixed', action='store_true')
-    ap.add_argument('--cpu-evals', type=int, default=2)
+    ap.add_argument('--cpu-evals', type=int, default=5)
     ap.add_argument('--no-batched', action='store_true')
     ap.add_argument('--batch-steps', type=int, default=200)
     ap.add_argument('--batch-systems', type=int, default=64)

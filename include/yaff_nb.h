@@ -189,13 +189,13 @@ typedef struct ynb_ewald {
   const int32_t *kblk_sys;/* [nk_tot/256]: system of each block of 256 k vectors */
   const int32_t *kpos;    /* [nk_tot]: force record of each k (-1: padding k) */
   const int32_t *krow_f;  /* [nrow_tot]: first force record of each row */
-  int32_t *fctr;          /* [npad/128] block counters of the force pass (zero-initialised; left at zero) */
+  int32_t *fctr;          /* [npad/64] block counters of the force pass (zero-initialised; left at zero) */
   int32_t nwork;
   int32_t nk_tot;
   int32_t apl;            /* atoms per lane of the structure-factor pass: 2, 4 or 8 */
   int32_t fslots;         /* partial-gradient slots: the largest ynb_sys.nslot */
   int32_t groups;         /* nonzero when some system has nlow > 0 or nhigh < natom */
-  int32_t fapl;           /* atoms per lane of the force pass: 2 or 4 */
+  int32_t fapl;           /* atoms per lane of the force pass: 1, 2 or 4 */
   int32_t flags;          /* YNB_EWALD_*: 0, or PREPARED when a ynb_verlet_kick_drift given this
                              struct has already filled frac and step for the current positions */
   int32_t pad_;

@@ -156,19 +156,20 @@ def test_batch_equals_alone():
             e, g, v = one.results(0)
             eb, gb, vb = res[i]
             assert e == eb and (g == gb).all() and (v == vb).all(), (precision, i)
-            one.set_tuning(pair_lanes=16)
-            if one.has_ewald:
-                one.set_tuning(apl=2, skb=256, fkb=32, fapl=2)
-            one.compute(build=False)
-            e, g, v = one.results(0)
-            de = abs(e - eb)/abs(eb)
-            dg = np.abs(g - gb).max()/np.abs(gb).max()
-            rms = np.sqrt(((g - gb)**2).sum(axis=1).mean()/(gb**2).sum(axis=1).mean())
-            print(precision, i, s.spec.kind, s.natom, 'other launch parameters: rel dE %.2e, max rel dG %.2e, '
-                  'rms rel dG %.2e' % (de, dg, rms), flush=True)
-            if precision == 'fp64':
-                # summation order only
-                assert de <= 1e-11 and dg <= 1e-11, (i, de, dg)
-            else:
-                # FP32 partial structure factors summed over other atom chunks: the mixed contract
-                assert de <= 1e-6 and rms <= 1e-5, (i, de, rms)
+            for lanes, other in ((16, dict(apl=2, skb=256, fkb=32, fapl=2)), (64, dict(apl=8, fkb=128, fapl=1))):
+                one.set_tuning(pair_lanes=lanes)
+                if one.has_ewald:
+                    one.set_tuning(**other)
+                one.compute(build=False)
+                e, g, v = one.results(0)
+                de = abs(e - eb)/abs(eb)
+                dg = np.abs(g - gb).max()/np.abs(gb).max()
+                rms = np.sqrt(((g - gb)**2).sum(axis=1).mean()/(gb**2).sum(axis=1).mean())
+                print(precision, i, s.spec.kind, s.natom, 'launch parameters %s: rel dE %.2e, max rel dG %.2e, '
+                      'rms rel dG %.2e' % (other, de, dg, rms), flush=True)
+                if precision == 'fp64':
+                    # summation order only
+                    assert de <= 1e-11 and dg <= 1e-11, (i, other, de, dg)
+                else:
+                    # FP32 partial structure factors summed over other atom chunks: the mixed contract
+                    assert de <= 1e-6 and rms <= 1e-5, (i, other, de, rms)

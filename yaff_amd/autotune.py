@@ -1,8 +1,8 @@
 """Autotuner of the launch parameters of a batch.
 
 Parameters: lanes per atom of the pair kernel (16/32/64), atoms per lane of
-the structure-factor pass (2/4/8) and the k vectors per row block of the
-structure-factor and force passes.  Each
+the structure-factor pass (2/4/8) and of the force pass (1/2/4), and the k
+vectors per row block of the structure-factor and force passes.  Each
 parameter affects a known set of kernels; coordinate descent times only
 those kernels (HIP events on the launch stream, median of a few runs) with
 the other parameters fixed.  Results are cached in a JSON file keyed by
@@ -22,7 +22,7 @@ CANDIDATES = [
     ('pair_lanes', (16, 32, 64), ('pair',)),
     ('apl', (2, 4, 8), ('ewald_sk', 'ewald_kreduce')),
     ('skb', None, ('ewald_sk', 'ewald_kreduce')),
-    ('fapl', (2, 4), ('ewald_force',)),
+    ('fapl', (1, 2, 4), ('ewald_force',)),
     ('fkb', None, ('ewald_force',)),
 ]
 

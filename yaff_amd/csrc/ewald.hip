@@ -687,7 +687,7 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
                               int32_t precision, double *gpos, int32_t accumulate, double *out, ynb_stream stream) {
   if (!b || !ew || !pos || !charges || !out || !ew->fctr || b->npad % 256 != 0 || ew->nk_tot % 256 != 0 ||
       ew->fslots < 0 ||
-      (ew->apl != 2 && ew->apl != 4 && ew->apl != 8) || (ew->fapl != 2 && ew->fapl != 4) ||
+      (ew->apl != 2 && ew->apl != 4 && ew->apl != 8) || (ew->fapl != 1 && ew->fapl != 2 && ew->fapl != 4) ||
       (precision != YNB_FP64 && precision != YNB_MIXED)) {
     set_error("ynb_ewald_reci: invalid arguments");
     return YNB_EINVAL;
@@ -734,9 +734,9 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
              accumulate, ew->kpart, out)
 #define YNB_FORCE_P(PR)                                            \
   if (ew->groups) {                                                \
-    if (P == 2) YNB_FORCE_G(PR, 2); else YNB_FORCE_G(PR, 4);       \
+    if (P == 1) YNB_FORCE_G(PR, 1); else if (P == 2) YNB_FORCE_G(PR, 2); else YNB_FORCE_G(PR, 4); \
   } else {                                                         \
-    if (P == 2) YNB_FORCE(PR, 2); else YNB_FORCE(PR, 4);           \
+    if (P == 1) YNB_FORCE(PR, 1); else if (P == 2) YNB_FORCE(PR, 2); else YNB_FORCE(PR, 4); \
   }
     if (precision == YNB_FP64) {
       YNB_FORCE_P(YNB_FP64);

@@ -319,8 +319,8 @@ class Engine(object):
             raise ValueError('apl must be 2, 4 or 8')
         self.apl = apl
         fapl = self.fapl if fapl is None else int(fapl)
-        if fapl not in (2, 4):
-            raise ValueError('fapl must be 2 or 4')
+        if fapl not in (1, 2, 4):
+            raise ValueError('fapl must be 1, 2 or 4')
         self.fapl = fapl
         nks = {i: len(ewald_kvectors_cached(self.systems[i])[1]) for i in ew_sys}
         if skb is None:
@@ -436,8 +436,8 @@ class Engine(object):
         self.stab_d = self._t(np.frombuffer(self.stab_h.tobytes(), np.uint8))
         self.rec_d = self._t(self.rec_h)
         # block counters of the last-block reductions (zero; kernels leave them at zero):
-        # [nsys] pair, [nsys] unused, [nsys] kick, [npad/128] Ewald force blocks
-        self.ctr_d = torch.zeros(3*self.nsys + self.npad//128 + 1, dtype=torch.int32, device=dev)
+        # [nsys] pair, [nsys] unused, [nsys] kick, [npad/64] Ewald force blocks
+        self.ctr_d = torch.zeros(3*self.nsys + self.npad//64 + 1, dtype=torch.int32, device=dev)
         self.pairpar = YnbPairpar(self.par_d.data_ptr(), self.cross_d.data_ptr(), self.stab_d.data_ptr(),
                                   self.rec_d.data_ptr(), self._ctr(0), self.pair_lanes, 0)
         # neighbour list
