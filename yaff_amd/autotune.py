@@ -4,7 +4,7 @@ Parameters: lanes per atom of the pair kernel (16/32/64), atoms per lane of
 the structure-factor pass (2/4/8) and of the force pass (1/2/4), and the k
 vectors per row block of the structure-factor and force passes.  Each
 parameter affects a known set of kernels; coordinate descent times only
-those kernels (HIP events on the launch stream, median of a few runs) with
+those kernels (HIP events on the launch stream, minimum of a few runs) with
 the other parameters fixed.  Results are cached in a JSON file keyed by
 architecture, precision, potential mix and size bucket.
 """
@@ -41,9 +41,13 @@ def cache_key(engine):
                                  int(round(math.log2(max(natom, 1)))), int(round(math.log2(max(engine.nsys, 1)))))
 
 
-def _time_kernels(engine, names, reps=3):
-    """Median over reps of the summed duration of the named kernels in one
-    force evaluation (the list is built once, outside the timing)."""
+def _time_kernels(engine, names, reps=5):
+    """Minimum over reps of the summed duration of the named kernels in one
+    force evaluation, after one untimed evaluation (a kernel's first launch
+    loads its code); the list is built once, outside the timing.  Timing
+    noise only adds time, so the minimum is the steadier estimate."""
+    engine.pair(gpos=True, accumulate=False)
+    engine.ewald_reci(gpos=True, accumulate=True)
     times = []
     for _ in range(reps):
         torch.cuda.synchronize()
@@ -53,8 +57,7 @@ def _time_kernels(engine, names, reps=3):
         prof = _lib.profile_read()
         _lib.profile_enable(False)
         times.append(sum(ms for n, ms in prof if n in names))
-    times.sort()
-    return times[len(times)//2]
+    return min(times)
 
 
 def tune(engine, cache=DEFAULT_CACHE, verbose=False):

@@ -502,6 +502,7 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
   const double rc1 = FORM == YNB_FORM_IONIC ? S.pot[1].rcut : 0.0;
   const double rc2 = FORM == YNB_FORM_IONIC ? S.pot[2].rcut : 0.0;
   const double rcmax = fmax(rc0, fmax(rc1, rc2));
+  const double tol0 = 1e-12 * rc0, tol1 = 1e-12 * rc1, tol2 = 1e-12 * rc2;
   const double alpha = FORM == YNB_FORM_IONIC ? S.pot[2].alpha : 0.0;
   const int trunc0 = S.pot[0].trunc;
   const double w0 = S.pot[0].trunc_par;
@@ -573,8 +574,10 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
       const double r2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
       double inv = rsqrt(r2);
       double dd = r2 * inv;
-      bool near_cut = false;
-      for (int p = 0; p < npot; ++p) near_cut |= fabs(dd - S.pot[p].rcut) <= 1e-12 * S.pot[p].rcut;
+      // the form's cutoffs are uniform registers (a loop over S.pot here
+      // would be a scalar load and wait per pair)
+      bool near_cut = fabs(dd - rc0) <= tol0;
+      if (FORM == YNB_FORM_IONIC) near_cut = near_cut || fabs(dd - rc1) <= tol1 || fabs(dd - rc2) <= tol2;
       if (near_cut) {
         // Yaff's own row vector decides the cutoff tests (no scaling rows in a form)
         const int lj = (int)(gj - S.atom0);
