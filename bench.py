@@ -201,27 +201,56 @@ def cpu_baseline(s, evals):
     return n*evals/t, t, res
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch (read, written) of the FP64 instance of `kernel`
+# bench kernel names -> rocprofv3 kernel names (the FP64 instance's template
+# arguments follow from the launch parameters)
+ROCPROF_NAMES = {'nlist_bin': 'k_bin', 'nlist_bin_order': 'k_bin_order', 'nlist_build_binned': 'k_build_binned',
+                 'pair': 'k_pair_form', 'ewald_prep': 'k_ewald_prep', 'ewald_sk': 'k_ewald_sk',
+                 'ewald_kreduce': 'k_ewald_kreduce', 'ewald_force': 'k_ewald_force', 'verlet_kick_drift': 'k_kick_drift',
+                 'verlet_kick': 'k_kick', 'total': 'k_total'}
+
+
+def fp64_instance(kernel, tuning):
+    """rocprofv3 name of the FP64 instance that the launch parameters select
+    (None where the name has no template arguments or they are not known)."""
+    base = ROCPROF_NAMES.get(kernel)
+    if base is None:
+        return None
+    if kernel == 'pair':
+        return '%s<0, 2, %d>' % (base, tuning['pair_lanes'])
+    if kernel == 'ewald_sk':
+        return '%s<0, %d>' % (base, tuning['apl'])
+    if kernel == 'ewald_force':
+        return '%s<0, %d>' % (base, tuning['fapl'])
+    if kernel == 'nlist_build_binned':
+        return '%s<false, 128>' % base
+    return base
+
+
+def profiled_traffic(kernel, tuning=None):
+    """HBM bytes per launch (read, written, rocprofv3 name) of `kernel`
     (bench name, e.g. 'ewald_force') from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary of the 6912-atom rocksalt step (counters
-    scaled by a copy kernel of known size); None when absent."""
+    scaled by a copy kernel of known size): the instance the launch
+    parameters select when the summary has it, else its FP64 instance of
+    other launch parameters (the name says which); None when absent."""
     path = os.path.join(ROOT, 'profiles', 'rocksalt6912_hbm_traffic.md')
-    if not os.path.exists(path):
+    base = ROCPROF_NAMES.get(kernel)
+    if base is None or not os.path.exists(path):
         return None
     text = open(path).read()
     if 'Machine-readable' not in text:
         return None
     block = text.split('Machine-readable', 1)[1].split('```')[1]
+    rows = {}
     for line in block.strip().splitlines():
-        parts = line.rsplit(' ', 2)
-        name = parts[0]
-        base = name.split('<')[0]
-        if base != 'k_' + kernel:
-            continue
-        if '<' in name and not name.split('<')[1].startswith('0'):
-            continue
-        return float(parts[1]), float(parts[2])
+        name, rd, wr = line.rsplit(' ', 2)
+        rows[name] = (float(rd), float(wr), name)
+    want = fp64_instance(kernel, tuning) if tuning else None
+    if want in rows:
+        return rows[want]
+    for name, row in rows.items():
+        if name.split('<')[0] == base and ('<' not in name or name.split('<')[1][0] in '0f'):
+            return row
     return None
 
 
@@ -455,13 +484,22 @@ def main():
         if k in algo:
             v['achieved_tflops'] = algo[k]/(v['avg_ms']*1e-3)/1e12
             v['frac_fp64_peak'] = v['achieved_tflops']/FP64_PEAK_TFLOPS
+        # achieved HBM rate: the profiled bytes per launch over this run's
+        # average launch duration
+        tr = profiled_traffic(k, tuning)
+        if tr:
+            v['hbm_bytes_per_launch'] = tr[0] + tr[1]
+            v['hbm_gbs'] = (tr[0] + tr[1])/(v['avg_ms']*1e-3)/1e9
+            v['frac_hbm_peak'] = v['hbm_gbs']/HBM_PEAK_GBS
+            v['hbm_profiled_as'] = tr[2]
     if dom in algo:
         ach = algo[dom]/(kt[dom]['avg_ms']*1e-3)/1e12
-        tr = profiled_traffic(dom)
+        tr = profiled_traffic(dom, tuning)
         roofline = {'kernel': dom, 'bound': 'valu', 'achieved': ach, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                     'frac': ach/FP64_PEAK_TFLOPS, 'traffic': (tr[0] + tr[1]) if tr else None,
-                    'traffic_source': 'profiles/rocksalt6912_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE '
-                                      'per launch, scaled by a 512 MiB copy kernel)' if tr else None,
+                    'traffic_source': ('profiles/rocksalt6912_hbm_traffic.md, %s (rocprofv3 FETCH_SIZE + '
+                                       'WRITE_SIZE per launch, scaled by a 512 MiB copy kernel)' % tr[2])
+                    if tr else None,
                     'algorithmic_flop_per_launch': algo[dom]}
     else:
         roofline = {'kernel': dom, 'bound': 'hbm', 'achieved': None, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
