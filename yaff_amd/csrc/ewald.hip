@@ -486,13 +486,17 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
         rp += 8;
         na = *(const double4 *)rp;
         nb = *(const double4 *)(rp + 4);
+        // keeps the request ahead of this group's arithmetic (the scheduler
+        // would otherwise sink it into the group, shortening its cover)
+        __builtin_amdgcn_sched_barrier(0);
         const double cs[4] = {ca.x, ca.z, cb.x, cb.z}, cc[4] = {ca.y, ca.w, cb.y, cb.w};
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
 #pragma unroll
           for (int p = 0; p < P; ++p) {
-            const double x = cs[m] * zr[p] - cc[m] * zi[p];
-            T[p] += x;
+            // T += cs zr - cc zi as two fused steps (7 FP64 instructions
+            // per (k, atom) with the running sum and the complex step)
+            T[p] = fma(cs[m], zr[p], fma(-cc[m], zi[p], T[p]));
             U[p] += T[p];
             const double nr = zr[p] * cr[p] - zi[p] * ci[p];
             zi[p] = zr[p] * ci[p] + zi[p] * cr[p];
