@@ -119,14 +119,17 @@ def run_steps(eng, K):
 
 
 def timed(eng, K, profile=True):
-    """K eager MD steps; per-kernel HIP events when profile."""
+    """K eager MD steps issued as the graphs of timed_graph hold them (K/S
+    sequences of S joined steps); per-kernel HIP events when profile."""
     import torch
     from yaff_amd import _lib
     torch.cuda.synchronize()
     if profile:
         _lib.profile_enable(True)
     t0 = time.perf_counter()
-    run_steps(eng, K)
+    S = graph_steps(K)
+    for _ in range(K//S):
+        eng.md_steps(S)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof = _lib.profile_read() if profile else []
@@ -206,7 +209,7 @@ def cpu_baseline(s, evals):
 ROCPROF_NAMES = {'nlist_bin': 'k_bin', 'nlist_bin_order': 'k_bin_order', 'nlist_build_binned': 'k_build_binned',
                  'pair': 'k_pair_form', 'ewald_prep': 'k_ewald_prep', 'ewald_sk': 'k_ewald_sk',
                  'ewald_kreduce': 'k_ewald_kreduce', 'ewald_force': 'k_ewald_force', 'verlet_kick_drift': 'k_kick_drift',
-                 'verlet_kick': 'k_kick', 'total': 'k_total'}
+                 'verlet_kick': 'k_kick', 'verlet_kick_kick_drift': 'k_kick_kick_drift', 'total': 'k_total'}
 
 
 def fp64_instance(kernel, tuning):
@@ -371,8 +374,7 @@ def batched(nsys, steps, warmup):
         from yaff_amd import _lib
         torch.cuda.synchronize()
         _lib.profile_enable(True)
-        for _ in range(3):
-            md.engine.md_step()
+        md.engine.md_steps(3)
         kt = kernel_table(_lib.profile_read(), 3)
         _lib.profile_enable(False)
         progress('batched %s %.3g atom-evals/s' % (prec, natom*nst/dt))
@@ -391,8 +393,7 @@ def batched(nsys, steps, warmup):
     from yaff_amd import _lib
     torch.cuda.synchronize()
     _lib.profile_enable(True)
-    for _ in range(10):
-        md.engine.md_step()
+    md.engine.md_steps(10)
     kt = kernel_table(_lib.profile_read(), 10)
     _lib.profile_enable(False)
     out['batched_mixed_skin1A'] = {'value': natom*steps/dt, 'ms_per_step': dt/steps*1e3, 'steps': steps,
@@ -515,7 +516,7 @@ def main():
                       'tuning': tuning},
            'roofline': roofline, 'kernels': kt,
            'timing': 'value: K MD steps as K/S replays of a HIP graph holding S = %d steps; kernels: HIP events '
-                     'around each launch of K eager steps on the launch stream' % graph_steps(K),
+                     'around each launch of the same K steps issued eagerly on the launch stream' % graph_steps(K),
            'eager': {'value': n*K/dt_eager, 'ms_per_step': dt_eager/K*1e3}}
 
     # mixed precision on the same system: rate and error against FP64

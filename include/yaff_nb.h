@@ -326,6 +326,17 @@ int ynb_verlet_kick(const ynb_batch *b, double *vel, double *gpos, const double 
                     const double *minv, const double *masses, const double *dt, double *partial,
                     int32_t *ctr, double *ekin2, const double *pair_out, const double *ewald_out,
                     double *total, ynb_stream stream);
+/* ynb_verlet_kick followed by ynb_verlet_kick_drift (the end of one step and
+ * the start of the next, verlet.py:354-355 then 344-347) in one launch, with
+ * the arguments of both; every output is bitwise what the two calls give
+ * (2*Ekin from the velocity after the first half kick).  One kernel boundary
+ * fewer per step of a multi-step sequence. */
+int ynb_verlet_kick_kick_drift(const ynb_batch *b, double *pos, double *vel, double *gpos,
+                               const double *gpart, const double *minv, const double *masses,
+                               const double *dt, double *partial, int32_t *ctr, double *ekin2,
+                               const double *pair_out, const double *ewald_out, double *total,
+                               const ynb_ewald *ew, const double *charges, void *nl_work,
+                               size_t nl_work_bytes, const ynb_nlist *nl, ynb_stream stream);
 
 /* out[s][k] = sum over slots of in[s][slot][k] for k < 10, plus optional
  * extra [nsys][10] terms; used to total a force-field evaluation on device. */

@@ -156,3 +156,56 @@ def test_skin_graph_replay_matches_eager():
     assert ea.nlist_status()[0] == 0 and eb.nlist_status()[0] == 0
     assert torch.equal(ea.pos_d, eb.pos_d) and torch.equal(ea.vel_d, eb.vel_d)
     assert torch.equal(ea.total_d, eb.total_d)
+
+
+@pytest.mark.parametrize('kind', ['ionic', 'batch'])
+def test_joined_steps_match_single_steps(kind):
+    """md_steps(n) joins the kick ending each step with the kick and drift
+    starting the next in one kernel: positions, velocities, gradients,
+    energy totals and kinetic energies bit for bit those of n md_step calls,
+    eagerly and replayed from a graph."""
+    import torch
+    from bench import maxwell_boltzmann
+    from yaff_amd import workloads
+    from yaff_amd.batch import nbsystem
+    from yaff_amd.engine import Engine
+    from yaff_amd.units import femtosecond
+    print('case joined steps, %s' % kind, flush=True)
+    if kind == 'ionic':
+        s, eng = _ionic_engine()
+        systems = [s]
+    else:
+        systems = workloads.batch_of_systems(5, (150, 700), (80.0, 100.0), seed=6)
+        eng = Engine([nbsystem(x) for x in systems], precision='mixed')
+    eng.ensure_nlist()
+    eng.compute(build=False)
+    eng.set_velocities([maxwell_boltzmann(x.masses, 300.0, 2 + i) for i, x in enumerate(systems)])
+    eng.set_timestep(1.0*femtosecond)
+    eng.md_step()
+    torch.cuda.synchronize()
+    assert eng.nlist_status()[0] == 0
+    bufs = (eng.pos_d, eng.vel_d, eng.gpos_d, eng.total_d, eng.ekin2_d)
+    state = [t.clone() for t in bufs]
+
+    def restore():
+        torch.cuda.synchronize()
+        for t, v in zip(bufs, state):
+            t.copy_(v)
+        torch.cuda.synchronize()
+
+    for _ in range(4):
+        eng.md_step()
+    ref = [t.clone() for t in bufs]
+    restore()
+    eng.md_steps(4)
+    torch.cuda.synchronize()
+    for i, (t, r) in enumerate(zip(bufs, ref)):
+        assert torch.equal(t, r), i
+    restore()
+    g = eng.capture_md(4)
+    restore()
+    g.replay()
+    torch.cuda.synchronize()
+    for i, (t, r) in enumerate(zip(bufs, ref)):
+        assert torch.equal(t, r), ('graph', i)
+    assert eng.nlist_status()[0] == 0

@@ -76,6 +76,7 @@ class YnbEwald(C.Structure):
 EXPORTS = ['ynb_version', 'ynb_last_error', 'ynb_sys_plan', 'ynb_nlist_workspace_bytes',
            'ynb_rows_workspace_bytes', 'ynb_nlist_build', 'ynb_nlist_rows', 'ynb_nlist_recompute',
            'ynb_pair_compute', 'ynb_pair_rows', 'ynb_ewald_reci', 'ynb_ewald_corr', 'ynb_verlet_kick_drift', 'ynb_verlet_kick',
+           'ynb_verlet_kick_kick_drift',
            'ynb_total', 'ynb_profile_enable', 'ynb_profile_count', 'ynb_profile_read',
            'ynb_nlist_skin_check']
 
@@ -114,6 +115,7 @@ def load(path=LIB_PATH):
     L.ynb_ewald_corr.argtypes = [P(YnbSys), vp, vp, vp, vp, i64, vp, i32, vp, vp]
     L.ynb_verlet_kick_drift.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp, P(YnbEwald), vp, vp, sz, P(YnbNlist), vp]
     L.ynb_verlet_kick.argtypes = [P(YnbBatch)] + [vp]*13
+    L.ynb_verlet_kick_kick_drift.argtypes = [P(YnbBatch)] + [vp]*13 + [P(YnbEwald), vp, vp, sz, P(YnbNlist), vp]
     L.ynb_total.argtypes = [P(YnbBatch), vp, vp, vp, vp]
     L.ynb_profile_enable.argtypes = [C.c_int]
     L.ynb_profile_read.argtypes = [C.c_int, P(C.c_char_p), P(C.c_float)]

@@ -149,6 +149,86 @@ __global__ __launch_bounds__(192) void k_kick(const ynb_sys *sys, const int32_t 
   }
 }
 
+// The kick that ends one step and the kick and drift that start the next
+// (verlet.py:354-355, then 344-347), one kernel: the k_kick block layout and
+// arithmetic (gradient slots summed into gpos, vel += dt/2 acc, 2 Ekin by the
+// system's last block, energy/virial totals), then vel += dt/2 acc once more
+// and pos += dt vel with the k_kick_drift expressions, so every value is
+// bitwise the one the two kernels give; then, per atom (the block's new
+// positions through LDS), the Ewald set-up and the binning of k_kick_drift.
+__global__ __launch_bounds__(192) void k_kick_kick_drift(
+    const ynb_sys *sys, const int32_t *wave_sys, int64_t npad, double *pos, double *vel, double *gpos,
+    const double *gpart, const double *minv, const double *masses, const double *dt, double *partial, int32_t *ctr,
+    double *ekin2, const double *pair_out, const double *ewald_out, double *total, const double *charges, double *frac,
+    double *step, int32_t cb, int32_t *bin_count, int32_t *slot_idx, double4 *xw, int4 *wr, int32_t *status) {
+  const int64_t blk = blockIdx.x;
+  const int sid = wave_sys[blk];  // the same for the whole block
+  const ynb_sys *S = sid >= 0 ? &sys[sid] : nullptr;
+  const int t = threadIdx.x;
+  const int64_t g = blk * 64 + t / 3;
+  const int64_t e = blk * 192 + t;  // = 3 g + c
+  __shared__ double l_p[192];
+  __shared__ double l_w[3];
+  l_p[t] = 0.0;
+  if (S) {
+    double e2 = 0.0;
+    if (g - S->atom0 < S->natom) {
+      double gr = gpos[e];
+      const int ns = S->nslot;
+      if (gpart && ns > 0) {
+        const double *src = gpart + e;
+        const int64_t stride = 3 * npad;
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        int sl = 0;
+        for (; sl + 8 <= ns; sl += 8) {
+          double v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) v[u] = src[(int64_t)(sl + u) * stride];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) a[u & 3] += v[u];
+        }
+        for (; sl < ns; ++sl) a[sl & 3] += src[(int64_t)sl * stride];
+        gr += (a[0] + a[1]) + (a[2] + a[3]);
+        gpos[e] = gr;
+      }
+      const double h = dt[sid];
+      const double mi = minv[g];
+      const double v = vel[e] + 0.5 * (-gr * mi) * h;  // end of this step
+      e2 = masses[g] * v * v;
+      const double a = -gr * mi;
+      const double v2 = v + 0.5 * a * h;  // start of the next
+      vel[e] = v2;
+      const double p = pos[e] + h * v2;
+      pos[e] = p;
+      l_p[t] = p;
+    }
+    e2 = wave_sum(e2);
+    if ((t & 63) == 0) l_w[t >> 6] = e2;
+    __syncthreads();
+    if (t == 0) publish(partial + blk, (l_w[0] + l_w[1]) + l_w[2]);
+    const int nb = S->natom > 0 ? (S->natom + 63) / 64 : 1;
+    if (last_block(ctr + sid, nb)) {
+      const int64_t b0 = S->atom0 / 64;
+      double acc = 0.0;
+      for (int w = t; w < nb; w += 192) acc += gather(partial + b0 + w);
+      acc = wave_sum(acc);
+      __syncthreads();
+      if ((t & 63) == 0) l_w[t >> 6] = acc;
+      __syncthreads();
+      if (t == 0) ekin2[sid] = (l_w[0] + l_w[1]) + l_w[2];
+      if (total && t < 10) system_total(*S, sid, t, pair_out, ewald_out, total);
+    }
+  }
+  __syncthreads();
+  if (t < 64) {
+    const int64_t ga = blk * 64 + t;
+    const bool active = S && ga - S->atom0 < S->natom;
+    const double p0 = l_p[3 * t], p1 = l_p[3 * t + 1], p2 = l_p[3 * t + 2];
+    if (frac) ewald_prep_atom(S, active, ga, npad, p0, p1, p2, charges, frac, step);
+    if (bin_count && S) bin_atom(*S, active, ga, p0, p1, p2, cb, bin_count, slot_idx, xw, wr, status);
+  }
+}
+
 __global__ void k_total(const ynb_sys *sys, int nsys, const double *pair_out, const double *ewald_out,
                         double *total) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -225,6 +305,34 @@ int ynb_verlet_kick(const ynb_batch *b, double *vel, double *gpos, const double 
   hipStream_t st = (hipStream_t)stream;
   YNB_LAUNCH("verlet_kick", k_kick, dim3((unsigned)(b->npad / 64)), dim3(192), 0, st, b->sys, b->wave_sys, b->npad,
              vel, gpos, gpart, minv, masses, dt, partial, ctr, ekin2, pair_out, ewald_out, total);
+  YNB_LAUNCHED();
+  return YNB_OK;
+}
+
+int ynb_verlet_kick_kick_drift(const ynb_batch *b, double *pos, double *vel, double *gpos, const double *gpart,
+                               const double *minv, const double *masses, const double *dt, double *partial,
+                               int32_t *ctr, double *ekin2, const double *pair_out, const double *ewald_out,
+                               double *total, const ynb_ewald *ew, const double *charges, void *nl_work,
+                               size_t nl_work_bytes, const ynb_nlist *nl, ynb_stream stream) {
+  if (!b || !pos || !vel || !gpos || !minv || !masses || !dt || !partial || !ctr || !ekin2 || b->npad % 256 != 0 ||
+      (ew && (!charges || !ew->frac || !ew->step)) ||
+      (nl_work && (!nl || nl->rebuild || b->max_bin_atoms < 1 || b->max_bin_atoms > 1024))) {
+    set_error("ynb_verlet_kick_kick_drift: invalid arguments");
+    return YNB_EINVAL;
+  }
+  Work w{};
+  const bool bin = nl_work && b->nbin > 0;
+  if (bin) {
+    w = carve(nl_work, b->npad, b->nbin, b->max_bin_atoms);
+    if (nl_work_bytes < w.total) {
+      set_error("ynb_verlet_kick_kick_drift: workspace of %zu bytes, need %zu", nl_work_bytes, w.total);
+      return YNB_EWORKSPACE;
+    }
+  }
+  YNB_LAUNCH("verlet_kick_kick_drift", k_kick_kick_drift, dim3((unsigned)(b->npad / 64)), dim3(192), 0,
+             (hipStream_t)stream, b->sys, b->wave_sys, b->npad, pos, vel, gpos, gpart, minv, masses, dt, partial, ctr,
+             ekin2, pair_out, ewald_out, total, charges, ew ? ew->frac : nullptr, ew ? ew->step : nullptr,
+             b->max_bin_atoms, bin ? w.bin_count : nullptr, w.slot_idx, w.xw, w.wr, bin ? nl->status : nullptr);
   YNB_LAUNCHED();
   return YNB_OK;
 }

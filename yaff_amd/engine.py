@@ -722,18 +722,36 @@ class Engine(object):
     def md_step(self):
         """kick, drift, neighbour list, forces, kick: one Verlet step of every
         system, on device, without synchronisation."""
-        st = self.stream()
+        self.md_steps(1)
+
+    def md_steps(self, nsteps):
+        """nsteps Verlet steps of every system, on device, without
+        synchronisation.  The kick that ends a step and the kick and drift
+        that start the next are one kernel (bitwise the same values as
+        md_step repeated); the sequence starts with a kick and drift and ends
+        with a kick, so the state between calls is that of md_step."""
+        self._md_kick_drift()
+        for k in range(nsteps):
+            self._md_forces()
+            self._md_kick(join=k + 1 < nsteps)
+
+    def _md_prebin(self):
         # the drift also prepares the Ewald pass and bins the atoms for the
         # list build (not with a skin: the skin test decides which systems
         # are rebuilt, after the drift)
-        prebin = not self.use_skin and self.batch.nbin > 0
+        return not self.use_skin and self.batch.nbin > 0
+
+    def _md_kick_drift(self):
+        prebin = self._md_prebin()
         ew = self.ewald if self.has_ewald else None
         check(self.lib.ynb_verlet_kick_drift(self.batch, self.pos_d.data_ptr(), self.vel_d.data_ptr(),
                                              self.gpos_d.data_ptr(), self.minv_d.data_ptr(), self.dt_d.data_ptr(),
                                              ew, self.charges_d.data_ptr(),
                                              self.nl_work_d.data_ptr() if prebin else None, self.nl_work_bytes,
-                                             self.nlist if prebin else None, st))
-        if prebin:
+                                             self.nlist if prebin else None, self.stream()))
+
+    def _md_forces(self):
+        if self._md_prebin():
             self.nlist.flags = _lib.NLIST_PREBINNED
             try:
                 self.build_nlist(force=False)
@@ -744,20 +762,32 @@ class Engine(object):
         self.pair(gpos=True, accumulate=False)
         # the Ewald partial gradients stay in their slots and are summed by
         # the kick, which leaves the total gradient in gpos_d ...
-        if ew is not None:
+        if self.has_ewald:
+            ew = self.ewald
             ew.flags = _lib.EWALD_PREPARED
             try:
                 self.ewald_reci(gpos=False)
             finally:
                 ew.flags = 0
-        # ... and the kick also totals energy and virial (in place of total())
-        check(self.lib.ynb_verlet_kick(self.batch, self.vel_d.data_ptr(), self.gpos_d.data_ptr(),
-                                       self.ew_gpart_d.data_ptr() if self.has_ewald else None,
-                                       self.minv_d.data_ptr(), self.masses_d.data_ptr(), self.dt_d.data_ptr(),
-                                       self.md_partial_d.data_ptr(), self._ctr(2*self.nsys), self.ekin2_d.data_ptr(),
-                                       self.pair_out_d.data_ptr(),
-                                       self.ewald_out_d.data_ptr() if self.has_ewald else None,
-                                       self.total_d.data_ptr(), st))
+
+    def _md_kick(self, join=False):
+        # ... and the kick also totals energy and virial (in place of
+        # total()); with join, the next step's kick and drift follow in the
+        # same kernel
+        common = (self.vel_d.data_ptr(), self.gpos_d.data_ptr(),
+                  self.ew_gpart_d.data_ptr() if self.has_ewald else None,
+                  self.minv_d.data_ptr(), self.masses_d.data_ptr(), self.dt_d.data_ptr(),
+                  self.md_partial_d.data_ptr(), self._ctr(2*self.nsys), self.ekin2_d.data_ptr(),
+                  self.pair_out_d.data_ptr(), self.ewald_out_d.data_ptr() if self.has_ewald else None,
+                  self.total_d.data_ptr())
+        if not join:
+            check(self.lib.ynb_verlet_kick(self.batch, *common, self.stream()))
+            return
+        prebin = self._md_prebin()
+        check(self.lib.ynb_verlet_kick_kick_drift(self.batch, self.pos_d.data_ptr(), *common,
+                                                  self.ewald if self.has_ewald else None, self.charges_d.data_ptr(),
+                                                  self.nl_work_d.data_ptr() if prebin else None, self.nl_work_bytes,
+                                                  self.nlist if prebin else None, self.stream()))
 
     def capture_md(self, nsteps=1):
         """Captures nsteps Verlet steps of every system in a HIP graph (all
@@ -772,8 +802,7 @@ class Engine(object):
         torch.cuda.current_stream().wait_stream(side)
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
-            for _ in range(nsteps):
-                self.md_step()
+            self.md_steps(nsteps)
         return graph
 
     # ---- host views ---------------------------------------------------
