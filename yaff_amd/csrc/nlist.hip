@@ -11,6 +11,8 @@
 // |r_i| <= rmax_i that limits Yaff in skewed cells.
 #include <hipcub/hipcub.hpp>
 
+#include <type_traits>
+
 #include "common.h"
 #include "nlist_work.h"
 
@@ -255,27 +257,46 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
       const bool wi_small = wi.x >= -255 && wi.x <= 255 && wi.y >= -255 && wi.y <= 255 && wi.z >= -255 && wi.z <= 255;
       if (!EXACT && !any_wide && wi_small) {  // uniform in the wave
         const int32_t wlin = wi.x + (wi.y << 10) + (wi.z << 20);
-        for (int q0 = 0; q0 < lim; q0 += 64) {
+        // the atom index and its row as wave-uniform values (the row is
+        // written with a scalar base and 32-bit offsets)
+        const int32_t gi_u = __builtin_amdgcn_readfirstlane((int32_t)gi);
+        int32_t *const rj = nl.j + (int64_t)gi_u * nl.kcap;
+        int32_t *const ri = nl.img + (int64_t)gi_u * nl.kcap;
+        // one step of 64 candidates; TAIL: the last, partial step; GROUPS:
+        // the nlow/nhigh test is needed
+        auto scan_step = [&](int q0, auto tail_c, auto groups_c) {
+          constexpr bool TAIL = decltype(tail_c)::value, GROUPS = decltype(groups_c)::value;
           const int q = q0 + lane;
-          const int qc = q < lim ? q : lim - 1;
+          const int qc = TAIL ? min(q, lim - 1) : q;
           const float dx = l_x[qc] - xr, dy = l_y[qc] - yr, dz = l_z[qc] - zr;
           const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
           const int32_t gj = l_g[qc];
           const int32_t code = l_m[qc] + wlin;
-          bool acc = (q < lim) && (d2 < hi32) && !(gj == gi && code == zero_code);
-          if (!no_groups) {
+          bool acc = (d2 < hi32) & ((gj != gi_u) | (code != zero_code));
+          if (TAIL) acc = acc & (q < lim);
+          if (GROUPS) {
             const int lj = gj - atom0;
             acc = acc && nlow_nhigh_ok(li > lj ? li : lj, li > lj ? lj : li, nlow, nhigh);
           }
           const uint64_t mask = __ballot(acc);
-          if (acc) {
-            const int o = cnt + lanes_below(mask);
-            if (o < nl.kcap) {
-              rowj[o] = gj;
-              rowi[o] = code;
-            }
+          // with room for a whole step in the row (uniform) no lane can pass
+          // the capacity
+          const bool room = cnt + 64 <= nl.kcap;
+          const uint32_t o = (uint32_t)(cnt + lanes_below(mask));
+          if (acc && (room || o < (uint32_t)nl.kcap)) {
+            const uint32_t ob = o * 4u;  // bytes; a row is far below 4 GiB
+            *(int32_t *)((char *)rj + ob) = gj;
+            *(int32_t *)((char *)ri + ob) = code;
           }
           cnt += __popcll(mask);
+        };
+        const int nfull = lim & ~63;
+        if (no_groups) {
+          for (int q0 = 0; q0 < nfull; q0 += 64) scan_step(q0, std::false_type{}, std::false_type{});
+          if (nfull < lim) scan_step(nfull, std::true_type{}, std::false_type{});
+        } else {
+          for (int q0 = 0; q0 < nfull; q0 += 64) scan_step(q0, std::false_type{}, std::true_type{});
+          if (nfull < lim) scan_step(nfull, std::true_type{}, std::true_type{});
         }
         continue;
       }
