@@ -329,8 +329,9 @@ int ynb_verlet_kick(const ynb_batch *b, double *vel, double *gpos, const double 
 /* ynb_verlet_kick followed by ynb_verlet_kick_drift (the end of one step and
  * the start of the next, verlet.py:354-355 then 344-347) in one launch, with
  * the arguments of both; every output is bitwise what the two calls give
- * (2*Ekin from the velocity after the first half kick).  One kernel boundary
- * fewer per step of a multi-step sequence. */
+ * (2*Ekin from the velocity after the first half kick) when dt is the same
+ * for both steps, as it is inside a sequence.  One kernel boundary fewer per
+ * step of a multi-step sequence. */
 int ynb_verlet_kick_kick_drift(const ynb_batch *b, double *pos, double *vel, double *gpos,
                                const double *gpart, const double *minv, const double *masses,
                                const double *dt, double *partial, int32_t *ctr, double *ekin2,

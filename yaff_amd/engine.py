@@ -728,8 +728,11 @@ class Engine(object):
         """nsteps Verlet steps of every system, on device, without
         synchronisation.  The kick that ends a step and the kick and drift
         that start the next are one kernel (bitwise the same values as
-        md_step repeated); the sequence starts with a kick and drift and ends
-        with a kick, so the state between calls is that of md_step."""
+        md_step repeated, dt_d being the same for both halves: it must not
+        change inside a sequence -- the server, which changes it between
+        iterations, calls md_step); the sequence starts with a kick and drift
+        and ends with a kick, so the state between calls is that of
+        md_step."""
         self._md_kick_drift()
         for k in range(nsteps):
             self._md_forces()
