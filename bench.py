@@ -438,8 +438,9 @@ def served(nreq, rates):
 def main():
     args = parse()
     if args.gpus != 1:
-        print('bench.py measures one GPU: every box here has a single MI355X and no run uses a second; '
-              'run with --gpus 1 (see DESIGN.md for how the path shards).', file=sys.stderr)
+        print('bench.py measures the throughput of one MI355X; no multi-GPU path is built (a batch or a '
+              'large system would shard by systems or by atom blocks, see the design notes); run with --gpus 1.',
+              file=sys.stderr)
         sys.exit(2)
     import torch
     from yaff_amd.engine import Engine
