@@ -174,3 +174,31 @@ def test_poisson_stream_and_admission():
     srv.waiting.append(rs[0])
     with pytest.raises(RuntimeError):
         srv._admit(0.0)
+
+
+def test_bench_reads_committed_traffic_summary():
+    """bench.py takes each kernel's HBM bytes per launch from the committed
+    rocprofv3 summary: the instance the launch parameters select when the
+    summary has it, else an FP64 instance of other launch parameters, named
+    in the result; the dominant kernel and the build are present."""
+    import os
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    tun = {'pair_lanes': 32, 'apl': 4, 'skb': 16, 'fapl': 1, 'fkb': 128}
+    assert bench.fp64_instance('ewald_force', tun) == 'k_ewald_force<0, 1>'
+    assert bench.fp64_instance('pair', dict(tun, pair_lanes=16)) == 'k_pair_form<0, 2, 16>'
+    assert bench.fp64_instance('nlist_bin_order', tun) == 'k_bin_order'
+    assert bench.fp64_instance('no_such_kernel', tun) is None
+    for k in ('ewald_force', 'ewald_sk', 'pair', 'nlist_build_binned', 'verlet_kick_kick_drift'):
+        tr = bench.profiled_traffic(k, tun)
+        assert tr is not None, k
+        rd, wr, name = tr
+        assert rd > 0 and wr > 0 and name.startswith(bench.ROCPROF_NAMES[k]), (k, tr)
+    # an instance the summary lacks falls back to one of the same kernel, named
+    rd, wr, name = bench.profiled_traffic('pair', dict(tun, pair_lanes=64))
+    assert name.startswith('k_pair_form<0,')
+    assert bench.profiled_traffic('nlist_skin_check', tun) is None
+    # the build writes its 8-byte entries: at least 8 bytes per half pair of the 6912-atom workload, twice over
+    assert bench.profiled_traffic('nlist_build_binned', tun)[1] >= 8*2*860000
