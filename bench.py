@@ -43,6 +43,9 @@ FLOP_EWALD_SK = 11           # ewald.c:62-76 per (k, atom): k.r 5, q cos/q sin 2
 FLOP_EWALD_FORCE = 9         # ewald.c:104-111 per (k, atom)
 
 
+from yaff_amd.batch import maxwell_boltzmann  # noqa: E402
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -69,28 +72,12 @@ def parse():
 
 
 def rocksalt_system(reps=(12, 12, 6), seed=0, angles=(90.0, 90.0, 90.0)):
+    """A workloads.rocksalt system and its engine input (ExpRep + LJ + EI,
+    Ewald reciprocal and correction)."""
     from yaff_amd import workloads
-    from yaff_amd.engine import NBSystem, PotSpec, EwaldSpec
-    from yaff_amd import _lib
+    from yaff_amd.batch import nbsystem
     s = workloads.rocksalt(reps, seed=seed, angles=angles)
-    sp = s.spec
-    n = s.natom
-    pots = [PotSpec(_lib.POT_EXPREP, sp.rcut, p0=s.ffatype_ids.astype(float), amp_cross=np.full((2, 2), sp.exprep_a),
-                    b_cross=np.full((2, 2), sp.exprep_b)),
-            PotSpec(_lib.POT_LJ, sp.rcut, p0=np.full(n, sp.lj_sigma), p1=np.full(n, sp.lj_epsilon)),
-            PotSpec(_lib.POT_EI, sp.rcut, p0=s.charges, alpha=sp.alpha)]
-    nb = NBSystem(s.pos, s.cell, pots, rc=sp.rcut, ewald=EwaldSpec(sp.alpha, sp.gcut), charges=s.charges,
-                  masses=s.masses)
-    return s, nb
-
-
-def maxwell_boltzmann(masses, temp, seed):
-    """yaff/sampling/utils.py:42-75 with scalevel0."""
-    from yaff_amd.units import boltzmann
-    rng = np.random.default_rng(seed)
-    v = rng.normal(0, 1, (len(masses), 3))*np.sqrt(boltzmann*temp/masses).reshape(-1, 1)
-    t = (v**2*masses.reshape(-1, 1)).mean()/boltzmann
-    return v*np.sqrt(temp/t)
+    return s, nbsystem(s)
 
 
 def pair_flops(engine, s_idx=0):
@@ -174,7 +161,8 @@ def kernel_table(prof, K):
 
 def cpu_baseline(s, evals):
     """The oracle (FP64 restatement of Yaff's C path, one host core) on the same
-    system: neighbour list, three pair potentials and the Ewald reciprocal."""
+    system: neighbour list, three pair potentials, the Ewald reciprocal and
+    correction terms."""
     from oracle import oracle as orc
     O = orc.Oracle()
     sp = s.spec
@@ -199,7 +187,8 @@ def cpu_baseline(s, evals):
             g += pg
             v += pv
         pe, pg, pv = O.ewald_reci(s.pos, s.charges, cell, sp.alpha, gmax, sp.gcut)
-        res = (e + pe, g + pg, v + pv)
+        ce, cg, cv = O.ewald_corr(s.pos, s.charges, cell, sp.alpha, None)
+        res = (e + pe + ce, g + pg + cg, v + pv + cv)
     t = time.perf_counter() - t0
     return n*evals/t, t, res
 

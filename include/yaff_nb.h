@@ -22,6 +22,7 @@
  *                                         switch3/hammer (yaff/pes/truncation.h)
  *   ynb_ewald_reci                     <- compute_ewald_reci yaff/pes/ewald.h:31-34
  *   ynb_ewald_corr                     <- compute_ewald_corr yaff/pes/ewald.h:39-43
+ *                                         (batched: every system of a ynb_batch)
  *   ynb_verlet_*                       <- VerletIntegrator.propagate
  *                                         yaff/sampling/verlet.py:339-367
  */
@@ -97,12 +98,17 @@ typedef struct ynb_sys {
   double alpha, dielectric;
   int32_t k0, nk;         /* k vectors [k0, k0+nk) of ynb_ewald.kvec; k0 and nk multiples of 256 */
   int32_t frb0, nfrb;     /* row blocks of the force pass: [frb0, frb0+nfrb) of ynb_ewald.fblk */
-  int32_t nslot, pad2_;   /* partial-gradient slots of the force pass: ceil(nfrb / 4) */
+  int32_t nslot;          /* partial-gradient slots of the force pass: ceil(nfrb / 4) */
+  int32_t ew_nlow;        /* atom groups of the Ewald reciprocal part (ForcePartEwaldReciprocal nlow/nhigh,
+                             yaff/pes/ff.py:331-401): its own, independent of the neighbour list's nlow/nhigh */
   int32_t chunk0, nchunk; /* atom chunks of the structure-factor pass (64 * apl atoms each) */
   int64_t spart0;         /* offset of this system's partial structure factors, in k entries */
-  int32_t nchunk_low, nchunk_high; /* chunks [0, low) below nlow, the last `high` above nhigh */
+  int32_t nchunk_low, nchunk_high; /* chunks [0, low) below ew_nlow, the last `high` at or above ew_nhigh */
   int32_t pair_form;      /* YNB_FORM_*: specialised pair kernel for this system's potential set */
-  int32_t pad1_;
+  int32_t ew_nhigh;
+  double ew_const;        /* constant added to the energy of the system's Ewald reciprocal out: the self term
+                             of its Ewald correction (ewald.c:286-290, times 1/dielectric) when the system has
+                             no scaled pairs, so that no ynb_ewald_corr launch is needed for it; else 0 */
 } ynb_sys;
 
 /* ynb_sys.pair_form: potential sets with a specialised totals kernel (no
@@ -296,11 +302,45 @@ int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const double *pos, c
                    int32_t precision, double *gpos, int32_t accumulate, double *out,
                    ynb_stream stream);
 
-/* Ewald correction for scaled pairs of one system plus the self term.
- * Processes the rows in order in one wavefront (deterministic). */
-int ynb_ewald_corr(const ynb_sys *h_sys, const ynb_sys *d_sys, const double *pos, const double *charges,
-                   const ynb_scaling_row *stab, int64_t nstab, double *gpos, int32_t accumulate,
-                   double *out, ynb_stream stream);
+/* Ewald correction (ForcePartEwaldCorrection) of every system of a batch:
+ * the self term and the erf term of the scaled pairs.  Per system: */
+typedef struct ynb_corr_sys {
+  double alpha, dielectric;
+  double self_energy;     /* -(alpha / sqrt(pi)) sum over nlow <= i < nhigh of q_i^2 (constant; the host
+                             sums it in atom order, as ewald.c:286-290) */
+  int32_t nlow, nhigh;    /* the part's atom groups (local indices) */
+  int32_t row0, nrow;     /* its scaling rows: [row0, row0 + nrow) of ynb_corr.stab */
+  int32_t blk0, nblk;     /* its row blocks of 256 rows: [blk0, blk0 + nblk), nblk >= 1 */
+} ynb_corr_sys;
+
+/* Tables of a batch (device).  atom_ref lists, for each padded atom in
+ * ascending row order, the rows that hold it: 2 r when it is row r's b (the
+ * gradient term is added), 2 r + 1 when it is row r's a (subtracted); so
+ * each atom's gradient is summed in the order in which compute_ewald_corr
+ * accumulates it, without atomics. */
+typedef struct ynb_corr {
+  const ynb_corr_sys *csys;     /* [nsys] */
+  const ynb_scaling_row *stab;  /* [nrow_tot] (a > b, local indices), each system's rows sorted as Scalings.stab */
+  const int32_t *blk_sys;       /* [nblk_tot]: system of each row block */
+  const int32_t *atom_ref0;     /* [npad + 1]: offsets of each atom's references in atom_ref */
+  const int32_t *atom_ref;      /* [2 nrow_tot] */
+  double *rowg;                 /* work [nrow_tot][3]: delta * g of each row (before 1/dielectric) */
+  double *partial;              /* work [nblk_tot][8]: energy and virial sums of each row block */
+  int32_t *ctr;                 /* [nsys] block counters (zero-initialised; left at zero) */
+  int32_t nrow_tot, nblk_tot;
+} ynb_corr;
+
+/* ynb_ewald_corr replaces compute_ewald_corr (yaff/pes/ewald.h:39-43) for
+ * every system of the batch at once.  out: [nsys][10] energy followed by the
+ * virial, written or, with out_accumulate, added (the Verlet step adds the
+ * correction to the reciprocal term's out this way); gpos (nullable)
+ * [npad][3] written or, with accumulate, added, for every atom of a system
+ * (atoms without scaled pairs get zero).  Two launches: the rows (energy,
+ * virial, delta * g per row; each system's last row block sums its block
+ * sums in a fixed order), then the atoms (each sums its rows in row order).
+ * Deterministic. */
+int ynb_ewald_corr(const ynb_batch *b, const ynb_corr *c, const double *pos, const double *charges,
+                   double *gpos, int32_t accumulate, double *out, int32_t out_accumulate, ynb_stream stream);
 
 /* Velocity Verlet halves (yaff/sampling/verlet.py:345-357): kick_drift does
  * vel += dt/2 * (-gpos/m); pos += dt * vel; kick does vel += dt/2 * (-gpos/m)

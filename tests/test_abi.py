@@ -41,7 +41,7 @@ def test_exports_every_declared_function(lib):
 
 STRUCTS = {
     'ynb_pot': 'YnbPot', 'ynb_sys': 'YnbSys', 'ynb_batch': 'YnbBatch', 'ynb_nlist': 'YnbNlist',
-    'ynb_pairpar': 'YnbPairpar', 'ynb_ewald': 'YnbEwald',
+    'ynb_pairpar': 'YnbPairpar', 'ynb_ewald': 'YnbEwald', 'ynb_corr_sys': 'YnbCorrSys', 'ynb_corr': 'YnbCorr',
 }
 
 

@@ -2,8 +2,8 @@
 
 Neighbour rows must be bit-identical to the rows of molmod/yaff's C kernels
 (SHA-256 of the 72-byte records plus count); energies, gradients and virials
-of every pair potential and of the Ewald reciprocal term must agree to a
-relative error of 1e-10 (FP64 mode).
+of every pair potential and of the Ewald reciprocal and correction terms
+must agree to a relative error of 1e-10 (FP64 mode).
 """
 import hashlib
 import os
@@ -31,7 +31,7 @@ def _cases():
 CASES = _cases()
 
 
-def _engine(c, precision='fp64'):
+def _engine(c, precision='fp64', correction=True):
     from yaff_amd.cell import Cell
     from yaff_amd.engine import NBSystem, PotSpec, EwaldSpec
     from yaff_amd import _lib
@@ -51,7 +51,7 @@ def _engine(c, precision='fp64'):
                              amp_cross=p.get('amp_cross'), b_cross=p.get('b_cross')))
     ew = None
     if c['ewald'] is not None:
-        ew = EwaldSpec(c['ewald']['alpha'], c['ewald']['gcut'], c['ewald'].get('dielectric', 1.0))
+        ew = EwaldSpec(c['ewald']['alpha'], c['ewald']['gcut'], c['ewald'].get('dielectric', 1.0), correction=correction)
     cell = Cell(c['rvecs'])
     from yaff_amd._lib import SCALING_DTYPE
     stab = None if c['stab'] is None else np.asarray(c['stab']).astype(SCALING_DTYPE)
@@ -102,7 +102,7 @@ def test_pair_fp64(name):
 def test_ewald_reci_fp64(name):
     print('case', name, flush=True)
     c = CASES[name]
-    eng = _engine(c)
+    eng = _engine(c, correction=False)   # the reciprocal term alone (no self term in its out)
     eng.gpos_d.zero_()
     eng.ewald_reci(gpos=True, accumulate=False)
     out = eng.ewald_out_d.cpu().numpy()[0]
@@ -133,9 +133,10 @@ def test_mixed_precision_error(name):
     g_ref = sum(REF['%s/pot%d/gpos' % (name, ip)] for ip in range(len(c['pots'])))
     v_ref = sum(REF['%s/pot%d/vtens' % (name, ip)] for ip in range(len(c['pots'])))
     if c['ewald'] is not None:
-        e_ref += float(REF[name + '/reci/e'])
-        g_ref = g_ref + REF[name + '/reci/gpos']
-        v_ref = v_ref + REF[name + '/reci/vtens']
+        for part in ('reci', 'corr'):
+            e_ref += float(REF[name + '/%s/e' % part])
+            g_ref = g_ref + REF[name + '/%s/gpos' % part]
+            v_ref = v_ref + REF[name + '/%s/vtens' % part]
     rms = np.sqrt(((g - g_ref)**2).sum(axis=1).mean())/np.sqrt((g_ref**2).sum(axis=1).mean())
     print('mixed', name, 'energy', abs(e - e_ref)/abs(e_ref), 'force rms', rms, flush=True)
     assert abs(e - e_ref) <= MIXED_REL_TOL*abs(e_ref)
@@ -146,7 +147,9 @@ def test_mixed_precision_error(name):
 @pytest.mark.parametrize('name', sorted(CASES))
 def test_total_fp64(name):
     """compute(): all potentials summed per pair (the MD and batch path) plus
-    the Ewald reciprocal term, against the sum of the reference parts."""
+    the Ewald reciprocal and correction terms (the batched correction
+    kernels where the case has scaled pairs), against the sum of the
+    reference parts."""
     print('case', name, flush=True)
     c = CASES[name]
     n = len(c['pos'])
@@ -158,10 +161,13 @@ def test_total_fp64(name):
     g_ref = sum(REF['%s/pot%d/gpos' % (name, ip)] for ip in range(len(c['pots'])))
     v_ref = sum(REF['%s/pot%d/vtens' % (name, ip)] for ip in range(len(c['pots'])))
     if c['ewald'] is not None:
-        e_ref += float(REF[name + '/reci/e'])
-        g_ref = g_ref + REF[name + '/reci/gpos']
-        v_ref = v_ref + REF[name + '/reci/vtens']
+        for part in ('reci', 'corr'):
+            e_ref += float(REF[name + '/%s/e' % part])
+            g_ref = g_ref + REF[name + '/%s/gpos' % part]
+            v_ref = v_ref + REF[name + '/%s/vtens' % part]
     scale_e = sum(abs(float(REF['%s/pot%d/e' % (name, ip)])) for ip in range(len(c['pots'])))
+    if c['ewald'] is not None:
+        scale_e += abs(float(REF[name + '/reci/e'])) + abs(float(REF[name + '/corr/e']))
     assert abs(e - e_ref) <= TOL*scale_e, (e, e_ref)
     assert rel(g, g_ref) <= TOL
     assert rel(v, v_ref) <= TOL

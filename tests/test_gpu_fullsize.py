@@ -40,7 +40,7 @@ def _gpu(s, precision):
 
 def _oracle(s):
     """Rows and the summed (energy, gradient, virial) of the pair parts and
-    the Ewald reciprocal term, from the oracle."""
+    the Ewald reciprocal and correction terms, from the oracle."""
     from oracle import oracle as orc
     O = orc.Oracle()
     sp = s.spec
@@ -57,6 +57,7 @@ def _oracle(s):
     parts = [O.pair_compute(rows, None, p, n) for p in pots]
     if sp.kind != 'lj':
         parts.append(O.ewald_reci(s.pos, s.charges, cell, sp.alpha, orc.gmax_of(cell, sp.gcut), sp.gcut))
+        parts.append(O.ewald_corr(s.pos, s.charges, cell, sp.alpha, None))  # the self term
     return rows, (sum(p[0] for p in parts), sum(p[1] for p in parts), sum(p[2] for p in parts))
 
 

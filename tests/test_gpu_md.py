@@ -17,10 +17,11 @@ pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason='n
 
 
 def _ionic_engine(reps=(3, 3, 2), precision='fp64'):
-    from bench import rocksalt_system
+    from yaff_amd import workloads
+    from yaff_amd.batch import nbsystem
     from yaff_amd.engine import Engine
-    s, nb = rocksalt_system(reps, seed=3, angles=(88.0, 91.0, 93.0))
-    return s, Engine([nb], precision=precision)
+    s = workloads.rocksalt(reps, seed=3, angles=(88.0, 91.0, 93.0))
+    return s, Engine([nbsystem(s)], precision=precision)
 
 
 def _lj_engine():
@@ -35,7 +36,7 @@ def _lj_engine():
 
 def test_graph_replay_matches_eager():
     import torch
-    from bench import maxwell_boltzmann
+    from yaff_amd.batch import maxwell_boltzmann
     from yaff_amd.units import femtosecond
     print('case rocksalt 144 atoms, graph vs eager', flush=True)
     s, eng = _ionic_engine()
@@ -70,7 +71,7 @@ def test_graph_replay_matches_eager():
 
 def test_nve_energy_conservation_lj864():
     import torch
-    from bench import maxwell_boltzmann
+    from yaff_amd.batch import maxwell_boltzmann
     from yaff_amd.units import femtosecond
     print('case LJ 864 NVE 100 steps', flush=True)
     s, eng = _lj_engine()
@@ -165,7 +166,7 @@ def test_joined_steps_match_single_steps(kind):
     energy totals and kinetic energies bit for bit those of n md_step calls,
     eagerly and replayed from a graph."""
     import torch
-    from bench import maxwell_boltzmann
+    from yaff_amd.batch import maxwell_boltzmann
     from yaff_amd import workloads
     from yaff_amd.batch import nbsystem
     from yaff_amd.engine import Engine

@@ -202,3 +202,56 @@ def test_bench_reads_committed_traffic_summary():
     assert bench.profiled_traffic('nlist_skin_check', tun) is None
     # the build writes its 8-byte entries: at least 8 bytes per half pair of the 6912-atom workload, twice over
     assert bench.profiled_traffic('nlist_build_binned', tun)[1] >= 8*2*860000
+
+
+def test_corr_plan_tables():
+    """The host tables of ynb_ewald_corr: each atom's row references, summed
+    in their order with the rows' delta * g (here from numpy), give the
+    oracle's correction gradient; the self term is the oracle's energy with
+    no scaled pairs; row blocks cover every system (one at least)."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden'))
+    import cases
+    from oracle import oracle as orc
+    from yaff_amd.engine import corr_plan
+    from yaff_amd import Scalings, System
+    from math import erf
+    f = cases.fixture_arrays('water32')
+    s = System(f['numbers'], f['pos'], bonds=f['bonds'], rvecs=f['rvecs'], charges=f['charges'])
+    st = Scalings(s, 0.0, 0.25, 0.5).stab
+    alpha, nlow, nhigh = 0.35, 15, 81
+    npad = 512
+    entries = [None, dict(atom0=256, natom=s.natom, alpha=alpha, dielectric=1.0, nlow=nlow, nhigh=nhigh, stab=st,
+                          charges=s.charges)]
+    p = corr_plan(entries, npad)
+    cs = p['csys']
+    assert list(cs['nblk']) == [1, 1] and p['nblk_tot'] == 2 and list(p['blk_sys']) == [0, 1]
+    assert cs[1]['nrow'] == len(st) == p['nrow_tot'] and cs[0]['self_energy'] == 0.0
+    O = orc.Oracle()
+    cell = O.cell(s.cell.rvecs)
+    e_self = O.ewald_corr(s.pos, s.charges, cell, alpha, None, 1.0, nlow, nhigh)[0]
+    assert abs(cs[1]['self_energy'] - e_self) <= 1e-14*abs(e_self)
+    # rows as the kernel forms them, gradients summed through the references
+    rowg = np.zeros((len(st), 3))
+    for r, w in enumerate(st):
+        a, b = int(w['a']), int(w['b'])
+        if (a < nlow and b < nlow) or (a >= nhigh and b >= nhigh):
+            continue
+        dl = O.mic(s.pos[b] - s.pos[a], cell)
+        d = np.sqrt((dl*dl).sum())
+        x = alpha*d
+        pot = erf(x)/d
+        fac = (1 - w['scale'])*s.charges[b]*s.charges[a]
+        rowg[r] = dl*(-fac*(2/np.sqrt(np.pi)*alpha*np.exp(-x*x) - pot)/d/d)
+    g = np.zeros((npad, 3))
+    for atom in range(npad):
+        for ref in p['atom_ref'][p['atom_ref0'][atom]:p['atom_ref0'][atom + 1]]:
+            g[atom] += -rowg[ref >> 1] if ref & 1 else rowg[ref >> 1]
+    ref = O.ewald_corr(s.pos, s.charges, cell, alpha, st, 1.0, nlow, nhigh)[1]
+    assert not g[:256].any() and not g[256 + s.natom:].any()
+    assert np.abs(g[256:256 + s.natom] - ref).max() <= 1e-13*np.abs(ref).max()
+    # each atom lists its rows in ascending row order
+    for atom in range(npad):
+        refs = p['atom_ref'][p['atom_ref0'][atom]:p['atom_ref0'][atom + 1]]
+        assert list(refs >> 1) == sorted(refs >> 1)

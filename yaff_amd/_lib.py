@@ -41,9 +41,10 @@ class YnbSys(C.Structure):
                 ('grid', C.c_int32 * 3), ('range', C.c_int32 * 3), ('bin0', C.c_int32), ('nbin', C.c_int32),
                 ('any_stab', C.c_int32), ('box_safe', C.c_int32), ('pot', YnbPot * MAX_POT),
                 ('alpha', C.c_double), ('dielectric', C.c_double), ('k0', C.c_int32), ('nk', C.c_int32),
-                ('frb0', C.c_int32), ('nfrb', C.c_int32), ('nslot', C.c_int32), ('pad2_', C.c_int32),
+                ('frb0', C.c_int32), ('nfrb', C.c_int32), ('nslot', C.c_int32), ('ew_nlow', C.c_int32),
                 ('chunk0', C.c_int32), ('nchunk', C.c_int32), ('spart0', C.c_int64), ('nchunk_low', C.c_int32),
-                ('nchunk_high', C.c_int32), ('pair_form', C.c_int32), ('pad1_', C.c_int32)]
+                ('nchunk_high', C.c_int32), ('pair_form', C.c_int32), ('ew_nhigh', C.c_int32),
+                ('ew_const', C.c_double)]
 
 
 class YnbBatch(C.Structure):
@@ -71,6 +72,18 @@ class YnbEwald(C.Structure):
                 ('frac', C.c_void_p), ('step', C.c_void_p),
                 ('spart', C.c_void_p), ('coef', C.c_void_p), ('frec', C.c_void_p), ('kpart', C.c_void_p),
                 ('gpart', C.c_void_p)]
+
+
+class YnbCorrSys(C.Structure):
+    _fields_ = [('alpha', C.c_double), ('dielectric', C.c_double), ('self_energy', C.c_double),
+                ('nlow', C.c_int32), ('nhigh', C.c_int32), ('row0', C.c_int32), ('nrow', C.c_int32),
+                ('blk0', C.c_int32), ('nblk', C.c_int32)]
+
+
+class YnbCorr(C.Structure):
+    _fields_ = [('csys', C.c_void_p), ('stab', C.c_void_p), ('blk_sys', C.c_void_p), ('atom_ref0', C.c_void_p),
+                ('atom_ref', C.c_void_p), ('rowg', C.c_void_p), ('partial', C.c_void_p), ('ctr', C.c_void_p),
+                ('nrow_tot', C.c_int32), ('nblk_tot', C.c_int32)]
 
 
 EXPORTS = ['ynb_version', 'ynb_last_error', 'ynb_sys_plan', 'ynb_nlist_workspace_bytes',
@@ -112,7 +125,7 @@ def load(path=LIB_PATH):
                                    vp, vp, vp]
     L.ynb_pair_rows.argtypes = [vp, i64, vp, i64, P(YnbPot), vp, vp, vp, vp, vp, vp, vp, vp]
     L.ynb_ewald_reci.argtypes = [P(YnbBatch), P(YnbEwald), vp, vp, i32, vp, i32, vp, vp]
-    L.ynb_ewald_corr.argtypes = [P(YnbSys), vp, vp, vp, vp, i64, vp, i32, vp, vp]
+    L.ynb_ewald_corr.argtypes = [P(YnbBatch), P(YnbCorr), vp, vp, vp, i32, vp, i32, vp]
     L.ynb_verlet_kick_drift.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp, P(YnbEwald), vp, vp, sz, P(YnbNlist), vp]
     L.ynb_verlet_kick.argtypes = [P(YnbBatch)] + [vp]*13
     L.ynb_verlet_kick_kick_drift.argtypes = [P(YnbBatch)] + [vp]*13 + [P(YnbEwald), vp, vp, sz, P(YnbNlist), vp]

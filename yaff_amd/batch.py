@@ -21,10 +21,13 @@ from .units import femtosecond, boltzmann
 __all__ = ['nbsystem', 'forcefield', 'timestep', 'maxwell_boltzmann', 'BatchMD']
 
 
-def nbsystem(s, skin=0.0):
+def nbsystem(s, skin=0.0, correction=True):
     """Engine input for a workloads.lj_fluid or workloads.rocksalt system;
     with skin > 0 the engine's MD steps rebuild its list only when an atom
-    has moved by skin/2."""
+    has moved by skin/2.  Rocksalt gets the Ewald reciprocal term and, with
+    ``correction``, the Ewald correction (its self term: the rocksalt
+    systems have no scaled pairs), as the drop-in force field of
+    ``forcefield`` has them."""
     sp = s.spec
     n = s.natom
     if sp.kind == 'lj':
@@ -35,8 +38,8 @@ def nbsystem(s, skin=0.0):
                     amp_cross=np.full((2, 2), sp.exprep_a), b_cross=np.full((2, 2), sp.exprep_b)),
             PotSpec(_lib.POT_LJ, sp.rcut, p0=np.full(n, sp.lj_sigma), p1=np.full(n, sp.lj_epsilon)),
             PotSpec(_lib.POT_EI, sp.rcut, p0=s.charges, alpha=sp.alpha)]
-    return NBSystem(s.pos, s.cell, pots, ewald=EwaldSpec(sp.alpha, sp.gcut), charges=s.charges, masses=s.masses,
-                    skin=skin)
+    return NBSystem(s.pos, s.cell, pots, ewald=EwaldSpec(sp.alpha, sp.gcut, correction=correction),
+                    charges=s.charges, masses=s.masses, skin=skin)
 
 
 def forcefield(s, skin=0.0):

@@ -259,7 +259,7 @@ __device__ void ewald_system_reduce(const ynb_sys &S, int s, const double *kpart
     const double e = r[0];
     const double f = 1.0 / S.dielectric;
     double *o = out + 10 * (int64_t)s;
-    o[0] = e * f;
+    o[0] = e * f + S.ew_const;
     o[1] = (r[1] - e) * f; o[2] = r[4] * f; o[3] = r[5] * f;
     o[4] = r[4] * f; o[5] = (r[2] - e) * f; o[6] = r[6] * f;
     o[7] = r[5] * f; o[8] = r[6] * f; o[9] = (r[3] - e) * f;
@@ -581,8 +581,8 @@ __global__ __launch_bounds__(256) void k_ewald_force_groups(const ynb_sys *__res
     cr[p] = step[g];
     ci[p] = step[npad + g];
     const int la = (int)(g - S.atom0);
-    wh[p] = la < S.nhigh ? 1.0 : 0.0;
-    wl[p] = la >= S.nlow ? 1.0 : 0.0;
+    wh[p] = la < S.ew_nhigh ? 1.0 : 0.0;
+    wl[p] = la >= S.ew_nlow ? 1.0 : 0.0;
     G0[p] = G1[p] = G2[p] = 0.0;
   }
   const int fb = 4 * (int)blockIdx.y + wv;
@@ -631,55 +631,118 @@ __global__ __launch_bounds__(256) void k_ewald_force_groups(const ynb_sys *__res
   if (gpos && last_block(fctr + blk, S.nslot)) ewald_gsum_block(S, blk * NA, NA, gpart, npad, gpos, accumulate);
 }
 
-// ewald.c:278-349 for one system, rows processed in order by lane 0 of one
-// wavefront; the self term is a wave-strided sum in a fixed order.
-__global__ void k_ewald_corr(const ynb_sys *sysd, const double *pos, const double *q, const ynb_scaling_row *stab,
-                             int64_t nstab, double *gpos, int accumulate, double *out) {
-  const ynb_sys &S = *sysd;
-  const int lane = threadIdx.x;
-  const double alpha = S.alpha;
-  const double x = alpha / 1.7724538509055160273;  // M_SQRT_PI
-  double self = 0.0;
-  for (int i = S.nlow + lane; i < S.nhigh; i += 64) self += x * q[S.atom0 + i] * q[S.atom0 + i];
-  self = wave_sum(self);
-  const double f = 1.0 / S.dielectric;
-  if (!accumulate && gpos)
-    for (int i = lane; i < S.natom; i += 64)
-      for (int c = 0; c < 3; ++c) gpos[3 * (int64_t)(S.atom0 + i) + c] = 0.0;
+// Sums of the block partials [b0, b0 + nb) of one group (8 doubles per
+// block, values 0..6 used), by one block of 256 threads: thread (value q,
+// part p) adds blocks b0 + p, b0 + p + 32, ... in order, then a fixed LDS
+// tree over the 32 parts.  The result of value q is in l_red[q] afterwards.
+__device__ void group_block_sum(const double *partial, int64_t b0, int64_t nb, double *l_red) {
+  const int t = threadIdx.x;
+  const int q = t & 7, part = t >> 3;
+  double a = 0.0;
+  if (q < 7)
+    for (int64_t b = part; b < nb; b += 32) a += gather(partial + (b0 + b) * 8 + q);
+  l_red[t] = a;
   __syncthreads();
-  if (lane != 0) return;
-  double energy = -self;
-  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int64_t r = 0; r < nstab; ++r) {
-    const int64_t a = stab[r].a, b = stab[r].b;
-    if (a < S.nlow && b < S.nlow) continue;
-    if (a >= S.nhigh && b >= S.nhigh) continue;
-    const int64_t ga = S.atom0 + a, gb = S.atom0 + b;
-    double dl[3] = {pos[3 * gb] - pos[3 * ga], pos[3 * gb + 1] - pos[3 * ga + 1], pos[3 * gb + 2] - pos[3 * ga + 2]};
-    int m[3];
-    mic_exact(dl, S.rvecs, S.gvecs, S.nvec, m);
-    const double d = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
-    const double xx = alpha * d;
-    const double e = erf(xx) / d;
-    const double fac = (1 - stab[r].scale) * q[gb] * q[ga];
-    const double gg = -fac * (1.1283791670955125739 * alpha * exp(-xx * xx) - e) / d / d;
-    if (gpos) {
-      for (int c = 0; c < 3; ++c) {
-        gpos[3 * gb + c] += dl[c] * gg * f;
-        gpos[3 * ga + c] -= dl[c] * gg * f;
-      }
-    }
-    v[0] += dl[0] * dl[0] * gg;
-    v[4] += dl[1] * dl[1] * gg;
-    v[8] += dl[2] * dl[2] * gg;
-    const double v01 = dl[1] * dl[0] * gg, v02 = dl[2] * dl[0] * gg, v12 = dl[2] * dl[1] * gg;
-    v[1] += v01; v[3] += v01;
-    v[2] += v02; v[6] += v02;
-    v[5] += v12; v[7] += v12;
-    energy -= fac * e;
+  for (int off = 16; off > 0; off >>= 1) {
+    if (part < off) l_red[t] += l_red[t + 8 * off];
+    __syncthreads();
   }
-  out[0] = energy * f;
-  for (int c = 0; c < 9; ++c) out[1 + c] = v[c] * f;
+}
+
+// The scaled-pair rows of ewald.c:291-334, one thread per row: delta = mic(pos_b
+// - pos_a), g = -fac (2/sqrt(pi) alpha exp(-x^2) - erf(x)/d) / d^2 with
+// fac = (1 - scale) q_a q_b, the row's delta * g to rowg, its energy and
+// virial to the block sums; the last row block of each system adds the
+// system's block sums in a fixed order and writes (or adds to) out with the
+// self term and the 1/dielectric factor of ewald.c:336-347.
+__global__ __launch_bounds__(256) void k_corr_rows(const ynb_sys *sys, ynb_corr c, const double *pos,
+                                                   const double *q, double *out, int out_accumulate) {
+  const int blk = blockIdx.x;
+  const int sid = c.blk_sys[blk];
+  const ynb_sys &S = sys[sid];
+  const ynb_corr_sys &P = c.csys[sid];
+  const int t = threadIdx.x;
+  const int lr = (blk - P.blk0) * 256 + t;
+  double e = 0.0, v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (lr < P.nrow) {
+    const int64_t r = (int64_t)P.row0 + lr;
+    const ynb_scaling_row w = c.stab[r];
+    double h[3] = {0.0, 0.0, 0.0};
+    if (!((w.a < P.nlow && w.b < P.nlow) || (w.a >= P.nhigh && w.b >= P.nhigh))) {
+      const int64_t ga = S.atom0 + w.a, gb = S.atom0 + w.b;
+      double dl[3] = {pos[3 * gb] - pos[3 * ga], pos[3 * gb + 1] - pos[3 * ga + 1], pos[3 * gb + 2] - pos[3 * ga + 2]};
+      int m[3];
+      mic_exact(dl, S.rvecs, S.gvecs, S.nvec, m);
+      const double d = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+      const double x = P.alpha * d;
+      const double pot = erf(x) / d;
+      const double fac = (1 - w.scale) * q[gb] * q[ga];
+      const double g = -fac * (1.1283791670955125739 * P.alpha * exp(-x * x) - pot) / d / d;
+      h[0] = dl[0] * g;
+      h[1] = dl[1] * g;
+      h[2] = dl[2] * g;
+      v[0] = dl[0] * dl[0] * g;
+      v[1] = dl[1] * dl[1] * g;
+      v[2] = dl[2] * dl[2] * g;
+      v[3] = dl[1] * dl[0] * g;
+      v[4] = dl[2] * dl[0] * g;
+      v[5] = dl[2] * dl[1] * g;
+      e = -fac * pot;
+    }
+    c.rowg[3 * r] = h[0];
+    c.rowg[3 * r + 1] = h[1];
+    c.rowg[3 * r + 2] = h[2];
+  }
+  // block sums: a fixed tree over the 256 rows
+  __shared__ double l_r[7][256];
+  l_r[0][t] = e;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) l_r[1 + k][t] = v[k];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off)
+#pragma unroll
+      for (int k = 0; k < 7; ++k) l_r[k][t] += l_r[k][t + off];
+    __syncthreads();
+  }
+  if (t < 8) publish(c.partial + (int64_t)blk * 8 + t, t < 7 ? l_r[t][0] : 0.0);
+  if (last_block(c.ctr + sid, P.nblk)) {
+    __shared__ double l_red[256];
+    group_block_sum(c.partial, P.blk0, P.nblk, l_red);
+    if (t == 0) {
+      const double f = 1.0 / P.dielectric;
+      const double r[10] = {(P.self_energy + l_red[0]) * f,
+                            l_red[1] * f, l_red[4] * f, l_red[5] * f,
+                            l_red[4] * f, l_red[2] * f, l_red[6] * f,
+                            l_red[5] * f, l_red[6] * f, l_red[3] * f};
+      double *o = out + 10 * (int64_t)sid;
+      for (int k = 0; k < 10; ++k) o[k] = out_accumulate ? o[k] + r[k] : r[k];
+    }
+  }
+}
+
+// Gradient of the correction, one thread per (atom, component): the atom's
+// rows in row order (+ delta g where it is b, - where it is a), as
+// compute_ewald_corr accumulates them into its zeroed buffer, then
+// 1/dielectric (ewald.c:336-341).
+__global__ __launch_bounds__(256) void k_corr_atoms(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
+                                                    ynb_corr c, double *gpos, int accumulate) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= 3 * npad) return;
+  const int64_t g = e / 3;
+  const int comp = (int)(e - 3 * g);
+  const int sid = wave_sys[g >> 6];
+  if (sid < 0) return;
+  if (g - sys[sid].atom0 >= sys[sid].natom) return;
+  const int k0 = c.atom_ref0[g], k1 = c.atom_ref0[g + 1];
+  double acc = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const int ref = c.atom_ref[k];
+    const double x = c.rowg[3 * (int64_t)(ref >> 1) + comp];
+    acc = (ref & 1) ? acc - x : acc + x;
+  }
+  const double G = acc * (1.0 / c.csys[sid].dielectric);
+  gpos[e] = accumulate ? gpos[e] + G : G;
 }
 
 }  // namespace
@@ -755,15 +818,23 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
   return YNB_OK;
 }
 
-extern "C" int ynb_ewald_corr(const ynb_sys *h_sys, const ynb_sys *d_sys, const double *pos, const double *charges,
-                              const ynb_scaling_row *stab, int64_t nstab, double *gpos, int32_t accumulate, double *out,
+extern "C" int ynb_ewald_corr(const ynb_batch *b, const ynb_corr *c, const double *pos, const double *charges,
+                              double *gpos, int32_t accumulate, double *out, int32_t out_accumulate,
                               ynb_stream stream) {
-  if (!h_sys || !d_sys || !pos || !charges || !out || nstab < 0 || (nstab > 0 && !stab)) {
+  if (!b || !c || !pos || !charges || !out || !c->csys || !c->blk_sys || !c->ctr || !c->partial ||
+      c->nrow_tot < 0 || c->nblk_tot < b->nsys || (c->nrow_tot > 0 && (!c->stab || !c->rowg)) ||
+      (gpos && (!c->atom_ref0 || (c->nrow_tot > 0 && !c->atom_ref))) || b->npad % 64 != 0) {
     set_error("ynb_ewald_corr: invalid arguments");
     return YNB_EINVAL;
   }
-  hipLaunchKernelGGL(k_ewald_corr, dim3(1), dim3(64), 0, (hipStream_t)stream, d_sys, pos, charges, stab, nstab, gpos,
-                     accumulate, out);
+  hipStream_t st = (hipStream_t)stream;
+  YNB_LAUNCH("ewald_corr_rows", k_corr_rows, dim3((unsigned)c->nblk_tot), dim3(256), 0, st, b->sys, *c, pos,
+             charges, out, out_accumulate);
   YNB_LAUNCHED();
+  if (gpos) {
+    YNB_LAUNCH("ewald_corr_atoms", k_corr_atoms, dim3((unsigned)ceil_div(3 * b->npad, 256)), dim3(256), 0, st,
+               b->sys, b->wave_sys, b->npad, *c, gpos, accumulate);
+    YNB_LAUNCHED();
+  }
   return YNB_OK;
 }

@@ -10,7 +10,7 @@ import torch
 
 from . import _lib
 from ._lib import MAX_POT, POT_LJ, POT_MM3, POT_EXPREP, POT_EI, TR_NONE, TR_SWITCH3, TR_HAMMER, SCALING_DTYPE
-from .engine import Engine, NBSystem, PotSpec, EwaldSpec
+from .engine import Engine, NBSystem, PotSpec, EwaldSpec, CorrTables, corr_plan
 
 __all__ = ['SystemDevice', 'set_default_precision']
 
@@ -159,18 +159,23 @@ class SystemDevice(object):
         return res
 
     def ewald_corr(self, alpha, stab, dielectric, nlow, nhigh):
+        """(energy, gpos, vtens) of a ForcePartEwaldCorrection with these
+        settings, by ynb_ewald_corr on the system's engine.  The tables and
+        buffers are made once per set of settings (and engine)."""
         eng = self.get_engine()
         s = self.system
-        h = _lib.YnbSys()
-        _lib.C.memmove(_lib.C.byref(h), _lib.C.byref(eng.hsys[0]), _lib.C.sizeof(_lib.YnbSys))
-        h.alpha, h.dielectric, h.nlow, h.nhigh = alpha, dielectric, nlow, nhigh
-        dsys = torch.from_numpy(np.frombuffer(bytes(h), np.uint8).copy()).to(eng.device)
         stab = np.zeros(0, SCALING_DTYPE) if stab is None else np.asarray(stab, dtype=SCALING_DTYPE)
-        stab_d = torch.from_numpy(np.frombuffer(stab.tobytes() or b'\0'*32, np.uint8).copy()).to(eng.device)
-        out = torch.zeros(10, dtype=torch.float64, device=eng.device)
-        gpos = torch.zeros((eng.npad, 3), dtype=torch.float64, device=eng.device)
-        _lib.check(eng.lib.ynb_ewald_corr(h, dsys.data_ptr(), eng.pos_d.data_ptr(), eng.charges_d.data_ptr(),
-                                          stab_d.data_ptr(), len(stab), gpos.data_ptr(), 1, out.data_ptr(),
-                                          Engine.stream()))
+        key = (id(eng), float(alpha), float(dielectric), int(nlow), int(nhigh), stab.tobytes())
+        cache = self.__dict__.setdefault('_corr_cache', {})
+        if key not in cache:
+            cache.clear()
+            plan = corr_plan([dict(atom0=0, natom=s.natom, alpha=alpha, dielectric=dielectric, nlow=nlow,
+                                   nhigh=nhigh, stab=stab, charges=s.charges)], eng.npad)
+            cache[key] = (CorrTables(plan, eng.device),
+                          torch.zeros(10, dtype=torch.float64, device=eng.device),
+                          torch.zeros((eng.npad, 3), dtype=torch.float64, device=eng.device))
+        tabs, out, gpos = cache[key]
+        _lib.check(eng.lib.ynb_ewald_corr(eng.batch, tabs.c, eng.pos_d.data_ptr(), eng.charges_d.data_ptr(),
+                                          gpos.data_ptr(), 0, out.data_ptr(), 0, Engine.stream()))
         o = out.cpu().numpy()
         return float(o[0]), gpos[:s.natom].cpu().numpy(), o[1:].reshape(3, 3).copy()

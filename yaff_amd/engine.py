@@ -44,12 +44,18 @@ class PotSpec(object):
 
 
 class EwaldSpec(object):
-    """Reciprocal-space settings; gmax defaults to ff.py:386 for the cell."""
+    """Reciprocal-space settings; gmax defaults to ff.py:386 for the cell.
+    nlow/nhigh are the Ewald parts' own atom groups (ff.py:331-532), not the
+    neighbour list's.  With ``correction`` the engine's evaluations and MD
+    steps also add ForcePartEwaldCorrection (ewald.c:278-349: the self term
+    and the erf term of the system's scaled pairs) with the same alpha,
+    dielectric and groups."""
 
-    def __init__(self, alpha, gcut, dielectric=1.0, nlow=0, nhigh=-1, gmax=None):
+    def __init__(self, alpha, gcut, dielectric=1.0, nlow=0, nhigh=-1, gmax=None, correction=False):
         self.alpha, self.gcut, self.dielectric = float(alpha), float(gcut), float(dielectric)
         self.nlow, self.nhigh = int(nlow), int(nhigh)
         self.gmax = None if gmax is None else np.asarray(gmax, dtype=np.int64)
+        self.correction = bool(correction)
 
 
 class NBSystem(object):
@@ -122,6 +128,84 @@ def ewald_kvectors_cached(s):
         cache.clear()
         cache[kkey] = ewald_kvectors(s.cell, ew.alpha, ew.gcut, ew.gmax)
     return cache[kkey]
+
+
+SQRT_PI = 1.77245385090551602729
+
+
+def corr_self_energy(alpha, charges, nlow, nhigh):
+    """-(alpha/sqrt(pi)) sum_{nlow <= i < nhigh} q_i^2, summed in atom order
+    with the operations of ewald.c:286-290 (a cumulative sum is sequential)."""
+    q = np.asarray(charges, dtype=np.float64)[nlow:nhigh]
+    terms = -((alpha/SQRT_PI*q)*q)
+    return float(np.cumsum(terms)[-1]) if len(terms) else 0.0
+
+
+def corr_plan(entries, npad):
+    """Host tables of ynb_ewald_corr (include/yaff_nb.h: ynb_corr) for a
+    batch.  ``entries``: one per system, None (no correction: no rows, no
+    self term) or a dict with atom0, natom, alpha, dielectric, nlow, nhigh
+    (-1: natom), stab (scaling rows, local indices) and charges.  The self
+    term is summed in atom order as ewald.c:286-290 does; each atom lists its
+    rows in row order (2 r: it is b, 2 r + 1: it is a)."""
+    csys = np.zeros(len(entries), dtype=np.dtype([('alpha', 'f8'), ('dielectric', 'f8'), ('self_energy', 'f8'),
+                                                  ('nlow', 'i4'), ('nhigh', 'i4'), ('row0', 'i4'), ('nrow', 'i4'),
+                                                  ('blk0', 'i4'), ('nblk', 'i4')]))
+    stabs, blk_sys, atoms, refs = [], [], [], []
+    row0 = blk0 = 0
+    for i, e in enumerate(entries):
+        c = csys[i]
+        c['dielectric'] = 1.0
+        nrow = 0
+        if e is not None:
+            st = np.asarray(e['stab'] if e['stab'] is not None else np.zeros(0, SCALING_DTYPE), dtype=SCALING_DTYPE)
+            nlow = int(e['nlow'])
+            nhigh = e['natom'] if e['nhigh'] == -1 else int(e['nhigh'])
+            c['alpha'], c['dielectric'] = e['alpha'], e['dielectric']
+            c['self_energy'] = corr_self_energy(e['alpha'], e['charges'], nlow, nhigh)
+            c['nlow'], c['nhigh'] = nlow, nhigh
+            nrow = len(st)
+            if nrow:
+                stabs.append(st)
+                r = row0 + np.arange(nrow, dtype=np.int64)
+                atoms.append(np.stack([e['atom0'] + st['b'], e['atom0'] + st['a']], axis=1).ravel())
+                refs.append(np.stack([2*r, 2*r + 1], axis=1).ravel())
+        nblk = max(1, _ceil_div(nrow, 256))
+        c['row0'], c['nrow'], c['blk0'], c['nblk'] = row0, nrow, blk0, nblk
+        blk_sys.append(np.full(nblk, i, np.int32))
+        row0 += nrow
+        blk0 += nblk
+    atoms = np.concatenate(atoms) if atoms else np.zeros(0, np.int64)
+    refs = np.concatenate(refs) if refs else np.zeros(0, np.int64)
+    order = np.argsort(atoms, kind='stable')
+    ref0 = np.zeros(npad + 1, np.int32)
+    ref0[1:] = np.cumsum(np.bincount(atoms, minlength=npad)[:npad])
+    return dict(csys=csys, stab=np.concatenate(stabs) if stabs else np.zeros(1, SCALING_DTYPE),
+                blk_sys=np.concatenate(blk_sys) if blk_sys else np.zeros(1, np.int32),
+                atom_ref0=ref0, atom_ref=refs[order].astype(np.int32) if len(refs) else np.zeros(1, np.int32),
+                nrow_tot=row0, nblk_tot=blk0)
+
+
+class CorrTables(object):
+    """Device copy of a corr_plan plus its work buffers, and the YnbCorr
+    structure that points at them."""
+
+    def __init__(self, plan, device):
+        def t(a):
+            return torch.from_numpy(np.frombuffer(np.ascontiguousarray(a).tobytes(), np.uint8).copy()).to(device)
+        self.plan = plan
+        self.csys_d = t(plan['csys'])
+        self.stab_d = t(plan['stab'])
+        self.blk_sys_d = t(plan['blk_sys'])
+        self.ref0_d = t(plan['atom_ref0'])
+        self.ref_d = t(plan['atom_ref'])
+        self.rowg_d = torch.zeros(3*max(plan['nrow_tot'], 1), dtype=torch.float64, device=device)
+        self.partial_d = torch.zeros(8*plan['nblk_tot'], dtype=torch.float64, device=device)
+        self.ctr_d = torch.zeros(len(plan['csys']), dtype=torch.int32, device=device)
+        self.c = _lib.YnbCorr(self.csys_d.data_ptr(), self.stab_d.data_ptr(), self.blk_sys_d.data_ptr(),
+                              self.ref0_d.data_ptr(), self.ref_d.data_ptr(), self.rowg_d.data_ptr(),
+                              self.partial_d.data_ptr(), self.ctr_d.data_ptr(), plan['nrow_tot'],
+                              plan['nblk_tot'])
 
 
 def _row_blocks(cnt, kmax):
@@ -252,6 +336,11 @@ class Engine(object):
         nbin = 0
         stab_off = 0
         self.has_ewald = any(s.ewald is not None for s in self.systems)
+        # systems whose correction has scaled pairs: ynb_ewald_corr (the others' self term is ew_const)
+        self.has_corr = any(s.ewald is not None and s.ewald.correction and len(s.stab) > 0 for s in self.systems)
+        # systems whose scaled pairs would lose the Ewald correction in compute() and MD steps
+        self.uncorrected = [i for i, s in enumerate(self.systems)
+                            if s.ewald is not None and not s.ewald.correction and len(s.stab) > 0]
         self.groups = 0
         for i, s in enumerate(self.systems):
             h = self.hsys[i]
@@ -268,6 +357,14 @@ class Engine(object):
             for k in range(3):
                 h.rmax[k] = int(s.rmax[k]) if k < cell.nvec else 0
             h.nlow, h.nhigh = s.nlow, s.nhigh
+            h.ew_nlow, h.ew_nhigh = 0, s.natom
+            h.ew_const = 0.0
+            if s.ewald is not None:
+                h.ew_nlow = s.ewald.nlow
+                h.ew_nhigh = s.natom if s.ewald.nhigh == -1 else s.ewald.nhigh
+                if s.ewald.correction and len(s.stab) == 0:
+                    # the correction is its self term alone: a constant of the reciprocal out
+                    h.ew_const = corr_self_energy(s.ewald.alpha, s.charges, h.ew_nlow, h.ew_nhigh)/s.ewald.dielectric
             check(L.ynb_sys_plan(h))
             h.bin0 = nbin
             nbin += h.nbin
@@ -462,6 +559,13 @@ class Engine(object):
         self.charges_d = self._t(charges)
         if self.has_ewald:
             self._alloc_ewald()
+        self.corr_tables = None
+        if self.has_corr:
+            entries = [dict(atom0=self.atom0[i], natom=s.natom, alpha=s.ewald.alpha, dielectric=s.ewald.dielectric,
+                            nlow=s.ewald.nlow, nhigh=s.ewald.nhigh, stab=s.stab, charges=s.charges)
+                       if s.ewald is not None and s.ewald.correction and len(s.stab) > 0 else None
+                       for i, s in enumerate(self.systems)]
+            self.corr_tables = CorrTables(corr_plan(entries, self.npad), dev)
         # MD state
         masses = np.ones(self.npad)
         for i, s in enumerate(self.systems):
@@ -693,6 +797,24 @@ class Engine(object):
                                       self.precision, self.gpos_d.data_ptr() if gpos else None,
                                       1 if accumulate else 0, self.ewald_out_d.data_ptr(), self.stream()))
 
+    def ewald_corr(self, gpos=True, accumulate=True, out=None, out_accumulate=True):
+        """ForcePartEwaldCorrection of every system whose EwaldSpec asks for
+        it: gradient into gpos_d, energy/virial into ``out`` (default: added
+        to ewald_out_d, so the system totals include it)."""
+        if self.corr_tables is None:
+            return
+        out = self.ewald_out_d if out is None else out
+        check(self.lib.ynb_ewald_corr(self.batch, self.corr_tables.c, self.pos_d.data_ptr(),
+                                      self.charges_d.data_ptr(), self.gpos_d.data_ptr() if gpos else None,
+                                      1 if accumulate else 0, out.data_ptr(), 1 if out_accumulate else 0,
+                                      self.stream()))
+
+    def _check_corrected(self):
+        if self.uncorrected:
+            raise ValueError('systems %s have scaled pairs and an Ewald reciprocal part but no Ewald correction: '
+                             'their energies and forces would miss ewald.c:278-349 (use '
+                             'EwaldSpec(..., correction=True))' % self.uncorrected)
+
     def total(self):
         check(self.lib.ynb_total(self.batch, self.pair_out_d.data_ptr(),
                                  self.ewald_out_d.data_ptr() if self.has_ewald else None,
@@ -702,10 +824,12 @@ class Engine(object):
         """Neighbour list, pair potentials and Ewald reciprocal for every
         system; energy+virial totals in total_d, gradient in gpos_d.
         Asynchronous (call nlist_status() to validate the list)."""
+        self._check_corrected()
         if build:
             self.build_nlist(force=False)
         self.pair(gpos=gpos, accumulate=False)
         self.ewald_reci(gpos=gpos, accumulate=True)
+        self.ewald_corr(gpos=gpos)
         self.total()
 
     # ---- MD (velocity Verlet, yaff/sampling/verlet.py:339-367) ----------
@@ -733,6 +857,7 @@ class Engine(object):
         iterations, calls md_step); the sequence starts with a kick and drift
         and ends with a kick, so the state between calls is that of
         md_step."""
+        self._check_corrected()
         self._md_kick_drift()
         for k in range(nsteps):
             self._md_forces()
@@ -772,6 +897,9 @@ class Engine(object):
                 self.ewald_reci(gpos=False)
             finally:
                 ew.flags = 0
+            # the correction adds its gradient to gpos_d and its energy/virial
+            # to ewald_out_d (after the reciprocal pass has written it)
+            self.ewald_corr()
 
     def _md_kick(self, join=False):
         # ... and the kick also totals energy and virial (in place of
