@@ -285,12 +285,21 @@ int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const double *po
                      double *partial, double *out, ynb_stream stream);
 
 /* pair_pot_compute over explicit Yaff rows (local atom indices) for one
- * potential: gpos (nullable) [natom][3] is added to with atomics; out[10] =
- * energy and virial; partial: ceil(nrow/256)*8 doubles.  Parameter arrays are
- * per atom (p1/p2/cross as in ynb_pairpar, nullable when unused). */
+ * potential (yaff/pes/pair_pot.h:61-64).  Parameter arrays are per atom
+ * (p1/p2/cross as in ynb_pairpar, nullable when unused).  out[10] = energy
+ * and, with derivs != 0, the virial (zeros otherwise).  With derivs and gpos
+ * (nullable, [natom][3]) the gradient is ADDED to gpos: each atom sums its
+ * rows in the order atom_ref lists them -- atom_ref0 [natom + 1] offsets,
+ * atom_ref the row references of each atom, 2 r where it is row r's b (+)
+ * and 2 r + 1 where it is a (-), in ascending row order (b before a for a
+ * self-image row), which is the order of pair_pot.c:147-157 -- so the result
+ * is deterministic and needs no atomics.  Work: rowh [nrow][3] (with
+ * gradients) and partial [max(1, ceil(nrow / 256))][8].  nrow <= 2^30. */
 int ynb_pair_rows(const ynb_neigh_row *rows, int64_t nrow, const ynb_scaling_row *stab, int64_t nstab,
                   const ynb_pot *pot, const double *p0, const double *p1, const double *p2,
-                  const double *cross, double *gpos, double *partial, double *out, ynb_stream stream);
+                  const double *cross, int32_t derivs, int64_t natom, const int32_t *atom_ref0,
+                  const int32_t *atom_ref, double *gpos, double *rowh, double *partial, double *out,
+                  ynb_stream stream);
 
 /* Ewald reciprocal term of every system with nk > 0.
  * out: [nsys][10] energy followed by the virial (zeros for systems without k

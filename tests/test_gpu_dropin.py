@@ -101,6 +101,24 @@ def test_forcefield_water32_matches_oracle():
     e2 = ei.compute(nlist.neighs, sc_ei.stab, g2, v2, nlist.nneigh)
     assert abs(e2 - ref[3][0]) <= TOL*abs(ref[3][0])
     assert np.abs(g2 - ref[3][1]).max() <= TOL*np.abs(ref[3][1]).max()
+    assert np.abs(v2 - ref[3][2]).max() <= TOL*np.abs(ref[3][2]).max()
+    # every potential and truncation through the row entry (gradients summed
+    # per atom in row order: bitwise the same on a second call), also with
+    # the virial alone and with neither derivative
+    for ip, (pot, st) in enumerate(((lj, sc_vdw.stab), (mm3, sc_vdw.stab), (ex, sc_vdw.stab), (ei, sc_ei.stab))):
+        ga, va = np.zeros((n, 3)), np.zeros((3, 3))
+        gb, vb = np.zeros((n, 3)), np.zeros((3, 3))
+        ea = pot.compute(nlist.neighs, st, ga, va, nlist.nneigh)
+        eb = pot.compute(nlist.neighs, st, gb, vb, nlist.nneigh)
+        assert ea == eb and np.array_equal(ga, gb) and np.array_equal(va, vb)
+        assert abs(ea - ref[ip][0]) <= TOL*abs(ref[ip][0]), ip
+        assert np.abs(ga - ref[ip][1]).max() <= TOL*np.abs(ref[ip][1]).max(), ip
+        assert np.abs(va - ref[ip][2]).max() <= TOL*np.abs(ref[ip][2]).max(), ip
+        vc = np.zeros((3, 3))
+        ec = pot.compute(nlist.neighs, st, None, vc, nlist.nneigh)
+        assert abs(ec - ea) <= 1e-14*abs(ea) and np.array_equal(vc, va)
+        ed = pot.compute(nlist.neighs, st, None, None, nlist.nneigh)
+        assert abs(ed - ea) <= TOL*abs(ea)
     # a second evaluation after moving the atoms
     pos = system.pos + rng.normal(0, 0.01, system.pos.shape)
     ff.update_pos(pos)

@@ -255,3 +255,25 @@ def test_corr_plan_tables():
     for atom in range(npad):
         refs = p['atom_ref'][p['atom_ref0'][atom]:p['atom_ref0'][atom + 1]]
         assert list(refs >> 1) == sorted(refs >> 1)
+
+
+def test_row_refs_order():
+    """_lib.row_refs: every (row, side) once, each atom's references in
+    ascending row order with b before a on a self-image row."""
+    from yaff_amd import _lib
+    rng = np.random.default_rng(4)
+    n = 300
+    rows = np.zeros(n, _lib.ROW_DTYPE)
+    rows['a'] = rng.integers(0, 20, n)
+    rows['b'] = np.where(rng.uniform(size=n) < 0.1, rows['a'], rng.integers(0, 20, n))
+    ref0, ref = _lib.row_refs(rows, 25)
+    assert ref0[0] == 0 and ref0[-1] == 2*n and sorted(ref) == list(range(2*n))
+    for atom in range(25):
+        mine = ref[ref0[atom]:ref0[atom + 1]]
+        want = []
+        for r in range(n):
+            if rows['b'][r] == atom:
+                want.append(2*r)
+            if rows['a'][r] == atom:
+                want.append(2*r + 1)
+        assert list(mine) == want

@@ -123,7 +123,7 @@ def load(path=LIB_PATH):
     L.ynb_nlist_recompute.argtypes = [P(YnbSys), vp, vp, vp, i64, vp]
     L.ynb_pair_compute.argtypes = [P(YnbBatch), P(YnbPairpar), vp, P(YnbNlist), C.c_uint32, i32, vp, i32, vp,
                                    vp, vp, vp]
-    L.ynb_pair_rows.argtypes = [vp, i64, vp, i64, P(YnbPot), vp, vp, vp, vp, vp, vp, vp, vp]
+    L.ynb_pair_rows.argtypes = [vp, i64, vp, i64, P(YnbPot), vp, vp, vp, vp, i32, i64, vp, vp, vp, vp, vp, vp, vp]
     L.ynb_ewald_reci.argtypes = [P(YnbBatch), P(YnbEwald), vp, vp, i32, vp, i32, vp, vp]
     L.ynb_ewald_corr.argtypes = [P(YnbBatch), P(YnbCorr), vp, vp, vp, i32, vp, i32, vp]
     L.ynb_verlet_kick_drift.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp, P(YnbEwald), vp, vp, sz, P(YnbNlist), vp]
@@ -137,6 +137,23 @@ def load(path=LIB_PATH):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
+
+
+def row_refs(rows, natom):
+    """(atom_ref0, atom_ref) of ynb_pair_rows for Yaff rows (fields a, b):
+    each atom's references in ascending row order, 2 r where it is row r's
+    b and 2 r + 1 where it is a (b first for a self-image row)."""
+    n = len(rows)
+    atoms = np.empty(2*n, np.int64)
+    atoms[0::2] = rows['b']
+    atoms[1::2] = rows['a']
+    refs = np.arange(2*n, dtype=np.int64)
+    refs[0::2] = 2*np.arange(n)
+    refs[1::2] = 2*np.arange(n) + 1
+    order = np.argsort(atoms, kind='stable')
+    ref0 = np.zeros(natom + 1, np.int32)
+    ref0[1:] = np.cumsum(np.bincount(atoms, minlength=natom)[:natom])
+    return ref0, refs[order].astype(np.int32)
 
 
 def check(rc):

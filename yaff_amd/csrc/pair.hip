@@ -672,14 +672,17 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
 }
 
 // pair_pot_compute over explicit Yaff rows (yaff/pes/pair_pot.c:84-176):
-// one thread per row, rows in any order.  Gradients are scattered with FP64
-// atomics; energy and virial are reduced per block and then in order.
+// one thread per row, rows in any order.  With derivs, the row's
+// delta * vg goes to rowh (when gradients are wanted) and its virial terms
+// to the block sums; the energy always.  No atomics: each atom's gradient
+// is summed afterwards by k_pair_rows_gather, and the block sums by
+// k_pair_rows_reduce, both in a fixed order.
 __global__ __launch_bounds__(256) void k_pair_rows(const ynb_neigh_row *rows, int64_t nrow,
                                                    const ynb_scaling_row *stab, int64_t nstab, ynb_pot P,
                                                    const double *p0, const double *p1, const double *p2,
-                                                   const double *cross, double *gpos, double *partial) {
+                                                   const double *cross, int derivs, double *rowh, double *partial) {
   const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  double e = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
+  double e = 0.0, v[6] = {0, 0, 0, 0, 0, 0}, h3[3] = {0.0, 0.0, 0.0};
   if (q < nrow) {
     const ynb_neigh_row w = rows[q];
     if (w.d < P.rcut) {
@@ -701,28 +704,24 @@ __global__ __launch_bounds__(256) void k_pair_rows(const ynb_neigh_row *rows, in
           }
           default: pe = PotEval<YNB_FP64>::ei(p0[w.a], p0[w.b], p1[w.a], p1[w.b], P.alpha, P.dielectric, d, g); break;
         }
-        if (gpos == nullptr) {
+        if (!derivs) {
           if (P.trunc != YNB_TR_NONE && pe != 0.0) {
             double hg;
             pe *= (P.trunc == YNB_TR_SWITCH3) ? PotEval<YNB_FP64>::switch3(d, P.rcut, P.trunc_par, hg)
                                               : PotEval<YNB_FP64>::hammer(d, P.rcut, P.trunc_par, hg);
           }
-        } else if (P.trunc != YNB_TR_NONE && (pe != 0.0 || g != 0.0)) {
-          double hg;
-          const double h = (P.trunc == YNB_TR_SWITCH3) ? PotEval<YNB_FP64>::switch3(d, P.rcut, P.trunc_par, hg)
-                                                        : PotEval<YNB_FP64>::hammer(d, P.rcut, P.trunc_par, hg);
-          g = g * h + pe * hg / d;
-          pe *= h;
-        }
-        g *= s;
-        e = s * pe;
-        if (gpos) {
-          const double dl[3] = {w.dx, w.dy, w.dz};
-          for (int c = 0; c < 3; ++c) {
-            const double h = dl[c] * g;
-            atomicAdd(gpos + 3 * w.b + c, h);
-            atomicAdd(gpos + 3 * w.a + c, -h);
+        } else {
+          if (P.trunc != YNB_TR_NONE && (pe != 0.0 || g != 0.0)) {
+            double hg;
+            const double h = (P.trunc == YNB_TR_SWITCH3) ? PotEval<YNB_FP64>::switch3(d, P.rcut, P.trunc_par, hg)
+                                                          : PotEval<YNB_FP64>::hammer(d, P.rcut, P.trunc_par, hg);
+            g = g * h + pe * hg / d;
+            pe *= h;
           }
+          g *= s;
+          h3[0] = w.dx * g;
+          h3[1] = w.dy * g;
+          h3[2] = w.dz * g;
           v[0] = w.dx * (w.dx * g);
           v[1] = w.dy * (w.dy * g);
           v[2] = w.dz * (w.dz * g);
@@ -730,7 +729,13 @@ __global__ __launch_bounds__(256) void k_pair_rows(const ynb_neigh_row *rows, in
           v[4] = w.dx * (w.dz * g);
           v[5] = w.dy * (w.dz * g);
         }
+        e = s * pe;
       }
+    }
+    if (rowh) {
+      rowh[3 * q] = h3[0];
+      rowh[3 * q + 1] = h3[1];
+      rowh[3 * q + 2] = h3[2];
     }
   }
   __shared__ double red[7][256];
@@ -743,23 +748,53 @@ __global__ __launch_bounds__(256) void k_pair_rows(const ynb_neigh_row *rows, in
       for (int c = 0; c < 7; ++c) red[c][t] += red[c][t + off];
     __syncthreads();
   }
-  if (t < 7) partial[8 * (int64_t)blockIdx.x + t] = red[t][0];
+  if (t < 8) partial[8 * (int64_t)blockIdx.x + t] = t < 7 ? red[t][0] : 0.0;
 }
 
-__global__ void k_pair_rows_reduce(int64_t nblk, const double *partial, double *out) {
+// The block sums of k_pair_rows in a fixed order by one block of 256
+// threads: thread (value c, part p) adds blocks p, p + 32, ... in order, then
+// a tree over the 32 parts; out = {E, V00 V01 V02 V10 V11 V12 V20 V21 V22}.
+__global__ __launch_bounds__(256) void k_pair_rows_reduce(int64_t nblk, const double *partial, double *out) {
+  __shared__ double l_red[256];
   const int t = threadIdx.x;
-  if (t >= 7) return;
-  double acc = 0.0;
-  for (int64_t b = 0; b < nblk; ++b) acc += partial[8 * b + t];
-  __shared__ double r[7];
-  r[t] = acc;
-  __syncwarp();
+  const int c = t & 7, part = t >> 3;
+  double a = 0.0;
+  if (c < 7)
+    for (int64_t b = part; b < nblk; b += 32) a += partial[8 * b + c];
+  l_red[t] = a;
+  __syncthreads();
+  for (int off = 16; off > 0; off >>= 1) {
+    if (part < off) l_red[t] += l_red[t + 8 * off];
+    __syncthreads();
+  }
   if (t == 0) {
+    const double *r = l_red;
     out[0] = r[0];
     out[1] = r[1]; out[2] = r[4]; out[3] = r[5];
     out[4] = r[4]; out[5] = r[2]; out[6] = r[6];
     out[7] = r[5]; out[8] = r[6]; out[9] = r[3];
   }
+}
+
+// gpos[x] += the atom's rows in row order: + delta vg where x is the row's
+// b, - where it is a (atom_ref: 2 r and 2 r + 1), the sequence in which
+// pair_pot_compute accumulates them (pair_pot.c:147-157).  One thread per
+// (atom, component).
+__global__ __launch_bounds__(256) void k_pair_rows_gather(int64_t natom, const int32_t *atom_ref0,
+                                                          const int32_t *atom_ref, const double *rowh,
+                                                          double *gpos) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= 3 * natom) return;
+  const int64_t x = e / 3;
+  const int comp = (int)(e - 3 * x);
+  const int k0 = atom_ref0[x], k1 = atom_ref0[x + 1];
+  double acc = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const int ref = atom_ref[k];
+    const double h = rowh[3 * (int64_t)(ref >> 1) + comp];
+    acc = (ref & 1) ? acc - h : acc + h;
+  }
+  gpos[e] += acc;
 }
 
 }  // namespace
@@ -818,19 +853,51 @@ extern "C" int ynb_pair_compute(const ynb_batch *b, const ynb_pairpar *pp, const
   return YNB_OK;
 }
 
+namespace ynb {
+namespace {
+
+// The three launches of ynb_pair_rows on the given work buffers.
+int pair_rows_launch(const ynb_neigh_row *rows, int64_t nrow, const ynb_scaling_row *stab, int64_t nstab,
+                     const ynb_pot *pot, const double *p0, const double *p1, const double *p2, const double *cross,
+                     int32_t derivs, int64_t natom, const int32_t *atom_ref0, const int32_t *atom_ref, double *gpos,
+                     double *rowh, double *partial, double *out, hipStream_t st) {
+  const int64_t nblk = nrow > 0 ? ceil_div(nrow, 256) : 1;
+  const bool grad = derivs && gpos != nullptr;
+  YNB_LAUNCH("pair_rows", k_pair_rows, dim3((unsigned)nblk), dim3(256), 0, st, rows, nrow, stab, nstab, *pot, p0,
+             p1 ? p1 : p0, p2 ? p2 : p0, cross ? cross : p0, derivs ? 1 : 0, grad ? rowh : nullptr, partial);
+  YNB_LAUNCHED();
+  YNB_LAUNCH("pair_rows_reduce", k_pair_rows_reduce, dim3(1), dim3(256), 0, st, nblk, partial, out);
+  YNB_LAUNCHED();
+  if (grad && natom > 0) {
+    YNB_LAUNCH("pair_rows_gather", k_pair_rows_gather, dim3((unsigned)ceil_div(3 * natom, 256)), dim3(256), 0, st,
+               natom, atom_ref0, atom_ref, rowh, gpos);
+    YNB_LAUNCHED();
+  }
+  return YNB_OK;
+}
+
+bool pair_rows_args_ok(const ynb_neigh_row *rows, int64_t nrow, const ynb_pot *pot, const double *p0,
+                       int32_t derivs, int64_t natom, const int32_t *atom_ref0, const int32_t *atom_ref,
+                       double *gpos, const double *out) {
+  if (!pot || nrow < 0 || nrow > (int64_t)1 << 30 || !out || (nrow > 0 && !rows) || !p0 || natom < 0) return false;
+  if (derivs && gpos && (!atom_ref0 || (nrow > 0 && !atom_ref))) return false;
+  return true;
+}
+
+}  // namespace
+}  // namespace ynb
+
 extern "C" int ynb_pair_rows(const ynb_neigh_row *rows, int64_t nrow, const ynb_scaling_row *stab, int64_t nstab,
                              const ynb_pot *pot, const double *p0, const double *p1, const double *p2,
-                             const double *cross, double *gpos, double *partial, double *out, ynb_stream stream) {
-  if (!pot || nrow < 0 || !partial || !out || (nrow > 0 && !rows) || !p0) {
+                             const double *cross, int32_t derivs, int64_t natom, const int32_t *atom_ref0,
+                             const int32_t *atom_ref, double *gpos, double *rowh, double *partial, double *out,
+                             ynb_stream stream) {
+  if (!pair_rows_args_ok(rows, nrow, pot, p0, derivs, natom, atom_ref0, atom_ref, gpos, out) || !partial ||
+      (derivs && gpos && nrow > 0 && !rowh)) {
     set_error("ynb_pair_rows: invalid arguments");
     return YNB_EINVAL;
   }
-  hipStream_t st = (hipStream_t)stream;
-  const int64_t nblk = nrow > 0 ? ceil_div(nrow, 256) : 1;
-  hipLaunchKernelGGL(k_pair_rows, dim3((unsigned)nblk), dim3(256), 0, st, rows, nrow, stab, nstab, *pot, p0,
-                     p1 ? p1 : p0, p2 ? p2 : p0, cross ? cross : p0, gpos, partial);
-  YNB_LAUNCHED();
-  hipLaunchKernelGGL(k_pair_rows_reduce, dim3(1), dim3(64), 0, st, nblk, partial, out);
-  YNB_LAUNCHED();
-  return YNB_OK;
+  return pair_rows_launch(rows, nrow, stab, nstab, pot, p0, p1, p2, cross, derivs, natom, atom_ref0, atom_ref, gpos,
+                          rowh, partial, out, (hipStream_t)stream);
 }
+

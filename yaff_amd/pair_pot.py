@@ -101,7 +101,9 @@ class PairPot(object):
 
     def compute(self, neighs, stab, gpos, vtens, nneigh):
         """pair_pot_compute over explicit rows (ext.pyx:689-744, pair_pot.c:84-176),
-        evaluated on the device.  gpos/vtens are added to; returns the energy."""
+        evaluated on the device.  gpos/vtens are added to; returns the energy.
+        Each atom's gradient is summed over its rows in row order (the index
+        is made here with a stable sort), so the result is deterministic."""
         from .engine import Engine
         if not torch.cuda.is_available():
             raise RuntimeError('PairPot.compute needs a HIP device')
@@ -128,13 +130,20 @@ class PairPot(object):
         if spec.amp_cross is not None:
             pot.nffatype = spec.amp_cross.shape[0]
             cross = up(np.concatenate([np.ravel(spec.amp_cross), np.ravel(spec.b_cross)]))
-        g_d = torch.zeros((natom, 3), dtype=torch.float64, device=dev) if gpos is not None else None
+        derivs = gpos is not None or vtens is not None
+        g_d = ref0_d = ref_d = rowh = None
+        if gpos is not None:
+            g_d = torch.zeros((natom, 3), dtype=torch.float64, device=dev)
+            ref0, ref = _lib.row_refs(neighs, natom)
+            ref0_d, ref_d = up(ref0, np.int32), up(ref if len(ref) else np.zeros(1, np.int32), np.int32)
+            rowh = torch.empty(3*max(len(neighs), 1), dtype=torch.float64, device=dev)
         nblk = max((len(neighs) + 255)//256, 1)
-        partial = torch.zeros(8*nblk, dtype=torch.float64, device=dev)
+        partial = torch.empty(8*nblk, dtype=torch.float64, device=dev)
         out = torch.zeros(10, dtype=torch.float64, device=dev)
         ptr = lambda t: None if t is None else t.data_ptr()
         _lib.check(lib.ynb_pair_rows(rows_d.data_ptr(), len(neighs), stab_d.data_ptr(), len(stab), pot, ptr(p0),
-                                     ptr(p1), ptr(p2), ptr(cross), ptr(g_d), partial.data_ptr(), out.data_ptr(),
+                                     ptr(p1), ptr(p2), ptr(cross), 1 if derivs else 0, natom, ptr(ref0_d),
+                                     ptr(ref_d), ptr(g_d), ptr(rowh), partial.data_ptr(), out.data_ptr(),
                                      Engine.stream()))
         o = out.cpu().numpy()
         if gpos is not None:
