@@ -301,6 +301,20 @@ int ynb_pair_rows(const ynb_neigh_row *rows, int64_t nrow, const ynb_scaling_row
                   const int32_t *atom_ref, double *gpos, double *rowh, double *partial, double *out,
                   ynb_stream stream);
 
+/* ynb_pair_rows with no work arguments: rowh and partial live in a static
+ * device array of the library (one per GPU, held until the process ends),
+ * so the call allocates nothing, for callers that cannot allocate (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  At most YNB_PAIR_ROWS_STATIC_MAX_ROWS rows
+ * (YNB_ECAPACITY beyond: about 6 MB of device memory per GPU).  All calls on
+ * one GPU share the array: calls that overlap on one GPU (other streams,
+ * other host threads) race and must be serialised by the caller. */
+#define YNB_PAIR_ROWS_STATIC_MAX_ROWS 262144
+int ynb_pair_rows_static(const ynb_neigh_row *rows, int64_t nrow, const ynb_scaling_row *stab, int64_t nstab,
+                         const ynb_pot *pot, const double *p0, const double *p1, const double *p2,
+                         const double *cross, int32_t derivs, int64_t natom, const int32_t *atom_ref0,
+                         const int32_t *atom_ref, double *gpos, double *out, ynb_stream stream);
+
 /* Ewald reciprocal term of every system with nk > 0.
  * out: [nsys][10] energy followed by the virial (zeros for systems without k
  * vectors); gpos as in ynb_pair_compute.  The partial gradients of each

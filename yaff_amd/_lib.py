@@ -21,6 +21,7 @@ BATCH_WIDE_SEARCH = 32
 NLIST_PREBINNED = 1
 EWALD_PREPARED = 1
 FORM_GENERIC, FORM_LJ, FORM_IONIC = 0, 1, 2
+PAIR_ROWS_STATIC_MAX_ROWS = 262144
 
 ROW_DTYPE = np.dtype([('a', np.int64), ('b', np.int64), ('d', np.float64), ('dx', np.float64),
                       ('dy', np.float64), ('dz', np.float64), ('r0', np.int64), ('r1', np.int64),
@@ -88,7 +89,7 @@ class YnbCorr(C.Structure):
 
 EXPORTS = ['ynb_version', 'ynb_last_error', 'ynb_sys_plan', 'ynb_nlist_workspace_bytes',
            'ynb_rows_workspace_bytes', 'ynb_nlist_build', 'ynb_nlist_rows', 'ynb_nlist_recompute',
-           'ynb_pair_compute', 'ynb_pair_rows', 'ynb_ewald_reci', 'ynb_ewald_corr', 'ynb_verlet_kick_drift', 'ynb_verlet_kick',
+           'ynb_pair_compute', 'ynb_pair_rows', 'ynb_pair_rows_static', 'ynb_ewald_reci', 'ynb_ewald_corr', 'ynb_verlet_kick_drift', 'ynb_verlet_kick',
            'ynb_verlet_kick_kick_drift',
            'ynb_total', 'ynb_profile_enable', 'ynb_profile_count', 'ynb_profile_read',
            'ynb_nlist_skin_check']
@@ -124,6 +125,7 @@ def load(path=LIB_PATH):
     L.ynb_pair_compute.argtypes = [P(YnbBatch), P(YnbPairpar), vp, P(YnbNlist), C.c_uint32, i32, vp, i32, vp,
                                    vp, vp, vp]
     L.ynb_pair_rows.argtypes = [vp, i64, vp, i64, P(YnbPot), vp, vp, vp, vp, i32, i64, vp, vp, vp, vp, vp, vp, vp]
+    L.ynb_pair_rows_static.argtypes = [vp, i64, vp, i64, P(YnbPot), vp, vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]
     L.ynb_ewald_reci.argtypes = [P(YnbBatch), P(YnbEwald), vp, vp, i32, vp, i32, vp, vp]
     L.ynb_ewald_corr.argtypes = [P(YnbBatch), P(YnbCorr), vp, vp, vp, i32, vp, i32, vp]
     L.ynb_verlet_kick_drift.argtypes = [P(YnbBatch), vp, vp, vp, vp, vp, P(YnbEwald), vp, vp, sz, P(YnbNlist), vp]

@@ -14,6 +14,8 @@
 // r == 0 rows see Yaff's bits.  Mixed mode takes the image vector
 // pos_j - pos_i + n.R in FP64 and evaluates the potentials in FP32,
 // accumulating gradient, energy and virial in FP64.
+#include <mutex>
+
 #include "common.h"
 #include "erfcx_table.h"
 
@@ -901,3 +903,67 @@ extern "C" int ynb_pair_rows(const ynb_neigh_row *rows, int64_t nrow, const ynb_
                           rowh, partial, out, (hipStream_t)stream);
 }
 
+// ---------------------------------------------------------------------------
+// ynb_pair_rows_static: ynb_pair_rows with its work buffers (rowh, partial) in
+// a file-scope device array instead of caller memory, for callers that cannot
+// allocate.  The array's host-side name is no device address: its address on
+// each device comes from hipGetSymbolAddress, made once per device (with that
+// device current) on the first call there, and only that address is passed to
+// the kernels.  Every call on one device shares the array; nothing in it needs
+// clearing (each launch writes what it reads).
+static __device__ __align__(256) double scratch[3 * YNB_PAIR_ROWS_STATIC_MAX_ROWS +
+                                                8 * (YNB_PAIR_ROWS_STATIC_MAX_ROWS / 256)];
+
+namespace ynb {
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+int scratch_address(double **ptr) {
+  int dev = -1;
+  YNB_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+    set_error("ynb_pair_rows_static: device %d beyond the %d devices the library tracks", dev, SCRATCH_MAX_DEVICES);
+    return YNB_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("ynb_pair_rows_static: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                hipGetErrorString(e));
+      return YNB_EHIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return YNB_OK;
+}
+
+}  // namespace
+}  // namespace ynb
+
+extern "C" int ynb_pair_rows_static(const ynb_neigh_row *rows, int64_t nrow, const ynb_scaling_row *stab,
+                                    int64_t nstab, const ynb_pot *pot, const double *p0, const double *p1,
+                                    const double *p2, const double *cross, int32_t derivs, int64_t natom,
+                                    const int32_t *atom_ref0, const int32_t *atom_ref, double *gpos, double *out,
+                                    ynb_stream stream) {
+  if (!pair_rows_args_ok(rows, nrow, pot, p0, derivs, natom, atom_ref0, atom_ref, gpos, out)) {
+    set_error("ynb_pair_rows_static: invalid arguments");
+    return YNB_EINVAL;
+  }
+  if (nrow > YNB_PAIR_ROWS_STATIC_MAX_ROWS) {
+    set_error("ynb_pair_rows_static: %lld rows exceed the static work array of %d rows (use ynb_pair_rows)",
+              (long long)nrow, YNB_PAIR_ROWS_STATIC_MAX_ROWS);
+    return YNB_ECAPACITY;
+  }
+  double *base = nullptr;
+  const int rc = scratch_address(&base);
+  if (rc != YNB_OK) return rc;
+  return pair_rows_launch(rows, nrow, stab, nstab, pot, p0, p1, p2, cross, derivs, natom, atom_ref0, atom_ref, gpos,
+                          base, base + 3 * (int64_t)YNB_PAIR_ROWS_STATIC_MAX_ROWS, out, (hipStream_t)stream);
+}
