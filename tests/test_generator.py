@@ -146,3 +146,42 @@ def test_units_and_chk_round_trip(tmp_path):
     e = load_chk(str(out))
     assert np.array_equal(e['pos'], d['pos']) and np.array_equal(e['numbers'], d['numbers'])
     assert list(e['ffatypes']) == list(d['ffatypes']) and e['energy'] == -1.5 and e['n'] == 3 and e['charges'] is None
+
+
+LJ_OK = ('LJ:UNIT SIGMA angstrom\nLJ:UNIT EPSILON kcalmol\nLJ:SCALE 1 0.0\nLJ:SCALE 2 0.0\nLJ:SCALE 3 1.0\n'
+         'LJ:PARS O 3.15 0.1521\nLJ:PARS H 0.4 0.046\n')
+
+
+@pytest.mark.parametrize('name,text,exc', [
+    ('duplicate pars', LJ_OK + 'LJ:PARS O 3.0 0.1\n', IOError),
+    ('scale range', LJ_OK.replace('SCALE 3 1.0', 'SCALE 3 1.5'), IOError),
+    ('scale missing', LJ_OK.replace('LJ:SCALE 3 1.0\n', ''), IOError),
+    ('scale conflict', LJ_OK + 'LJ:SCALE 2 0.5\n', IOError),
+    ('unknown suffix', LJ_OK + 'LJ:CPARS O H 1.0 2.0\n', IOError),
+    ('missing unit', LJ_OK.replace('LJ:UNIT EPSILON kcalmol\n', ''), IOError),
+    ('bad unit', LJ_OK.replace('kcalmol', 'furlong'), IOError),
+    ('not a number', LJ_OK.replace('0.1521', 'x'), IOError),
+    ('mix rule', open(os.path.join(PAR, 'parameters_water_exprep1.txt')).read().replace('GEOMETRIC', 'HARMONIC'),
+     IOError),
+    ('fixq no dielectric', '\n'.join(l for l in open(os.path.join(PAR, 'parameters_water_fixq.txt')).read()
+                                     .splitlines() if 'DIELECTRIC' not in l), TypeError),
+])
+def test_generator_errors(tmp_path, name, text, exc):
+    p = tmp_path / 'pars.txt'
+    p.write_text(text)
+    with pytest.raises(exc):
+        ya.ForceField.generate(water32(), str(p))
+
+
+def test_parameters_write_round_trip(tmp_path):
+    from yaff_amd.parameters import Parameters
+    a = Parameters.from_file([os.path.join(PAR, 'parameters_water_exprep1.txt'),
+                              os.path.join(PAR, 'parameters_water_fixq.txt')])
+    out = tmp_path / 'out.txt'
+    a.copy().write_to_file(str(out))
+    b = Parameters.from_file(str(out))
+    assert sorted(a.sections) == sorted(b.sections) == ['EXPREP', 'FIXQ']
+    for prefix in a.sections:
+        for suffix, d in a[prefix].definitions.items():
+            assert [x for _, x in d] == [x for _, x in b[prefix][suffix]]
+    assert len(b['LJ']['PARS']) == 0 and 'EXPREP' in b and 'MIX' in b['EXPREP']

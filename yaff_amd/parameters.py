@@ -1,27 +1,98 @@
-"""Yaff parameter files (yaff/pes/parameters.py:35-206).
+"""Parameter files in Yaff's format (read as yaff/pes/parameters.py reads them).
 
-Line-based: ``PREFIX:SUFFIX DATA``, comments from ``#`` to the end of the
-line, blank lines ignored, the order of lines free.  Prefixes name sections
-(one generator each), suffixes name definitions; a definition keeps its data
-lines with their line numbers so errors can name them.
+The format is line based: ``PREFIX:SUFFIX DATA``; ``#`` starts a comment
+that runs to the end of the line; blank lines and the order of lines do not
+matter.  A prefix selects a section (one generator), a suffix a definition
+inside it; prefixes and suffixes are case-insensitive (kept upper case), the
+data are kept as written.  Every data line remembers its file and line number
+so that the generators can point at the offending line.
+
+Reading is one pass: each line is matched against the grammar below and the
+first rule it breaks is reported as an ``IOError`` (the exception type of the
+reference) naming the file and the line.
 """
+import collections
+import re
+
 __all__ = ['Complain', 'Parameters', 'ParameterSection', 'ParameterDefinition']
+
+# PREFIX ':' SUFFIX whitespace DATA, with optional blanks around the colon
+_LINE = re.compile(r'^(?P<prefix>[^:]*):\s*(?P<suffix>\S*)(?P<sep>\s*)(?P<data>.*)$')
 
 
 class Complain(object):
-    """parameters.py:35-43: raises IOError naming the file and line."""
+    """Raises IOError for a problem in a parameter file; ``where`` is a line
+    number or None (a problem of a whole section or definition)."""
 
     def __init__(self, filename='__nofile__'):
         self.filename = filename
 
-    def __call__(self, counter, message):
-        if counter is None:
-            raise IOError('The parameter file %s %s.' % (self.filename, message))
-        raise IOError('Line %i in the parameter file %s %s.' % (counter, self.filename, message))
+    def __call__(self, where, message):
+        place = 'parameter file %s' % self.filename if where is None else \
+            'parameter file %s, line %i' % (self.filename, where)
+        raise IOError('%s: %s' % (place, message))
+
+
+def _parse_line(text, number, complain):
+    """(PREFIX, SUFFIX, data) of one non-empty, comment-stripped line."""
+    m = _LINE.match(text)
+    if m is None:
+        complain(number, 'a line needs the form PREFIX:SUFFIX DATA (no colon found)')
+    prefix = m.group('prefix').strip()
+    if not prefix or len(prefix.split()) != 1:
+        complain(number, 'the prefix before the colon must be a single word')
+    if not m.group('suffix'):
+        complain(number, 'nothing follows the colon')
+    if not m.group('data'):
+        complain(number, 'the definition %s has no data' % m.group('suffix'))
+    return prefix.upper(), m.group('suffix').upper(), m.group('data')
+
+
+class ParameterDefinition(object):
+    """The data lines of one PREFIX:SUFFIX, as (line number, data) pairs in
+    file order; iterating yields those pairs."""
+
+    def __init__(self, suffix, lines=None, complain=None):
+        self.suffix = suffix
+        self.lines = [] if lines is None else lines
+        self.complain = Complain() if complain is None else complain
+
+    def __iter__(self):
+        return iter(self.lines)
+
+    def __getitem__(self, index):
+        return self.lines[index]
+
+    def __len__(self):
+        return len(self.lines)
+
+    def copy(self):
+        return ParameterDefinition(self.suffix, list(self.lines), self.complain)
+
+
+class ParameterSection(object):
+    """The definitions of one prefix.  Looking up a suffix that the file
+    does not have gives an empty definition."""
+
+    def __init__(self, prefix, definitions=None, complain=None):
+        self.prefix = prefix
+        self.definitions = {} if definitions is None else definitions
+        self.complain = Complain() if complain is None else complain
+
+    def __getitem__(self, suffix):
+        found = self.definitions.get(suffix.upper())
+        return found if found is not None else ParameterDefinition(suffix.upper(), [], self.complain)
+
+    def __contains__(self, suffix):
+        return suffix.upper() in self.definitions
+
+    def copy(self):
+        return ParameterSection(self.prefix, {k: v.copy() for k, v in self.definitions.items()}, self.complain)
 
 
 class Parameters(object):
-    """parameters.py:46-162"""
+    """All sections of one or more parameter files (later files add lines to
+    the sections of earlier ones)."""
 
     def __init__(self, sections=None):
         self.sections = {} if sections is None else sections
@@ -30,91 +101,43 @@ class Parameters(object):
     def from_file(cls, filenames):
         if isinstance(filenames, str):
             filenames = [filenames]
-        result = cls({})
+        grouped = collections.OrderedDict()
+        complains = {}
         for filename in filenames:
             complain = Complain(filename)
             with open(filename) as f:
-                for counter, line in enumerate(f, 1):
-                    hash_at = line.find('#')
-                    if hash_at >= 0:
-                        line = line[:hash_at]
-                    line = line.strip()
-                    if not line:
+                for number, raw in enumerate(f, 1):
+                    text = raw.split('#', 1)[0].strip()
+                    if not text:
                         continue
-                    prefix, suffix, data = cls._split(line, counter, complain)
-                    section = result.sections.get(prefix)
-                    if section is None:
-                        section = result.sections[prefix] = ParameterSection(prefix, {}, complain)
-                    definition = section.definitions.get(suffix)
-                    if definition is None:
-                        definition = section.definitions[suffix] = ParameterDefinition(suffix, [], complain)
-                    definition.lines.append((counter, data))
-        return result
-
-    @staticmethod
-    def _split(line, counter, complain):
-        colon = line.find(':')
-        if colon == -1:
-            complain(counter, 'does not contain a colon')
-        prefix = line[:colon].upper()
-        rest = line[colon + 1:].strip()
-        if not rest:
-            complain(counter, 'does not have text after the colon')
-        if len(prefix.split()) > 1:
-            complain(counter, 'has a prefix that contains whitespace')
-        space = rest.find(' ')
-        if space == 0:
-            complain(counter, 'does not have a definition after the prefix')
-        elif space == -1:
-            complain(counter, 'does not have data after the definition')
-        return prefix, rest[:space].upper(), rest[space + 1:]
-
-    def copy(self):
-        return Parameters({p: s.copy() for p, s in self.sections.items()})
-
-    def write_to_file(self, filename):
-        with open(filename, 'w') as f:
-            for prefix, section in self.sections.items():
-                for suffix, definition in section.definitions.items():
-                    for counter, data in definition.lines:
-                        f.write('%s:%s %s\n' % (prefix, suffix, data))
-                    f.write('\n')
-                f.write('\n\n')
+                    prefix, suffix, data = _parse_line(text, number, complain)
+                    complains.setdefault(prefix, complain)
+                    grouped.setdefault(prefix, collections.OrderedDict()).setdefault(suffix, []).append(
+                        (number, data))
+        sections = {}
+        for prefix, defs in grouped.items():
+            complain = complains[prefix]
+            sections[prefix] = ParameterSection(
+                prefix, {sfx: ParameterDefinition(sfx, lines, complain) for sfx, lines in defs.items()}, complain)
+        return cls(sections)
 
     def __getitem__(self, prefix):
-        s = self.sections.get(prefix.upper())
-        return ParameterSection(prefix, {}) if s is None else s
+        found = self.sections.get(prefix.upper())
+        return found if found is not None else ParameterSection(prefix.upper())
 
-
-class ParameterSection(object):
-    """parameters.py:165-186"""
-
-    def __init__(self, prefix, definitions=None, complain=None):
-        self.prefix = prefix
-        self.definitions = {} if definitions is None else definitions
-        self.complain = Complain() if complain is None else complain
-
-    def __getitem__(self, suffix):
-        d = self.definitions.get(suffix.upper())
-        return ParameterDefinition(suffix, [], self.complain) if d is None else d
+    def __contains__(self, prefix):
+        return prefix.upper() in self.sections
 
     def copy(self):
-        return ParameterSection(self.prefix, {k: d.copy() for k, d in self.definitions.items()}, self.complain)
+        return Parameters({k: v.copy() for k, v in self.sections.items()})
 
-
-class ParameterDefinition(object):
-    """parameters.py:189-206"""
-
-    def __init__(self, suffix, lines=None, complain=None):
-        self.suffix = suffix
-        self.lines = [] if lines is None else lines
-        self.complain = Complain() if complain is None else complain
-
-    def __getitem__(self, index):
-        return self.lines[index]
-
-    def __iter__(self):
-        return iter(self.lines)
-
-    def copy(self):
-        return ParameterDefinition(self.suffix, list(self.lines), self.complain)
+    def write_to_file(self, filename):
+        """Writes every data line back as PREFIX:SUFFIX DATA, section by
+        section (line numbers are not kept)."""
+        out = []
+        for prefix, section in self.sections.items():
+            for suffix, definition in section.definitions.items():
+                out.extend('%s:%s %s' % (prefix, suffix, data) for _, data in definition)
+            out.append('')
+        with open(filename, 'w') as f:
+            f.write('\n'.join(out) + '\n')
