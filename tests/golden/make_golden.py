@@ -4,7 +4,9 @@ the reference tree and oracle/_ref exist):
     make -C oracle && python tests/golden/make_golden.py
 
 * systems.npz     -- coordinate fixtures of the reference test suite
-                     (yaff/test/common.py), read as text with the ast module
+                     (yaff/test/common.py: water32, water, quartz, graphene8,
+                     polyethylene4, caffeine, the 76-atom MIL-53 cell), read
+                     as text with the ast module
                      (nothing of the reference is imported or executed), in
                      bohr; quartz is also stored as its 2x2x2 supercell.
 * ref_outputs.npz -- outputs of molmod/yaff's own C kernels (oracle/_ref,
@@ -58,14 +60,18 @@ def extract_systems():
     tree = ast.parse(open(REF_COMMON).read())
     wanted = {'get_system_water32': 'water32', 'get_system_quartz': 'quartz', 'get_system_graphene8': 'graphene8',
               'get_system_polyethylene4': 'polyethylene4', 'get_system_caffeine': 'caffeine',
-              'get_system_water': 'water'}
+              'get_system_water': 'water', 'get_system_mil53': 'mil53'}
     out = {}
     for fn in tree.body:
         if not isinstance(fn, ast.FunctionDef) or fn.name not in wanted:
             continue
         name = wanted[fn.name]
         call = [n for n in ast.walk(fn) if isinstance(n, ast.Call) and getattr(n.func, 'id', None) == 'System'][0]
-        kw = {k.arg: k.value for k in call.keywords}
+        # keyword values that name a local variable (numbers=numbers) take that variable's assignment
+        local = {a.targets[0].id: a.value for a in fn.body
+                 if isinstance(a, ast.Assign) and isinstance(a.targets[0], ast.Name)}
+        kw = {k.arg: local.get(k.value.id, k.value) if isinstance(k.value, ast.Name) else k.value
+              for k in call.keywords}
         out[name + '_numbers'] = np.asarray(_literal(kw['numbers']), np.int64)
         out[name + '_pos'] = np.asarray(_literal(kw['pos']), float)
         if 'rvecs' in kw and not (isinstance(kw['rvecs'], ast.Constant) and kw['rvecs'].value is None):

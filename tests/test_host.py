@@ -277,3 +277,69 @@ def test_row_refs_order():
             if rows['a'][r] == atom:
                 want.append(2*r + 1)
         assert list(mine) == want
+
+
+def test_scalings_check_mic_and_bond_paths():
+    """Scalings.check_mic (yaff/pes/scaling.py:108-170): the reference's
+    MIL-53 unit cell (yaff/test/common.py:308) is too small for its 1-3/1-4
+    exclusions (AssertionError, test_scaling.py:102-105), water32 and quartz
+    2x2x2 are fine; bond_paths on caffeine gives the paths of
+    test_scaling.py:78-99."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden'))
+    import cases
+    from yaff_amd import Scalings, System
+    from yaff_amd.scaling import bond_paths
+
+    def fixture(name):
+        f = cases.fixture_arrays(name)
+        return System(f['numbers'], f['pos'], bonds=f['bonds'], rvecs=f.get('rvecs'))
+    with pytest.raises(AssertionError):
+        Scalings(fixture('mil53'))
+    Scalings(fixture('mil53'), 1.0, 1.0, 1.0, 1.0)   # nothing scaled: nothing to check
+    Scalings(fixture('water32'), 0.0, 0.0, 0.5)
+    Scalings(fixture('quartz222'), 0.0, 0.0, 0.0)
+    caf = fixture('caffeine')
+    assert set(bond_paths(caf, 2, 8, 3)) == {(2, 7, 6, 8), (2, 9, 4, 8)}
+    assert set(bond_paths(caf, 13, 5, 5)) == {(13, 4, 8, 6, 7, 5), (13, 4, 9, 2, 7, 5)}
+    assert set(bond_paths(caf, 18, 19, 2)) == {(18, 12, 19)}
+
+
+def test_failure_detection():
+    """The reference's guards, on the host before any device work:
+    ForcePart.compute refuses NaN energies, gradients and virials with
+    ValueError (yaff/pes/ff.py:158-167); NeighborList.update refuses more
+    than 10 atoms per bohr^3 (yaff/pes/nlist.py:160-162) and a negative skin
+    (nlist.py:93-94); nlow/nhigh are checked (ff.py:1265-1277)."""
+    from yaff_amd import System, NeighborList
+    from yaff_amd.ff import ForcePart, check_nlow_nhigh
+    rng = np.random.default_rng(0)
+    s = System(np.ones(100, int), rng.uniform(0, 2.0, (100, 3)), rvecs=np.eye(3)*2.0)
+
+    class Fake(ForcePart):
+        def __init__(self, system, what):
+            ForcePart.__init__(self, 'fake', system)
+            self.what = what
+
+        def _internal_compute(self, gpos, vtens):
+            if self.what == 'gpos' and gpos is not None:
+                gpos[3, 1] = np.nan
+            if self.what == 'vtens' and vtens is not None:
+                vtens[2, 2] = np.nan
+            return np.nan if self.what == 'energy' else 1.0
+    for what in ('energy', 'gpos', 'vtens'):
+        with pytest.raises(ValueError):
+            Fake(s, what).compute(np.zeros((100, 3)), np.zeros((3, 3)))
+    assert Fake(s, 'gpos').compute() == 1.0    # no gradient asked: nothing to check
+    nl = NeighborList(s)
+    nl.request_rcut(1.0)
+    with pytest.raises(ValueError, match='density'):
+        nl.update()
+    with pytest.raises(ValueError):
+        NeighborList(s, skin=-1.0)
+    with pytest.raises(ValueError):
+        check_nlow_nhigh(s, -1, -1)
+    with pytest.raises(ValueError):
+        check_nlow_nhigh(s, 50, 10)
+    assert check_nlow_nhigh(s, 5, -1) == (5, 100)
