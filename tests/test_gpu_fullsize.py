@@ -107,3 +107,49 @@ def test_rocksalt55296_triclinic_invariants():
     rms = np.sqrt(((gm - g)**2).sum(axis=1).mean()/(g**2).sum(axis=1).mean())
     print('mixed: rel dE %.2e, rms rel dG %.2e, rel dV %.2e' % (abs(em - e)/abs(e), rms, _rel(vm, v)), flush=True)
     assert abs(em - e) <= 1e-6*abs(e) and rms <= 1e-5 and _rel(vm, v) <= 1e-6
+
+
+def test_rocksalt55296_matches_reference_fixture():
+    """The precision workload against molmod/yaff's own C kernels (fixture
+    tests/golden/rocksalt55296_ref.npz, written by
+    make_rocksalt55296_golden.py): FP64 rows bit-identical (count and
+    SHA-256), energy and virial to 1e-10, the gradient of 8192 seeded atoms
+    to 1e-10 and 8 random projections of the whole gradient to 1e-10 of
+    their bound; mixed precision within its contract (energy and virial
+    1e-6, RMS force error over the sample 1e-5)."""
+    import hashlib
+    import os
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    path = os.path.join(here, 'golden', 'rocksalt55296_ref.npz')
+    ref = np.load(path)
+    sys.path.insert(0, os.path.join(here, 'golden'))
+    from make_rocksalt55296_golden import workload, sample_and_weights
+    s, angles = workload()
+    assert s.natom == int(ref['natom']) == 55296 and np.allclose(angles, ref['angles'])
+    print('case rocksalt55296 vs reference fixture', flush=True)
+    idx, w = sample_and_weights(s.natom)
+    assert np.array_equal(idx, ref['sample'])
+    eng = _gpu(s, 'fp64')
+    assert eng.nlist_status()[0] == 0
+    rows = eng.rows(0)
+    assert len(rows) == int(ref['nrows'])
+    assert hashlib.sha256(rows.tobytes()).hexdigest() == str(ref['rows_sha256'])
+    del rows
+    e, g, v = eng.results(0)
+    gnorm = np.sqrt(float(ref['gpos_sumsq']))
+    print('fp64: rel dE %.2e, rel dV %.2e, sample rel dG %.2e' % (
+        abs(e - ref['e'])/abs(ref['e']), _rel(v, ref['vtens']), _rel(g[idx], ref['gpos_sample'])), flush=True)
+    assert abs(e - ref['e']) <= TOL*abs(ref['e'])
+    assert _rel(v, ref['vtens']) <= TOL
+    assert _rel(g[idx], ref['gpos_sample']) <= TOL
+    proj = np.einsum('kij,ij->k', w, g)
+    bound = np.sqrt((w*w).sum(axis=(1, 2)))*gnorm
+    assert (np.abs(proj - ref['gpos_proj']) <= TOL*bound).all()
+    del eng
+    em, gm, vm = _gpu(s, 'mixed').results(0)
+    gs = ref['gpos_sample']
+    rms = np.sqrt(((gm[idx] - gs)**2).sum(axis=1).mean()/(gs**2).sum(axis=1).mean())
+    print('mixed vs reference: rel dE %.2e, rms rel dG (sample) %.2e, rel dV %.2e' % (
+        abs(em - ref['e'])/abs(ref['e']), rms, _rel(vm, ref['vtens'])), flush=True)
+    assert abs(em - ref['e']) <= 1e-6*abs(ref['e']) and rms <= 1e-5 and _rel(vm, ref['vtens']) <= 1e-6
