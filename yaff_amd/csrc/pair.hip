@@ -17,191 +17,10 @@
 #include <mutex>
 
 #include "common.h"
-#include "erfcx_table.h"
+#include "pair_eval.h"
 
 namespace ynb {
 namespace {
-
-constexpr double TWO_DIV_SQRT_PI = 1.1283791670955125739;  // yaff/pes/constants.h:31
-
-// Transcendentals of the two precisions.
-//
-// FP64: exp_fp64 (Cody-Waite, Taylor to r^13); erfc(x) = exp(-x^2) erfcx(x) with erfcx
-// a degree-24 polynomial in y = 4/(2 + x) - 1 (relative error < 3e-14 on
-// [0, 27], table generated by gen_erfcx_table.py), so that erfc and the EI
-// gradient share one exp(-x^2).
-//
-// Mixed: every argument reduction in FP64, only a well-conditioned
-// remainder in FP32:
-//   exp(a)   = 2^k * exp2f(f),  k = rint(a log2 e), f = a log2 e - k in FP64;
-//   erfc(x)  = exp(-x^2) * erfcxf(x), erfcx = exp(x^2) erfc(x) being smooth,
-//              so rounding x to FP32 barely moves it.
-__device__ inline double erfcx_poly(double x) {  // x >= 0
-  // y = 4 / (2 + x) - 1; p(y) = E(y^2) + y O(y^2), the even and odd parts by
-  // Horner's rule in y^2 (two independent chains of 12 and 11 steps)
-  const double y = fma(4.0, rcp_fp64(2.0 + x), -1.0);
-  const double z = y * y;
-  double e = ERFCX_MONO[ERFCX_NCOEF - 1];
-  double o = ERFCX_MONO[ERFCX_NCOEF - 2];
-#pragma unroll
-  for (int j = (ERFCX_NCOEF - 3) / 2; j >= 0; --j) {
-    e = fma3(e, z, ERFCX_MONO[2 * j]);
-    if (2 * j + 1 < ERFCX_NCOEF - 2) o = fma3(o, z, ERFCX_MONO[2 * j + 1]);
-  }
-  return fma(y, o, e);
-}
-
-template <int PREC>
-struct Tr {
-  __device__ static double ex(double a) { return exp_fp64(a); }
-  // erfc(x) given e2 = exp(-x^2)
-  __device__ static double erfc_e2(double x, double e2) {
-    return x >= 0.0 ? e2 * erfcx_poly(x) : 2.0 - e2 * erfcx_poly(-x);
-  }
-  __device__ static double erfc_(double x) { return erfc_e2(x, exp_fp64(-x * x)); }
-  __device__ static double erf_(double x) { return 1.0 - erfc_(x); }
-  // erfc(x) for x >= 0 given e2 = exp(-x^2)
-  __device__ static double erfc_pos(double x, double e2) { return e2 * erfcx_poly(x); }
-};
-template <>
-struct Tr<YNB_MIXED> {
-  __device__ static double ex(double a) {
-    const double t = a * 1.4426950408889634074;  // log2(e)
-    const double k = rint(t);
-    return ldexp((double)exp2f((float)(t - k)), (int)k);
-  }
-  __device__ static double erfc_(double x) { return ex(-x * x) * (double)erfcxf((float)x); }
-  __device__ static double erf_(double x) { return 1.0 - erfc_(x); }
-  __device__ static double erfc_e2(double x, double e2) { return e2 * (double)erfcxf((float)x); }
-  __device__ static double erfc_pos(double x, double e2) { return e2 * (double)erfcxf((float)x); }
-};
-
-template <int PREC>
-struct PotEval {
-  using R = Tr<PREC>;
-  // Returns the energy; g receives dE/dd / d (Yaff's vg).
-  __device__ static double lj(double sa, double sb, double ea, double eb, double d, double &g) {  // pair_pot.c:277-301
-    const double sigma = 0.5 * (sa + sb);
-    const double eps = sqrt(ea * eb);
-    double x = sigma / d;
-    x *= x;
-    x *= x * x;
-    g = 24.0 * eps / d / d * x * (1.0 - 2.0 * x);
-    return 4.0 * eps * (x * (x - 1.0));
-  }
-  __device__ static double mm3(double sa, double sb, double ea, double eb, int pauli, double d,
-                               double &g) {  // pair_pot.c:321-352
-    const double sigma = sa + sb;
-    const double eps = sqrt(ea * eb);
-    double x = sigma / d;
-    const double e = 1.84e5 * R::ex(-12.0 / x);
-    if (pauli != 0) {
-      g = eps / d * (-12.0 / sigma * e);
-      return eps * e;
-    }
-    x *= x;
-    x *= 2.25 * x * x;
-    g = eps / d * (-12.0 / sigma * e + 6.0 / d * x);
-    return eps * (e - x);
-  }
-  __device__ static double exprep(double amp, double b, double d, double &g) {  // pair_pot.c:473-489
-    if (amp == 0.0 || b == 0.0) {
-      g = 0.0;
-      return 0.0;
-    }
-    const double e = amp * R::ex(-b * d);
-    g = -e * b / d;
-    return e;
-  }
-  __device__ static double ei(double qa, double qb, double ra, double rb, double alpha, double diel, double d,
-                              double &g) {  // pair_pot.c:711-751
-    const double qq = (qa * qb) / diel;
-    const double rab = sqrt(ra * ra + rb * rb);
-    double e;
-    if (alpha > 0.0) {
-      const double x = alpha * d;
-      if (rab > 0.0) {
-        const double y = d / rab;
-        e = qq / d * (R::erfc_(x) - R::erfc_(y));
-        g = (TWO_DIV_SQRT_PI * (R::ex(-y * y) / rab - R::ex(-x * x) * alpha) * qq - e) / (d * d);
-      } else {
-        e = qq * R::erfc_(x) / d;
-        g = (-TWO_DIV_SQRT_PI * alpha * R::ex(-x * x) * qq - e) / (d * d);
-      }
-    } else {
-      if (rab > 0.0) {
-        const double y = d / rab;
-        e = qq / d * R::erf_(y);
-        g = (TWO_DIV_SQRT_PI / rab * R::ex(-y * y) * qq - e) / (d * d);
-      } else {
-        e = qq / d;
-        g = -e / (d * d);
-      }
-    }
-    return e;
-  }
-  __device__ static double switch3(double d, double rcut, double w, double &hg) {  // truncation.c:59-76
-    if (d < rcut) {
-      double x = rcut - d;
-      if (x > w) {
-        hg = 0.0;
-        return 1.0;
-      }
-      x /= w;
-      hg = -6 * x * (1 - x) / w;
-      return (3 - 2 * x) * x * x;
-    }
-    hg = 0.0;
-    return 0.0;
-  }
-  __device__ static double hammer(double d, double rcut, double tau, double &hg) {  // truncation.c:30-41
-    if (d < rcut) {
-      const double x = d - rcut;
-      const double h = R::ex(tau / x);
-      hg = -h * tau / x / x;
-      return h;
-    }
-    hg = 0.0;
-    return 0.0;
-  }
-};
-
-// The same potentials for the neighbour-list kernel, written around one
-// reciprocal distance inv = 1/d per pair (the reference divides by d in every
-// term).  LJ takes sqrt(epsilon) per atom (precomputed on the host), so
-// eps_ab = se_a * se_b instead of sqrt(eps_a * eps_b); EI takes 1/dielectric
-// and shares exp(-alpha^2 d^2) between erfc and the gradient.  These change
-// roundings only (relative 1e-16 per term).
-template <int PREC>
-struct PotFast {
-  using R = Tr<PREC>;
-  __device__ static double lj(double sa, double sb, double eps, double inv, double inv2, double &g) {
-    const double sigma = 0.5 * (sa + sb);
-    double x = sigma * inv;
-    x *= x;
-    x *= x * x;
-    g = 24.0 * eps * inv2 * x * (1.0 - 2.0 * x);
-    return 4.0 * eps * (x * (x - 1.0));
-  }
-  __device__ static double exprep(double amp, double b, double d, double inv, double &g) {
-    if (amp == 0.0 || b == 0.0) {
-      g = 0.0;
-      return 0.0;
-    }
-    const double e = amp * R::ex(-b * d);
-    g = -e * b * inv;
-    return e;
-  }
-  // alpha > 0, point charges (the common case); qq = qa qb / dielectric.
-  // x = alpha d > 0, so erfc needs only its non-negative branch.
-  __device__ static double ei_point(double qq, double alpha, double d, double inv, double inv2, double &g) {
-    const double x = alpha * d;
-    const double e2 = R::ex(-x * x);
-    const double e = qq * R::erfc_pos(x, e2) * inv;
-    g = (-TWO_DIV_SQRT_PI * alpha * e2 * qq - e) * inv2;
-    return e;
-  }
-};
 
 // pair_pot.c:65-81 as a binary search over the system's sorted rows.
 __device__ inline double stab_scale(const ynb_scaling_row *st, int n, int64_t a, int64_t b) {
@@ -485,8 +304,6 @@ template <int PREC, int FORM, int G>
 __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
                                                    ynb_pairpar pp, const double *pos, ynb_nlist nl, uint32_t mask,
                                                    double *gpos, int accumulate, double *partial, double *out) {
-  using PE = PotEval<PREC>;
-  using PF = PotFast<PREC>;
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t gi = t / G;
   const int s = (int)(t % G);
@@ -499,28 +316,24 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
   const int li = (int)(gi - S.atom0);
   const bool active = li < S.natom;
   const int npot = S.npot;
-  // per-system constants (uniform)
-  const double rc0 = S.pot[0].rcut;
-  const double rc1 = FORM == YNB_FORM_IONIC ? S.pot[1].rcut : 0.0;
-  const double rc2 = FORM == YNB_FORM_IONIC ? S.pot[2].rcut : 0.0;
-  const double rcmax = fmax(rc0, fmax(rc1, rc2));
-  const double tol0 = 1e-12 * rc0, tol1 = 1e-12 * rc1, tol2 = 1e-12 * rc2;
-  const double alpha = FORM == YNB_FORM_IONIC ? S.pot[2].alpha : 0.0;
-  const int trunc0 = S.pot[0].trunc;
-  const double w0 = S.pot[0].trunc_par;
-  const int nf = FORM == YNB_FORM_IONIC ? S.pot[0].nffatype : 0;
-  const double *amp = pp.cross + (FORM == YNB_FORM_IONIC ? S.pot[0].cross0 : 0);
-  const double *bex = amp + (int64_t)nf * nf;
+  FormConst C = form_const<FORM>(S, pp.cross);
   // ExpRep cross tables of up to 16 atom types go to LDS (a block's atoms
   // belong to one system, so the condition is the same for the block)
   __shared__ double l_amp[256], l_bex[256];
-  const bool lds_cross = FORM == YNB_FORM_IONIC && nf <= 16;
-  if (lds_cross) {
-    for (int q = threadIdx.x; q < nf * nf; q += blockDim.x) {
-      l_amp[q] = amp[q];
-      l_bex[q] = bex[q];
+  __shared__ float l_ampf[256], l_bexf[256];
+  if (FORM == YNB_FORM_IONIC && C.nf <= 16) {
+    for (int q = threadIdx.x; q < C.nf * C.nf; q += blockDim.x) {
+      l_amp[q] = C.amp[q];
+      l_bex[q] = C.bex[q];
+      l_ampf[q] = (float)C.amp[q];
+      l_bexf[q] = (float)C.bex[q];
     }
     __syncthreads();
+    C.lds_cross = true;
+    C.l_amp = l_amp;
+    C.l_bex = l_bex;
+    C.l_ampf = l_ampf;
+    C.l_bexf = l_bexf;
   }
 
   double acc_e = 0.0, acc_v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, acc_g[3] = {0.0, 0.0, 0.0};
@@ -571,65 +384,8 @@ __global__ __launch_bounds__(256) void k_pair_form(const ynb_sys *sys, const int
         u[1] = fma(na, S.rvecs[3 * a + 1], u[1]);
         u[2] = fma(na, S.rvecs[3 * a + 2], u[2]);
       }
-      // 1/d from the device rsqrt (a few ulp) and d = r2 / d: every decision
-      // that depends on the last bits of d is made on the exact path below
-      const double r2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-      double inv = rsqrt(r2);
-      double dd = r2 * inv;
-      // the form's cutoffs are uniform registers (a loop over S.pot here
-      // would be a scalar load and wait per pair)
-      bool near_cut = fabs(dd - rc0) <= tol0;
-      if (FORM == YNB_FORM_IONIC) near_cut = near_cut || fabs(dd - rc1) <= tol1 || fabs(dd - rc2) <= tol2;
-      if (near_cut) {
-        // Yaff's own row vector decides the cutoff tests (no scaling rows in a form)
-        const int lj = (int)(gj - S.atom0);
-        const bool self = (gj == gi);
-        const bool i_is_a = li > lj || (li == lj);
-        const int nab[3] = {i_is_a ? nn[0] : -nn[0], i_is_a ? nn[1] : -nn[1], i_is_a ? nn[2] : -nn[2]};
-        double v[3];
-        int r[3];
-        yaff_row_vector(i_is_a ? pi : pj, i_is_a ? pj : pi, self, nab, S.rvecs, S.gvecs, S.nvec, S.rmax, v, r);
-        u[0] = i_is_a ? v[0] : -v[0];
-        u[1] = i_is_a ? v[1] : -v[1];
-        u[2] = i_is_a ? v[2] : -v[2];
-        dd = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        inv = 1.0 / dd;
-      }
-      if (!(dd < rcmax)) continue;
-      const double d = dd;
-      const double inv2 = inv * inv;
-      double e_sum = 0.0, vg_sum = 0.0;
-      if (FORM == YNB_FORM_LJ) {
-        if (d < rc0) {
-          double g;
-          double e = PF::lj(ri.x, rj.x, ri.y * rj.y, inv, inv2, g);
-          if (trunc0 == YNB_TR_SWITCH3 && (e != 0.0 || g != 0.0)) {
-            double hg;
-            const double h = PE::switch3(d, rc0, w0, hg);
-            g = g * h + e * hg * inv;
-            e *= h;
-          }
-          e_sum = e;
-          vg_sum = g;
-        }
-      } else {
-        if (d < rc0) {  // ExpRep
-          const int ci = ti * nf + (int)rj.w;
-          double g;
-          e_sum += PF::exprep(lds_cross ? l_amp[ci] : amp[ci], lds_cross ? l_bex[ci] : bex[ci], d, inv, g);
-          vg_sum += g;
-        }
-        if (d < rc1) {  // LJ
-          double g;
-          e_sum += PF::lj(ri.x, rj.x, ri.y * rj.y, inv, inv2, g);
-          vg_sum += g;
-        }
-        if (d < rc2) {  // EI
-          double g;
-          e_sum += PF::ei_point(ri.z * rj.z, alpha, d, inv, inv2, g);
-          vg_sum += g;
-        }
-      }
+      double e_sum, vg_sum;
+      if (!form_pair<PREC, FORM>(S, C, li, gi, gj, pi, pos, nn, ri, rj, ti, u, e_sum, vg_sum)) continue;
       acc_e += e_sum;
       acc_g[0] -= u[0] * vg_sum;
       acc_g[1] -= u[1] * vg_sum;
