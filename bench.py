@@ -403,23 +403,32 @@ def served(nreq, rates):
     """Poisson request streams (single evaluations and 20-step MD segments,
     half and half; systems log-uniform in 100-50000 atoms, LJ fluid /
     rocksalt alternating, triclinic angles 80-100 degrees) served by the
-    continuous-batching server, FP64 and mixed, at each arrival rate:
-    p50/p99 latency (arrival to results on the host) and throughput."""
+    continuous-batching server, FP64 and mixed, at each arrival rate, and at
+    saturation (every request waiting at the start): p50/p99 latency (arrival
+    to results on the host), throughput, and the host's share of the span
+    spent placing requests (admission, retirement and table uploads) and
+    waiting for quanta."""
     from yaff_amd import serve
     out = {'workload': '%d requests per rate, log-uniform 100-50000 atoms, 50%% eval / 50%% 20-step MD' % nreq}
     warm = serve.poisson_requests(6, 1000.0, natom_range=(100, 3000), nsteps=2, seed=99)
     serve.Server(precision='mixed').run(warm, realtime=False)
     for prec in ('mixed', 'fp64'):
         res = []
-        for rate in rates:
-            reqs = serve.poisson_requests(nreq, rate, seed=11)
-            srv = serve.Server(precision=prec)
-            done = srv.run(reqs)
+        srv = serve.Server(precision=prec)   # one persistent engine for every rate
+        for rate in rates + [None]:
+            reqs = serve.poisson_requests(nreq, 1000.0 if rate is None else rate, seed=11)
+            srv.stats.clear()
+            done = srv.run(reqs, realtime=rate is not None)
             summ = serve.latency_summary(done)
-            summ.update({'rate_per_s': rate, 'batch_forms': srv.stats['forms'], 'iterations': srv.stats['iterations'],
-                         'form_s': srv.stats['form_s'], 'peak_est_bytes': srv.peak_bytes})
+            summ.update({'rate_per_s': 'saturation' if rate is None else rate, 'commits': srv.stats['forms'],
+                         'iterations': srv.stats['iterations'], 'form_s': srv.stats['form_s'],
+                         'form_frac': srv.stats['form_s']/summ['span_s'], 'wait_s': srv.stats['wait_s'],
+                         'regrow': srv.stats['regrow'], 'peak_est_bytes': srv.peak_bytes,
+                         'engine_bytes': srv.engine.device_bytes()})
             res.append(summ)
-            progress('served %s rate %g: p50 %.1f ms p99 %.1f ms' % (prec, rate, summ['p50_ms'], summ['p99_ms']))
+            progress('served %s rate %s: p50 %.1f ms p99 %.1f ms, %.3g atom-evals/s, placing %.1f%% of the span'
+                     % (prec, summ['rate_per_s'], summ['p50_ms'], summ['p99_ms'], summ['throughput'],
+                        100*summ['form_frac']))
         out[prec] = res
     return out
 

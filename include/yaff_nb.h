@@ -11,7 +11,11 @@
  * system s live at [sys.atom0, sys.atom0 + sys.natom) of the padded atom
  * arrays; atom0 is a multiple of 256 so that neither a wavefront nor the
  * atom blocks of the Ewald force pass (up to 256 atoms) hold atoms of two
- * systems.  A single system is a batch of one.
+ * systems.  A single system is a batch of one.  A batch may hold free
+ * ranges between its systems (wavefronts, bins, k-vector blocks and
+ * structure-factor work items whose owner entry is -1, skipped by every
+ * kernel) and free systems (natom 0, no potentials, nk 0): a persistent
+ * batch admits and retires systems by rewriting these tables in place.
  *
  * Reference interfaces replaced (molmod/yaff):
  *   ynb_nlist_build + ynb_nlist_rows   <- nlist_build_low   yaff/pes/nlist.h:37-39
@@ -124,7 +128,7 @@ typedef struct ynb_sys {
 typedef struct ynb_batch {
   const ynb_sys *sys;        /* device [nsys] */
   const int32_t *wave_sys;   /* device [npad/64]: system of each wavefront, -1 = none */
-  const int32_t *bin_sys;    /* device [nbin]: system of each bin */
+  const int32_t *bin_sys;    /* device [nbin]: system of each bin, -1 = none (a free range) */
   int32_t nsys;
   int32_t nbin;              /* total bins (0 when every system uses the all-pairs path) */
   int64_t npad;              /* padded atom count, multiple of 256 */
@@ -191,8 +195,8 @@ typedef struct ynb_ewald {
   const int32_t *skblk;   /* [nskblk][2]: global row range [begin, end) of each structure-factor block */
   const int32_t *fblk;    /* [nfblk][2]: global row range of each force block (system s: ynb_sys.frb0..) */
   const int32_t *chunk;   /* [nchunk_tot][4]: system, first local atom, atom count, group 0 mid/1 low/2 high */
-  const int32_t *work;    /* [nwork][4]: system, structure-factor block, global chunk, 0 */
-  const int32_t *kblk_sys;/* [nk_tot/256]: system of each block of 256 k vectors */
+  const int32_t *work;    /* [nwork][4]: system (-1: skip), structure-factor block, global chunk, 0 */
+  const int32_t *kblk_sys;/* [nk_tot/256]: system of each block of 256 k vectors, -1 = none */
   const int32_t *kpos;    /* [nk_tot]: force record of each k (-1: padding k) */
   const int32_t *krow_f;  /* [nrow_tot]: first force record of each row */
   int32_t *fctr;          /* [npad/64] block counters of the force pass (zero-initialised; left at zero) */
@@ -233,6 +237,39 @@ int ynb_profile_read(int i, const char **name, float *ms);
  * sys->rc and sys->natom.  Systems with nvec < 3 use the all-pairs path
  * (nbin = 0). */
 int ynb_sys_plan(ynb_sys *sys);
+
+/* Host only.  The k vectors of compute_ewald_reci (yaff/pes/ewald.c:38-58)
+ * for reciprocal rows gvecs [3][3] (no 2 pi), cell volume, alpha, gcut and
+ * the image range gmax[3], with the same FP64 expressions for k and |k|^2
+ * (hence the same set), in row order (g2, then g1, then g0 ascending):
+ * kvec [nk][4] = kx, ky, kz, c_k = 4 pi/V exp(-k^2/4 alpha^2)/k^2; g [nk][3];
+ * rows, the runs of consecutive g0 at fixed (g1, g2): krow [nrow][4] = first
+ * g0, g1, g2, count and krow_k [nrow] = the row's first k.  Every output
+ * holds cap entries (nrow <= nk); YNB_ECAPACITY when more k vectors pass. */
+int ynb_ewald_kvectors(const double *gvecs, double volume, double alpha, double gcut, const int64_t *gmax,
+                       int64_t cap, double *kvec, int32_t *g, int32_t *krow, int32_t *krow_k, int64_t *nk,
+                       int64_t *nrow);
+
+/* Host only.  Consecutive rows (krow as above) grouped greedily into blocks
+ * of at most kmax k vectors (a longer row is a block of its own): blk
+ * [nblk][2] = first row, end row; blk holds nrow entries. */
+int ynb_row_blocks(const int32_t *krow, int64_t nrow, int32_t kmax, int32_t *blk, int64_t *nblk);
+
+/* Host only.  Force records of one system's rows (krow as above): each
+ * row padded to a multiple of 4 records; krow_f [nrow] = the row's first
+ * record, kpos [nkp] = the record of each k (-1 for k >= nk), *nrec the
+ * total (ynb_ewald.krow_f, kpos and frec before the batch offsets). */
+int ynb_ewald_records(const int32_t *krow, int64_t nrow, int64_t nkp, int32_t *krow_f, int32_t *kpos,
+                      int64_t *nrec);
+
+/* Host only.  The structure-factor work items of one system, every (row
+ * block, atom chunk), block major: work [nblk * nchunk][4] = sys, blk0 +
+ * block, chunk0 + chunk, 0 (ynb_ewald.work). */
+int ynb_ewald_work(int64_t nblk, int64_t nchunk, int32_t sys, int32_t blk0, int32_t chunk0, int32_t *work);
+/* The same items written by a kernel into device memory (work: the
+ * [nblk * nchunk][4] slice of ynb_ewald.work, 16-byte aligned), on stream. */
+int ynb_ewald_work_device(int32_t *work, int64_t nblk, int64_t nchunk, int32_t sys, int32_t blk0, int32_t chunk0,
+                          ynb_stream stream);
 
 /* Workspace of ynb_nlist_build in bytes, for bins of bin_cap atoms
  * (ynb_batch.max_bin_atoms).  The caller zero-initialises it once; the build
@@ -401,6 +438,23 @@ int ynb_verlet_kick_kick_drift(const ynb_batch *b, double *pos, double *vel, dou
                                const double *pair_out, const double *ewald_out, double *total,
                                const ynb_ewald *ew, const double *charges, void *nl_work,
                                size_t nl_work_bytes, const ynb_nlist *nl, ynb_stream stream);
+
+/* Table updates of a persistent batch in one launch.  No counterpart in the
+ * reference (Yaff evaluates one system at a time; this serves batches whose
+ * members change in place).  Segment i (device memory, one block each)
+ * copies nbytes from src + seg.src to seg.dst or, with seg.src < 0, fills
+ * them with seg.fill; elements of seg.elem = 4 or 8 bytes (8: the low 4
+ * bytes of fill for 4); dst, src offsets and nbytes are multiples of elem.
+ * Segments must not overlap one another. */
+typedef struct ynb_segment {
+  uint64_t dst;     /* device address */
+  int64_t src;      /* byte offset into src, or -1: fill */
+  int64_t nbytes;
+  uint64_t fill;    /* fill pattern (bits of one element) */
+  int32_t elem;     /* 4 or 8 */
+  int32_t pad_;
+} ynb_segment;
+int ynb_scatter(const ynb_segment *segs, int32_t nseg, const void *src, ynb_stream stream);
 
 /* out[s][k] = sum over slots of in[s][slot][k] for k < 10, plus optional
  * extra [nsys][10] terms; used to total a force-field evaluation on device. */

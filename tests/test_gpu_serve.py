@@ -66,7 +66,6 @@ def test_md_segment_matches_eager_engine():
 
 
 def test_admission_budget_and_mixed_accuracy():
-    import torch
     from yaff_amd import serve
     print('case serve budget + mixed accuracy', flush=True)
     rs64 = _requests(seed=9, n=8)
@@ -84,14 +83,12 @@ def test_admission_budget_and_mixed_accuracy():
         print(a.rid, a.natom, 'mixed rms force err %.2e energy err %.2e' % (rms, abs(a.energy - b.energy)/abs(a.energy)))
         assert rms < 1e-5
         assert abs(a.energy - b.energy) <= 1e-6*abs(a.energy)
-    # the allocator's view of one engine agrees with the estimate within a factor 2
+    # the server's persistent engine holds, after serving one request, memory
+    # within a factor 2 of the estimate for that request
     r = _requests(seed=9, n=8)[1]
-    torch.cuda.synchronize()
-    before = torch.cuda.memory_allocated()
-    srv = serve.Server(precision='fp64')
-    srv.active = [r]
-    srv._form()
-    used = torch.cuda.memory_allocated() - before
+    srv = serve.Server(precision='fp64', reserve_atoms=256)
+    srv.run([r], realtime=False)
+    used = srv.engine.device_bytes()
     est = serve.estimate_bytes([r.nb])
-    print('estimate %d allocated %d' % (est, used))
+    print('estimate %d engine tables %d' % (est, used))
     assert 0.5*used <= est <= 2.0*used

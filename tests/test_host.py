@@ -3,6 +3,8 @@ tables, scaling tables, the drop-in classes' bookkeeping, workload
 generators and the serving loop's admission."""
 import itertools
 import math
+import os
+import sys
 
 import numpy as np
 import pytest
@@ -52,7 +54,7 @@ def test_ewald_local_plan_covers_every_k_and_atom():
     s = workloads.rocksalt((3, 2, 2), angles=(88.0, 93.0, 95.0))
     nb = nbsystem(s)
     nb.ewald.nlow, nb.ewald.nhigh = 5, s.natom - 9
-    _, g, _ = ewald_kvectors_cached(nb)
+    g = ewald_kvectors_cached(nb)[1]
     for apl, skb, fkb in ((2, 16, 32), (4, 64, 128), (8, 512, 16)):
         p = _ewald_local_plan(nb, apl, skb, fkb)
         nk = p['nk']
@@ -92,7 +94,10 @@ def test_ewald_local_plan_covers_every_k_and_atom():
             assert (p['kpos'][k:k + ln] == f + np.arange(ln)).all()
         assert (p['kpos'][nk:] == -1).all() and len(set(p['kpos'][:nk])) == nk
         # work items: every (structure-factor block, chunk) pair once
-        pairs = [(int(w[1]), int(w[2])) for w in p['work']]
+        from yaff_amd.engine import work_table
+        wk = work_table(p, 3, 10, 20)
+        assert (wk[:, 0] == 3).all() and (wk[:, 3] == 0).all()
+        pairs = [(int(w[1]) - 10, int(w[2]) - 20) for w in wk]
         assert sorted(pairs) == sorted(itertools.product(range(len(p['skblk'])), range(len(p['chunks']))))
 
 
@@ -343,3 +348,125 @@ def test_failure_detection():
     with pytest.raises(ValueError):
         check_nlow_nhigh(s, 50, 10)
     assert check_nlow_nhigh(s, 5, -1) == (5, 100)
+
+
+def test_ranges_first_fit():
+    """The slot engine's range allocator: first fit, aligned, freed ranges
+    reused, the high-water mark and the capacity a growth needs."""
+    from yaff_amd.slots import Ranges
+    r = Ranges(1024, 256)
+    a = r.alloc(300)            # 512 units
+    b = r.alloc(1)              # 256
+    assert (a, b) == (0, 512) and r.high == 768
+    assert r.alloc(300) is None            # 512 more do not fit in 1024
+    assert r.need(300) == 1280
+    r.free(a)
+    assert r.high == 768
+    c = r.alloc(256)            # first fit: the freed range's start
+    d = r.alloc(200)
+    assert (c, d) == (0, 256)
+    assert r.alloc(256) == 768 and r.high == 1024
+    with pytest.raises(KeyError):
+        r.free(5)
+    r2 = Ranges(10)
+    assert [r2.alloc(3), r2.alloc(3), r2.alloc(3), r2.alloc(3)] == [0, 3, 6, None]
+    r2.free(3)
+    assert r2.alloc(4) is None and r2.alloc(2) == 3 and r2.alloc(1) == 5
+
+
+def test_system_entry_offsets_and_forms():
+    """Per-system planning shared by the batch and slot engines: the pair
+    form and its record, LJ sqrt(epsilon), scaling and cross offsets local to
+    the system until shifted, and the Ewald self term of a correction
+    without scaled pairs."""
+    import yaff_amd as ya
+    from yaff_amd import _lib, workloads
+    from yaff_amd.batch import nbsystem
+    from yaff_amd.engine import system_entry, shift_offsets, corr_self_energy
+    lib = _lib.load()
+    lj = nbsystem(workloads.lj_fluid((3, 3, 3), seed=1))
+    e = system_entry(lib, lj, 512)
+    assert e.h.atom0 == 512 and e.h.natom == lj.natom and e.h.pair_form == _lib.FORM_LJ and e.h.nbin > 0
+    np.testing.assert_array_equal(e.rec[:, 1], np.sqrt(lj.pots[0].p1))
+    np.testing.assert_array_equal(e.par[0, 2], np.sqrt(lj.pots[0].p1))
+    salt = nbsystem(workloads.rocksalt((2, 2, 2), seed=1))
+    e = system_entry(lib, salt, 0)
+    assert e.h.pair_form == _lib.FORM_IONIC and e.h.pot[0].cross0 == 0 and len(e.cross) == 1
+    q = salt.charges
+    assert e.h.ew_const == corr_self_energy(salt.ewald.alpha, q, 0, salt.natom)/salt.ewald.dielectric
+    np.testing.assert_array_equal(e.rec[:, 2], q/np.sqrt(salt.pots[2].dielectric))
+    shift_offsets(e.h, 7, 12)
+    assert e.h.pot[0].cross0 == 12 and e.h.pot[0].stab0 == 7 and e.h.pot[1].stab0 == 7
+    # a system with scaling rows takes the generic form
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden'))
+    import cases
+    f = cases.fixture_arrays('water32')
+    w = ya.System(f['numbers'], f['pos'], ffatypes=['O', 'H', 'H']*32, bonds=f['bonds'], rvecs=f['rvecs'],
+                  charges=f['charges'])
+    sc = ya.Scalings(w, 0.0, 0.0, 0.5)
+    from yaff_amd.engine import NBSystem, PotSpec
+    nb = NBSystem(w.pos, w.cell, [PotSpec(_lib.POT_EI, 9.0, p0=w.charges, alpha=0.2),
+                                  PotSpec(_lib.POT_LJ, 9.0, p0=np.full(96, 5.0), p1=np.full(96, 1e-4))],
+                  stab=sc.stab)
+    e = system_entry(lib, nb, 0)
+    assert e.h.pair_form == _lib.FORM_GENERIC and e.h.any_stab == 1
+    assert (e.h.pot[0].stab0, e.h.pot[0].nstab, e.h.pot[1].stab0) == (0, len(sc.stab), len(sc.stab))
+    assert len(e.stabs) == 2
+
+
+def test_stager_segments(monkeypatch):
+    """The slot engine's staged table updates: copies and fills resolved by
+    attribute into one buffer (segment table, then data), split into
+    segments of at most 64 KiB, applied here by a CPU stand-in of
+    ynb_scatter (copy or fill each segment); overlapping updates refused."""
+    import ctypes
+    import torch
+    from yaff_amd import _lib, slots
+    monkeypatch.setattr(slots, '_pinned', lambda n: torch.empty(n, dtype=torch.uint8))
+
+    class Lib(object):
+        def ynb_scatter(self, segs, nseg, src, stream):
+            tab = np.frombuffer(ctypes.string_at(segs, nseg*_lib.SEGMENT_DTYPE.itemsize), _lib.SEGMENT_DTYPE)
+            assert (tab['nbytes'] <= slots.SEG_BYTES).all() and (tab['nbytes'] % tab['elem'] == 0).all()
+            for s in tab:
+                n, el = int(s['nbytes']), int(s['elem'])
+                if s['src'] < 0:
+                    ctypes.memmove(int(s['dst']), int(s['fill']).to_bytes(8, 'little')[:el]*(n//el), n)
+                else:
+                    ctypes.memmove(int(s['dst']), src + int(s['src']), n)
+            return 0
+
+    class Eng(object):
+        pass
+    e = Eng()
+    e.lib, e.device, e.stream = Lib(), 'cpu', (lambda: None)
+    e._stage_d = torch.zeros(16, dtype=torch.uint8)
+    e.a = torch.zeros((100, 3), dtype=torch.float64)
+    e.b = torch.full((50,), 7, dtype=torch.int32)
+    e.c = torch.zeros(80, dtype=torch.uint8)
+    e.d = torch.zeros(50000, dtype=torch.float64)
+    st = slots._Stager()
+    x = np.random.default_rng(0).normal(size=(10, 3))
+    big = np.random.default_rng(1).normal(size=40000)
+    st.put('a', 30, x)
+    st.fill('a', 60, 9, 1e150)
+    st.fill('b', 5, 10, -1)
+    st.put('b', 20, np.arange(7, dtype=np.int32))
+    st.put('c', 8, np.arange(16, dtype=np.uint8))
+    st.put('d', 3, big)
+    st.flush(e)
+    a, b, c, d = e.a.numpy(), e.b.numpy(), e.c.numpy(), e.d.numpy()
+    assert np.array_equal(a[10:20], x) and (a[20:23] == 1e150).all() and not a[:10].any() and not a[23:].any()
+    assert (b[5:15] == -1).all() and (b[:5] == 7).all() and (b[15:20] == 7).all() and (b[27:] == 7).all()
+    assert (b[20:27] == np.arange(7)).all()
+    assert (c[8:24] == np.arange(16)).all() and not c[:8].any() and not c[24:].any()
+    assert np.array_equal(d[3:40003], big) and d[0] == 0 and d[40003] == 0
+    assert st.ops == []
+    st.put('b', 0, np.arange(3, dtype=np.int32))
+    st.fill('b', 2, 4, 5)
+    with pytest.raises(ValueError):
+        st.flush(e)
+    s2 = slots._Stager()   # bytes into a byte table go as 8-byte elements
+    s2.put('c', 0, np.arange(3, dtype=np.uint8))
+    with pytest.raises(ValueError):
+        s2.flush(e)

@@ -27,6 +27,9 @@ ROW_DTYPE = np.dtype([('a', np.int64), ('b', np.int64), ('d', np.float64), ('dx'
                       ('dy', np.float64), ('dz', np.float64), ('r0', np.int64), ('r1', np.int64),
                       ('r2', np.int64)])
 SCALING_DTYPE = np.dtype([('a', np.int64), ('b', np.int64), ('scale', np.float64), ('nbond', np.int64)])
+# ynb_segment (include/yaff_nb.h)
+SEGMENT_DTYPE = np.dtype([('dst', np.uint64), ('src', np.int64), ('nbytes', np.int64), ('fill', np.uint64),
+                          ('elem', np.int32), ('pad', np.int32)])
 
 
 class YnbPot(C.Structure):
@@ -92,7 +95,8 @@ EXPORTS = ['ynb_version', 'ynb_last_error', 'ynb_sys_plan', 'ynb_nlist_workspace
            'ynb_pair_compute', 'ynb_pair_rows', 'ynb_pair_rows_static', 'ynb_ewald_reci', 'ynb_ewald_corr', 'ynb_verlet_kick_drift', 'ynb_verlet_kick',
            'ynb_verlet_kick_kick_drift',
            'ynb_total', 'ynb_profile_enable', 'ynb_profile_count', 'ynb_profile_read',
-           'ynb_nlist_skin_check']
+           'ynb_nlist_skin_check', 'ynb_ewald_kvectors', 'ynb_row_blocks', 'ynb_scatter',
+           'ynb_ewald_records', 'ynb_ewald_work', 'ynb_ewald_work_device']
 
 
 class YnbError(RuntimeError):
@@ -114,6 +118,13 @@ def load(path=LIB_PATH):
     L.ynb_version.restype = C.c_char_p
     L.ynb_last_error.restype = C.c_char_p
     L.ynb_sys_plan.argtypes = [P(YnbSys)]
+    L.ynb_ewald_kvectors.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp, i64, vp, vp, vp, vp, P(i64),
+                                     P(i64)]
+    L.ynb_row_blocks.argtypes = [vp, i64, i32, vp, P(i64)]
+    L.ynb_ewald_records.argtypes = [vp, i64, i64, vp, vp, P(i64)]
+    L.ynb_ewald_work.argtypes = [i64, i64, i32, i32, i32, vp]
+    L.ynb_ewald_work_device.argtypes = [vp, i64, i64, i32, i32, i32, vp]
+    L.ynb_scatter.argtypes = [vp, i32, vp, vp]
     L.ynb_nlist_workspace_bytes.restype = sz
     L.ynb_nlist_workspace_bytes.argtypes = [i64, i32, i32]
     L.ynb_rows_workspace_bytes.restype = sz

@@ -12,7 +12,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
-SOURCES = ['nlist.hip', 'pair.hip', 'ewald.hip', 'md.hip']
+SOURCES = ['nlist.hip', 'pair.hip', 'ewald.hip', 'md.hip', 'tables.hip', 'ewald_plan.cpp']
 LIB = os.path.join(HERE, 'libyaff_nb.so')
 ARCH = os.environ.get('YNB_ARCH', 'gfx950')
 

@@ -151,6 +151,7 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
   const int m_r = lane & (T - 1), part = lane >> 3;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const int4 wk = *(const int4 *)(work + 4 * w);
+    if (wk.x < 0) continue;  // an item of no system (the same for the wavefront)
     const ynb_sys &S = sys[wk.x];
     const int4 ch = *(const int4 *)(chunk + 4 * wk.z);
     const int rb = skblk[2 * wk.y], re = skblk[2 * wk.y + 1];
@@ -274,7 +275,8 @@ __global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_e
   const int t = threadIdx.x;
   const int kl = t & 31, part = t >> 5;
   const int k = blockIdx.x * 32 + kl;
-  const int sid = ew.kblk_sys[k >> 8];
+  const int sid = ew.kblk_sys[k >> 8];  // the same for the block
+  if (sid < 0) return;                   // k vectors of no system
   const ynb_sys &S = sys[sid];
   const int lk = k - S.k0;
   const double4 kv = *(const double4 *)(ew.kvec + 4 * (int64_t)k);

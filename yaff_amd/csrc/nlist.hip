@@ -46,7 +46,8 @@ __global__ __launch_bounds__(256) void k_bin_order(int32_t cb, int32_t *bin_coun
                                                    const int32_t *rebuild) {
   __shared__ int32_t l_idx[1024];
   const int b = blockIdx.x;
-  if (rebuild && !rebuild[bin_sys[b]]) return;
+  const int sid = bin_sys[b];
+  if (sid < 0 || (rebuild && !rebuild[sid])) return;  // a bin of no system holds no atoms
   const int n = min(bin_count[b], cb);
   for (int e = threadIdx.x; e < n; e += blockDim.x) l_idx[e] = slot_idx[(int64_t)b * cb + e];
   __syncthreads();
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = uniform_int(tid >> 6);
   const int sid = bin_sys[b];
-  if (nl.rebuild && !nl.rebuild[sid]) return;
+  if (sid < 0 || (nl.rebuild && !nl.rebuild[sid])) return;
   const ynb_sys &S = sys[sid];
   const int start = bin_start[b], end = bin_end[b];
   if (start + (int)blockIdx.y * BW >= end) return;

@@ -90,43 +90,44 @@ class NBSystem(object):
         return len(self.pos)
 
 
-def ewald_kvectors(cell, alpha, gcut, gmax=None):
-    """The half-space k vectors compute_ewald_reci visits (ewald.c:41-58), with
-    the same FP64 expressions for k and |k|^2 so that the cutoff test picks the
-    same set.  Returned in row order (g2, g1 major, g0 ascending) with
-    c_k = 4 pi/V exp(-k^2/4 alpha^2)/k^2."""
+def ewald_kplan(cell, alpha, gcut, gmax=None):
+    """The half-space k vectors compute_ewald_reci visits (ewald.c:38-58),
+    from the library's host planner (ynb_ewald_kvectors: the same FP64
+    expressions for k and |k|^2, so the cutoff test picks the same set), in
+    row order (g2, g1 major, g0 ascending) with c_k = 4 pi/V
+    exp(-k^2/4 alpha^2)/k^2: (gmax, g [nk][3], kv [nk][4], krow [nrow][4] =
+    first g0, g1, g2, count, krow_k [nrow] = first k of each row)."""
     if gmax is None:
         gmax = np.ceil(gcut/cell.gspacings - 0.5).astype(np.int64)
-    kv = TWO_PI*cell.gvecs_full
-    # g2 outermost, g0 innermost: already the row order
-    g2, g1, g0 = np.meshgrid(np.arange(0, gmax[2] + 1), np.arange(-gmax[1], gmax[1] + 1),
-                             np.arange(-gmax[0], gmax[0] + 1), indexing='ij')
-    g = np.stack([g0.ravel(), g1.ravel(), g2.ravel()], axis=1)
-    half = (g[:, 2] > 0) | ((g[:, 2] == 0) & ((g[:, 1] > 0) | ((g[:, 1] == 0) & (g[:, 0] > 0))))
-    g = g[half]
-    gf = g.astype(np.float64)
-    kx = gf[:, 0]*kv[0, 0] + gf[:, 1]*kv[1, 0] + gf[:, 2]*kv[2, 0]
-    ky = gf[:, 0]*kv[0, 1] + gf[:, 1]*kv[1, 1] + gf[:, 2]*kv[2, 1]
-    kz = gf[:, 0]*kv[0, 2] + gf[:, 1]*kv[1, 2] + gf[:, 2]*kv[2, 2]
-    ksq = kx*kx + ky*ky + kz*kz
-    gc = gcut*TWO_PI
-    gc *= gc
-    keep = ~(ksq > gc)
-    g, kx, ky, kz, ksq = g[keep], kx[keep], ky[keep], kz[keep], ksq[keep]
-    fac1 = FOUR_PI/cell.volume
-    fac2 = 0.25/alpha/alpha
-    c = fac1*np.exp(-ksq*fac2)/ksq
-    return gmax, g, np.stack([kx, ky, kz, c], axis=1)
+    gmax = np.ascontiguousarray(gmax, dtype=np.int64)
+    cap = int((2*gmax[0] + 1)*(2*gmax[1] + 1)*(gmax[2] + 1))
+    kv = np.empty((cap, 4))
+    g = np.empty((cap, 3), np.int32)
+    krow = np.empty((cap, 4), np.int32)
+    krow_k = np.empty(cap, np.int32)
+    nk, nrow = _lib.C.c_int64(0), _lib.C.c_int64(0)
+    gv = np.ascontiguousarray(cell.gvecs_full, dtype=np.float64)
+    L = _lib.load()
+    check(L.ynb_ewald_kvectors(gv.ctypes.data, float(cell.volume), float(alpha), float(gcut), gmax.ctypes.data,
+                               cap, kv.ctypes.data, g.ctypes.data, krow.ctypes.data, krow_k.ctypes.data,
+                               _lib.C.byref(nk), _lib.C.byref(nrow)))
+    nk, nrow = nk.value, nrow.value
+    return gmax, g[:nk].astype(np.int64), kv[:nk].copy(), krow[:nrow].copy(), krow_k[:nrow].copy()
+
+
+def ewald_kvectors(cell, alpha, gcut, gmax=None):
+    """(gmax, g [nk][3], kv [nk][4]) of ewald_kplan."""
+    return ewald_kplan(cell, alpha, gcut, gmax)[:3]
 
 
 def ewald_kvectors_cached(s):
-    """ewald_kvectors of an NBSystem, cached on it."""
+    """ewald_kplan of an NBSystem, cached on it."""
     ew = s.ewald
     cache = s.__dict__.setdefault('_plan_cache', {})
     kkey = ('kv', s.cell.rvecs_full.tobytes(), ew.alpha, ew.gcut, None if ew.gmax is None else tuple(ew.gmax))
     if kkey not in cache:
         cache.clear()
-        cache[kkey] = ewald_kvectors(s.cell, ew.alpha, ew.gcut, ew.gmax)
+        cache[kkey] = ewald_kplan(s.cell, ew.alpha, ew.gcut, ew.gmax)
     return cache[kkey]
 
 
@@ -208,19 +209,25 @@ class CorrTables(object):
                               plan['nblk_tot'])
 
 
-def _row_blocks(cnt, kmax):
+def work_table(plan, sys, blk0, chunk0):
+    """The structure-factor work items of a system's plan (every row block
+    x every chunk, block major) with batch offsets: [nwork][4] int32
+    (ynb_ewald_work)."""
+    work = np.empty((max(plan['nwork'], 1), 4), np.int32)
+    check(_lib.load().ynb_ewald_work(len(plan['skblk']), len(plan['chunks']), int(sys), int(blk0), int(chunk0),
+                                     work.ctypes.data))
+    return work[:plan['nwork']]
+
+
+def _row_blocks(krow, kmax):
     """Consecutive rows grouped into blocks of at most kmax k vectors (a
-    longer row is a block of its own): [n][2] local row ranges."""
-    blocks = []
-    r0, acc = 0, 0
-    for r, c in enumerate(cnt):
-        if acc > 0 and acc + c > kmax:
-            blocks.append((r0, r))
-            r0, acc = r, 0
-        acc += int(c)
-    if len(cnt):
-        blocks.append((r0, len(cnt)))
-    return np.asarray(blocks, np.int32).reshape(-1, 2)
+    longer row is a block of its own): [n][2] local row ranges
+    (ynb_row_blocks)."""
+    krow = np.ascontiguousarray(krow, dtype=np.int32)
+    blk = np.empty((max(len(krow), 1), 2), np.int32)
+    n = _lib.C.c_int64(0)
+    check(_lib.load().ynb_row_blocks(krow.ctypes.data, len(krow), int(kmax), blk.ctypes.data, _lib.C.byref(n)))
+    return blk[:n.value].copy()
 
 
 def _ewald_local_plan(s, apl, skb, fkb):
@@ -231,7 +238,7 @@ def _ewald_local_plan(s, apl, skb, fkb):
     pass (<= fkb k each), atom chunks of 64 * apl atoms and the
     structure-factor work items (every row block x every chunk)."""
     ew = s.ewald
-    gmax, g, kv = ewald_kvectors_cached(s)
+    gmax, g, kv, krow, krow_k = ewald_kvectors_cached(s)
     cache = s._plan_cache
     kkey = next(k for k in cache if k[0] == 'kv')
     pkey = ('plan', kkey, apl, skb, fkb)
@@ -241,19 +248,16 @@ def _ewald_local_plan(s, apl, skb, fkb):
     nkp = max(_round_up(nk, KBLOCK), KBLOCK)
     kpad = np.zeros((nkp, 4))
     kpad[:nk] = kv
-    rows = _runs(g)
-    st, cnt = (rows[:, 0], rows[:, 1]) if len(rows) else (np.zeros(0, np.int64), np.zeros(0, np.int64))
-    krow = np.stack([g[st, 0], g[st, 1], g[st, 2], cnt], axis=1).astype(np.int32) if len(st) else \
-        np.zeros((0, 4), np.int32)
-    skblk = _row_blocks(cnt, skb)
-    fblk = _row_blocks(cnt, fkb)
+    skblk = _row_blocks(krow, skb)
+    fblk = _row_blocks(krow, fkb)
     # force records: rows padded to multiples of 4 records
-    rlen4 = (cnt + 3)//4*4
-    krow_f = (np.cumsum(rlen4) - rlen4).astype(np.int32)
-    kpos = np.full(nkp, -1, np.int32)
-    if nk:
-        rid = np.repeat(np.arange(len(st)), cnt)
-        kpos[:nk] = krow_f[rid] + (np.arange(nk) - st[rid])
+    krow_f = np.empty(max(len(krow), 1), np.int32)
+    kpos = np.empty(nkp, np.int32)
+    nrec = _lib.C.c_int64(0)
+    krow = np.ascontiguousarray(krow, dtype=np.int32)
+    check(_lib.load().ynb_ewald_records(krow.ctypes.data, len(krow), nkp, krow_f.ctypes.data, kpos.ctypes.data,
+                                        _lib.C.byref(nrec)))
+    krow_f = krow_f[:len(krow)]
     # atom chunks: below nlow (group 1), [nlow, nhigh) (group 0), above nhigh (group 2)
     nhigh = s.natom if ew.nhigh == -1 else ew.nhigh
     nlow = ew.nlow
@@ -263,26 +267,115 @@ def _ewald_local_plan(s, apl, skb, fkb):
         a = np.arange(lo, hi, csize)
         parts.append(np.stack([np.zeros_like(a), a, np.minimum(csize, hi - a), np.full_like(a, grp)], axis=1))
     chunks = np.concatenate(parts).astype(np.int32)
-    wb, wc = np.meshgrid(np.arange(len(skblk)), np.arange(len(chunks)), indexing='ij')
-    work = np.stack([np.zeros(wb.size, np.int64), wb.ravel(), wc.ravel(), np.zeros(wb.size, np.int64)],
-                    axis=1).astype(np.int32)
-    plan = dict(gmax=gmax, nk=nk, nkp=nkp, kpad=kpad, krow=krow, krow_k=st.astype(np.int32), nrow=len(st),
-                skblk=skblk, fblk=fblk, chunks=chunks, work=work, krow_f=krow_f, kpos=kpos, nrec=int(rlen4.sum()),
-                grouped=bool(nlow > 0 or nhigh < s.natom),
+    plan = dict(gmax=gmax, nk=nk, nkp=nkp, kpad=kpad, krow=krow, krow_k=krow_k.astype(np.int32), nrow=len(krow),
+                skblk=skblk, fblk=fblk, chunks=chunks, nwork=len(skblk)*len(chunks), krow_f=krow_f, kpos=kpos,
+                nrec=int(nrec.value), grouped=bool(nlow > 0 or nhigh < s.natom),
                 nchunk_low=int((chunks[:, 3] == 1).sum()), nchunk_high=int((chunks[:, 3] == 2).sum()))
     cache[pkey] = plan
     return plan
 
 
-def _runs(g):
-    """Rows: maximal runs of consecutive g0 at fixed (g1, g2)."""
-    if len(g) == 0:
-        return np.zeros((0, 2), np.int64)
-    brk = np.ones(len(g), bool)
-    brk[1:] = (g[1:, 1] != g[:-1, 1]) | (g[1:, 2] != g[:-1, 2]) | (g[1:, 0] != g[:-1, 0] + 1)
-    starts = np.nonzero(brk)[0]
-    ends = np.append(starts[1:], len(g))
-    return np.stack([starts, ends - starts], axis=1)
+class SystemEntry(object):
+    """One system's part of the batch tables: its ynb_sys (offsets into the
+    batch tables left at their system-local values, see shift_offsets), per-atom
+    pair parameters [MAX_POT][3][natom] and records [natom][4], its ExpRep
+    cross tables and scaling tables (one per potential, in potential order)."""
+
+    def __init__(self, h, par, rec, cross, stabs):
+        self.h, self.par, self.rec, self.cross, self.stabs = h, par, rec, cross, stabs
+
+
+def system_entry(lib, s, a0):
+    """SystemEntry of NBSystem ``s`` placed at padded atom offset ``a0``:
+    cell, cutoff, image range and groups; ynb_sys_plan's bin grid; each
+    potential's kind, truncation and parameters; the specialised pair form
+    and its per-atom record; the Ewald groups and, for a correction without
+    scaled pairs, its self term as ew_const."""
+    h = YnbSys()
+    cell = s.cell
+    for k in range(9):
+        h.rvecs[k] = float(cell.rvecs_full.ravel()[k])
+        h.gvecs[k] = float(cell.gvecs_full.ravel()[k])
+    h.rc = s.rc
+    h.volume = cell.volume
+    h.nvec = cell.nvec
+    h.natom = s.natom
+    h.atom0 = a0
+    h.npot = len(s.pots)
+    for k in range(3):
+        h.rmax[k] = int(s.rmax[k]) if k < cell.nvec else 0
+    h.nlow, h.nhigh = s.nlow, s.nhigh
+    h.ew_nlow, h.ew_nhigh = 0, s.natom
+    h.ew_const = 0.0
+    if s.ewald is not None:
+        h.ew_nlow = s.ewald.nlow
+        h.ew_nhigh = s.natom if s.ewald.nhigh == -1 else s.ewald.nhigh
+        if s.ewald.correction and len(s.stab) == 0:
+            # the correction is its self term alone: a constant of the reciprocal out
+            h.ew_const = corr_self_energy(s.ewald.alpha, s.charges, h.ew_nlow, h.ew_nhigh)/s.ewald.dielectric
+    check(lib.ynb_sys_plan(h))
+    par = np.zeros((MAX_POT, 3, s.natom), np.float64)
+    cross, stabs = [], []
+    stab_off = cross_off = 0
+    for p, pot in enumerate(s.pots):
+        hp = h.pot[p]
+        st = s.stab if pot.stab is None else pot.stab
+        hp.stab0, hp.nstab = stab_off, len(st)
+        stab_off += len(st)
+        if len(st):
+            stabs.append(np.asarray(st, dtype=SCALING_DTYPE))
+            h.any_stab = 1
+        hp.kind, hp.trunc, hp.rcut, hp.trunc_par = pot.kind, pot.trunc, pot.rcut, pot.trunc_par
+        hp.alpha, hp.dielectric = pot.alpha, pot.dielectric
+        for q, arr in enumerate((pot.p0, pot.p1, pot.p2)):
+            if arr is not None:
+                par[p, q] = np.asarray(arr, dtype=np.float64)
+        if pot.kind == POT_LJ and pot.p1 is not None:
+            # sqrt(epsilon) per atom: the pair kernel forms eps_ab = se_a se_b
+            par[p, 2] = np.sqrt(np.asarray(pot.p1, dtype=np.float64))
+        if pot.amp_cross is not None:
+            hp.nffatype = pot.amp_cross.shape[0]
+            hp.cross0 = cross_off
+            c = np.concatenate([np.ascontiguousarray(pot.amp_cross, dtype=float).ravel(),
+                                np.ascontiguousarray(pot.b_cross, dtype=float).ravel()])
+            cross.append(c)
+            cross_off += len(c)
+    h.pair_form, rec = _pair_form(s)
+    return SystemEntry(h, par, rec, cross, stabs)
+
+
+def shift_offsets(h, stab_off, cross_off):
+    """Moves a SystemEntry's scaling-table and cross-table offsets to where
+    its tables start in the batch tables."""
+    for p in range(h.npot):
+        h.pot[p].stab0 += stab_off
+        if h.pot[p].nffatype:
+            h.pot[p].cross0 += cross_off
+
+
+def _pair_form(s):
+    """pair_form of a system (a specialised pair kernel for the potential
+    sets of the LJ fluid and of the rocksalt-type ionic force field) and the
+    per-atom record [natom][4] that kernel reads."""
+    n = s.natom
+    rec = np.zeros((n, 4))
+    pots = s.pots
+    if any(len(s.stab if p.stab is None else p.stab) for p in pots) or n == 0:
+        return _lib.FORM_GENERIC, rec
+    kinds = [p.kind for p in pots]
+    if kinds == [_lib.POT_LJ] and pots[0].trunc in (_lib.TR_NONE, _lib.TR_SWITCH3) and pots[0].p1 is not None:
+        rec[:, 0] = np.asarray(pots[0].p0, dtype=float)
+        rec[:, 1] = np.sqrt(np.asarray(pots[0].p1, dtype=float))
+        return _lib.FORM_LJ, rec
+    if (kinds == [_lib.POT_EXPREP, _lib.POT_LJ, _lib.POT_EI] and all(p.trunc == TR_NONE for p in pots)
+            and pots[2].alpha > 0 and (pots[2].p1 is None or not np.any(np.asarray(pots[2].p1)))
+            and pots[1].p1 is not None):
+        rec[:, 0] = np.asarray(pots[1].p0, dtype=float)
+        rec[:, 1] = np.sqrt(np.asarray(pots[1].p1, dtype=float))
+        rec[:, 2] = np.asarray(pots[2].p0, dtype=float)/math.sqrt(pots[2].dielectric)
+        rec[:, 3] = np.asarray(pots[0].p0, dtype=float)
+        return _lib.FORM_IONIC, rec
+    return _lib.FORM_GENERIC, rec
 
 
 def _round_up(x, m):
@@ -328,13 +421,13 @@ class Engine(object):
             self.wave_sys[a0//64:(a0 + max(s.natom, 1) + 63)//64] = i
 
     def _plan_sys(self):
-        L = self.lib
         self.hsys = (YnbSys*self.nsys)()
         par = np.zeros((MAX_POT, 3, self.npad), np.float64)
+        rec = np.zeros((self.npad, 4))
         cross = []
         stabs = []
         nbin = 0
-        stab_off = 0
+        stab_off = cross_off = 0
         self.has_ewald = any(s.ewald is not None for s in self.systems)
         # systems whose correction has scaled pairs: ynb_ewald_corr (the others' self term is ew_const)
         self.has_corr = any(s.ewald is not None and s.ewald.correction and len(s.stab) > 0 for s in self.systems)
@@ -343,60 +436,25 @@ class Engine(object):
                             if s.ewald is not None and not s.ewald.correction and len(s.stab) > 0]
         self.groups = 0
         for i, s in enumerate(self.systems):
-            h = self.hsys[i]
-            cell = s.cell
-            for k in range(9):
-                h.rvecs[k] = float(cell.rvecs_full.ravel()[k])
-                h.gvecs[k] = float(cell.gvecs_full.ravel()[k])
-            h.rc = s.rc
-            h.volume = cell.volume
-            h.nvec = cell.nvec
-            h.natom = s.natom
-            h.atom0 = self.atom0[i]
-            h.npot = len(s.pots)
-            for k in range(3):
-                h.rmax[k] = int(s.rmax[k]) if k < cell.nvec else 0
-            h.nlow, h.nhigh = s.nlow, s.nhigh
-            h.ew_nlow, h.ew_nhigh = 0, s.natom
-            h.ew_const = 0.0
-            if s.ewald is not None:
-                h.ew_nlow = s.ewald.nlow
-                h.ew_nhigh = s.natom if s.ewald.nhigh == -1 else s.ewald.nhigh
-                if s.ewald.correction and len(s.stab) == 0:
-                    # the correction is its self term alone: a constant of the reciprocal out
-                    h.ew_const = corr_self_energy(s.ewald.alpha, s.charges, h.ew_nlow, h.ew_nhigh)/s.ewald.dielectric
-            check(L.ynb_sys_plan(h))
+            e = system_entry(self.lib, s, self.atom0[i])
+            h = e.h
+            shift_offsets(h, stab_off, cross_off)
             h.bin0 = nbin
             nbin += h.nbin
+            stabs.extend(e.stabs)
+            stab_off += sum(len(st) for st in e.stabs)
+            cross.extend(e.cross)
+            cross_off += sum(len(c) for c in e.cross)
             a0 = self.atom0[i]
-            for p, pot in enumerate(s.pots):
-                hp = h.pot[p]
-                st = s.stab if pot.stab is None else pot.stab
-                hp.stab0, hp.nstab = stab_off, len(st)
-                stab_off += len(st)
-                if len(st):
-                    stabs.append(np.asarray(st, dtype=SCALING_DTYPE))
-                    h.any_stab = 1
-                hp.kind, hp.trunc, hp.rcut, hp.trunc_par = pot.kind, pot.trunc, pot.rcut, pot.trunc_par
-                hp.alpha, hp.dielectric = pot.alpha, pot.dielectric
-                for q, arr in enumerate((pot.p0, pot.p1, pot.p2)):
-                    if arr is not None:
-                        par[p, q, a0:a0 + s.natom] = np.asarray(arr, dtype=np.float64)
-                if pot.kind == POT_LJ and pot.p1 is not None:
-                    # sqrt(epsilon) per atom: the pair kernel forms eps_ab = se_a se_b
-                    par[p, 2, a0:a0 + s.natom] = np.sqrt(np.asarray(pot.p1, dtype=np.float64))
-                if pot.amp_cross is not None:
-                    nff = pot.amp_cross.shape[0]
-                    hp.nffatype = nff
-                    hp.cross0 = sum(len(c) for c in cross)
-                    cross.append(np.concatenate([np.ascontiguousarray(pot.amp_cross, dtype=float).ravel(),
-                                                 np.ascontiguousarray(pot.b_cross, dtype=float).ravel()]))
+            par[:, :, a0:a0 + s.natom] = e.par
+            rec[a0:a0 + s.natom] = e.rec
+            self.hsys[i] = h
         self.nbin = nbin
         self.bin_sys = np.zeros(max(nbin, 1), np.int32)
         for i in range(self.nsys):
             self.bin_sys[self.hsys[i].bin0:self.hsys[i].bin0 + self.hsys[i].nbin] = i
         self.par_h = par
-        self.rec_h = self._plan_forms()
+        self.rec_h = rec
         self.cross_h = np.concatenate(cross) if cross else np.zeros(1)
         self.stab_h = np.concatenate(stabs) if stabs else np.zeros(1, SCALING_DTYPE)
         if self.has_ewald:
@@ -465,11 +523,7 @@ class Engine(object):
             ch = p['chunks'].copy()
             ch[:, 0] = i
             chunk.append(ch)
-            wk = p['work'].copy()
-            wk[:, 0] = i
-            wk[:, 1] += sb0
-            wk[:, 2] += chunk0
-            work.append(wk)
+            work.append(work_table(p, i, sb0, chunk0))
             if p['grouped']:
                 self.groups = 1
             nch = len(ch)
@@ -577,33 +631,6 @@ class Engine(object):
         self.dt_d = torch.zeros(self.nsys, dtype=torch.float64, device=dev)
         self.ekin2_d = torch.zeros(self.nsys, dtype=torch.float64, device=dev)
         self.md_partial_d = torch.zeros(self.npad//64, dtype=torch.float64, device=dev)
-
-    def _plan_forms(self):
-        """pair_form of every system (a specialised pair kernel for the
-        potential sets of the LJ fluid and of the rocksalt-type ionic force
-        field) and the per-atom records those kernels read."""
-        rec = np.zeros((self.npad, 4))
-        for i, s in enumerate(self.systems):
-            h = self.hsys[i]
-            h.pair_form = _lib.FORM_GENERIC
-            pots = s.pots
-            if any(len(s.stab if p.stab is None else p.stab) for p in pots) or s.natom == 0:
-                continue
-            a0, n = self.atom0[i], s.natom
-            kinds = [p.kind for p in pots]
-            if kinds == [_lib.POT_LJ] and pots[0].trunc in (_lib.TR_NONE, _lib.TR_SWITCH3) and pots[0].p1 is not None:
-                h.pair_form = _lib.FORM_LJ
-                rec[a0:a0 + n, 0] = np.asarray(pots[0].p0, dtype=float)
-                rec[a0:a0 + n, 1] = np.sqrt(np.asarray(pots[0].p1, dtype=float))
-            elif (kinds == [_lib.POT_EXPREP, _lib.POT_LJ, _lib.POT_EI] and all(p.trunc == TR_NONE for p in pots)
-                  and pots[2].alpha > 0 and (pots[2].p1 is None or not np.any(np.asarray(pots[2].p1)))
-                  and pots[1].p1 is not None):
-                h.pair_form = _lib.FORM_IONIC
-                rec[a0:a0 + n, 0] = np.asarray(pots[1].p0, dtype=float)
-                rec[a0:a0 + n, 1] = np.sqrt(np.asarray(pots[1].p1, dtype=float))
-                rec[a0:a0 + n, 2] = np.asarray(pots[2].p0, dtype=float)/math.sqrt(pots[2].dielectric)
-                rec[a0:a0 + n, 3] = np.asarray(pots[0].p0, dtype=float)
-        return rec
 
     def _batch_flags(self):
         """Build paths the batch needs: all-pairs (systems without bins), the

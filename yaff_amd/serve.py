@@ -2,22 +2,28 @@
 
 Requests (a single energy/gradient/virial evaluation, or an MD segment of
 n velocity-Verlet steps) arrive at any time.  The server keeps the systems of
-the admitted requests resident on the device in ONE batch engine and advances
-them together, one batch iteration at a time (kick/drift, neighbour list,
-pair potentials, Ewald reciprocal, kick for every member in the same
-launches):
+the admitted requests resident on the device in ONE persistent batch engine
+(yaff_amd.slots.SlotEngine, created once and kept) and advances them
+together, one batch iteration at a time (kick/drift, neighbour list, pair
+potentials, Ewald reciprocal, kick for every member in the same launches):
 
 * an evaluation needs one iteration (time step 0: the kicks and the drift do
   nothing, the forces are computed);
 * an MD segment needs 1 + n iterations (the first, with time step 0, gives the
   initial forces; the next n are Verlet steps);
-* between quanta of iterations, finished requests leave the batch and waiting
-  ones join it: the batch is re-formed (per-system tables are cached, so this
-  is a concatenation), and the positions, velocities and gradients of the
-  members that stay are carried over on the device;
+* iterations run in quanta; a quantum is enqueued with the copies of the
+  results of the members that finish in it and an event; arrivals are
+  planned by a few host threads (k vectors and tables: native code and
+  large array operations that release the interpreter lock) while the
+  device works; when the event has fired, the neighbour-list status is
+  checked (an overflow grows the capacities and repeats the quantum from
+  its start), finished requests leave their slots and waiting ones are
+  placed in free ranges: only their own slices of the device tables are
+  written, the members that stay are not touched;
 * admission is first come, first served, while the estimated device memory
   of the batch with the candidate stays within the budget and the batch has
-  a free slot.
+  a free slot; a candidate whose planning has not finished waits for a later
+  quantum unless the batch is empty.
 
 Launch parameters are fixed for the server (not re-tuned per batch), and every
 kernel of the engine is deterministic and per-system, so a request gets the
@@ -26,14 +32,16 @@ same bits alone or in any mix.
 The velocity-Verlet splitting follows yaff/sampling/verlet.py:339-367.
 """
 import collections
+import concurrent.futures
 import time
 
 import numpy as np
 import torch
 
 from . import _lib
-from .engine import Engine, _round_up, _ceil_div, ewald_kvectors_cached, ALIGN
+from .engine import _round_up, _ceil_div, ewald_kvectors_cached, ALIGN
 from .batch import nbsystem, timestep, maxwell_boltzmann
+from .slots import SlotEngine
 
 __all__ = ['Request', 'Server', 'estimate_bytes', 'poisson_requests', 'latency_summary', 'SERVER_TUNING']
 
@@ -144,28 +152,38 @@ def latency_summary(done):
 
 
 class Server(object):
-    """Continuous-batching server of Requests on one device."""
+    """Continuous-batching server of Requests on one device, over one
+    persistent SlotEngine (created on the first run and kept)."""
 
     def __init__(self, precision='mixed', hbm_budget=64 << 30, max_systems=64, max_quantum=4,
-                 tuning=SERVER_TUNING):
+                 tuning=SERVER_TUNING, planners=4, reserve_atoms=1 << 16):
+        self.planners = int(planners)
+        self.reserve_atoms = int(reserve_atoms)   # starting atom capacity of the engine
         self.precision = precision
         self.hbm_budget = int(hbm_budget)
         self.max_systems = int(max_systems)
         self.max_quantum = int(max_quantum)
         self.tuning = dict(tuning)
-        self.active = []            # admitted requests, in batch order
+        self.active = []            # admitted requests
         self.waiting = collections.deque()
         self.engine = None
-        self.engine_members = []
+        self.slot_of = {}           # id(request) -> slot
         self.done = []
         self.stats = collections.Counter()
         self.peak_bytes = 0
 
-    # -- admission ---------------------------------------------------------
+    # -- admission policy (host only) --------------------------------------
     def _admit(self, now):
-        changed = False
+        """Moves waiting requests into the batch, first come first served,
+        while the estimated device memory of the batch stays within the
+        budget and a slot is free; returns the admitted requests."""
+        new = []
         while self.waiting and len(self.active) < self.max_systems:
             cand = self.waiting[0]
+            f = getattr(cand, '_plan', None)
+            if f is not None and not f.done() and self.active:
+                break     # join at a later quantum rather than keep the device waiting
+            self._planned(cand)   # its k vectors, which the estimate counts
             need = estimate_bytes([r.nb for r in self.active] + [cand.nb], self.tuning)
             if need > self.hbm_budget:
                 if not self.active:
@@ -175,104 +193,130 @@ class Server(object):
             self.waiting.popleft()
             cand.t_admit = now
             self.active.append(cand)
+            new.append(cand)
             self.peak_bytes = max(self.peak_bytes, need)
-            changed = True
-        return changed
+        return new
 
-    # -- batch formation ----------------------------------------------------
-    def _form(self):
-        """Builds the engine of the current members; carries the device state
-        of members that were in the previous batch."""
+    @staticmethod
+    def _planned(r):
+        """The request's planning (SlotEngine.prepare, run by the planning
+        threads from its arrival on); waits for it when not done yet."""
+        f = getattr(r, '_plan', None)
+        return None if f is None else f.result()
+
+    # -- device side ---------------------------------------------------------
+    def _engine(self):
+        if self.engine is None:
+            t = self.tuning
+            self.engine = SlotEngine(self.max_systems, self.precision, pair_lanes=t['pair_lanes'], apl=t['apl'],
+                                     skb=t['skb'], fkb=t['fkb'], fapl=t['fapl'], atom_cap=self.reserve_atoms)
+            self._status_h = torch.zeros(4, dtype=torch.int32).pin_memory()
+            self._total_h = torch.zeros((self.max_systems, 10), dtype=torch.float64).pin_memory()
+        return self.engine
+
+    def _place(self, new):
+        """Places newly admitted requests on the engine and commits every
+        change since the last commit (their slices, the slots freed by
+        retired requests)."""
         t0 = time.perf_counter()
-        old = self.engine
-        old_members = self.engine_members
-        eng = Engine([r.nb for r in self.active], precision=self.precision, upload=False,
-                     pair_lanes=self.tuning['pair_lanes'], apl=self.tuning['apl'], skb=self.tuning['skb'],
-                     fkb=self.tuning['fkb'], fapl=self.tuning['fapl'])
-        host_pos = np.zeros((eng.npad, 3))
-        host_vel = np.zeros((eng.npad, 3))
-        src, dst = [], []
-        old_index = {id(r): i for i, r in enumerate(old_members)}
-        for i, r in enumerate(self.active):
-            a0 = eng.atom0[i]
-            j = old_index.get(id(r))
-            if j is not None and old is not None and r.iters_done > 0:
-                b0 = old.atom0[j]
-                src.append(np.arange(b0, b0 + r.natom))
-                dst.append(np.arange(a0, a0 + r.natom))
-            else:
-                host_pos[a0:a0 + r.natom] = r.nb.pos
-                if r.vel is not None:
-                    host_vel[a0:a0 + r.natom] = r.vel
-        eng.pos_d.copy_(torch.from_numpy(host_pos))
-        eng.vel_d.copy_(torch.from_numpy(host_vel))
-        if src:
-            si = torch.from_numpy(np.concatenate(src)).to(eng.device)
-            di = torch.from_numpy(np.concatenate(dst)).to(eng.device)
-            for name in ('pos_d', 'vel_d', 'gpos_d'):
-                getattr(eng, name).index_copy_(0, di, getattr(old, name).index_select(0, si))
-        self.engine = eng
-        self.engine_members = list(self.active)
+        eng = self.engine
+        for r in new:
+            self.slot_of[id(r)] = eng.admit(r.nb, vel=r.vel if r.kind == 'md' else None, prepared=self._planned(r))
+        eng.commit()
         self.stats['forms'] += 1
         self.stats['form_s'] += time.perf_counter() - t0
 
-    # -- iterations --------------------------------------------------------
-    def _quantum(self):
+    def _launch(self):
+        """Enqueues one quantum of iterations of every member, the copies of
+        the results of the members that finish in it and an event; returns
+        the in-flight record.  Nothing here waits for the device."""
         eng = self.engine
         nq = min(min(r.iters_total - r.iters_done for r in self.active), self.max_quantum)
         dts = np.zeros((nq, eng.nsys))
-        for i, r in enumerate(self.active):
-            for q in range(nq):
-                it = r.iters_done + q
-                dts[q, i] = r.dt if (r.kind == 'md' and it > 0) else 0.0
-        dts_d = torch.from_numpy(dts).to(eng.device)
-        snap = [t.clone() for t in (eng.pos_d, eng.vel_d, eng.gpos_d)]
-        for attempt in range(3):
-            for q in range(nq):
-                eng.dt_d.copy_(dts_d[q])
-                eng.md_step()
-            flags, longest = eng.nlist_status()   # synchronises
-            if not flags:
-                break
-            # a row or a bin overflowed: grow and redo the quantum from the snapshot
-            eng.grow(flags, longest)
-            for t, v in zip((eng.pos_d, eng.vel_d, eng.gpos_d), snap):
-                t.copy_(v)
-            self.stats['regrow'] += 1
-        else:
-            raise RuntimeError('neighbour list capacities still exceeded')
-        self.stats['iterations'] += nq
         for r in self.active:
-            r.iters_done += nq
-        return nq
+            sl = self.slot_of[id(r)]
+            for q in range(nq):
+                dts[q, sl] = r.dt if (r.kind == 'md' and r.iters_done + q > 0) else 0.0
+        dts_h = torch.empty(dts.shape, dtype=torch.float64, pin_memory=True)
+        dts_h.numpy()[:] = dts
+        dts_d = dts_h.to(eng.device, non_blocking=True)
+        npad = eng.npad
+        snap = [t[:npad].clone() for t in (eng.pos_d, eng.vel_d, eng.gpos_d)]
+        for q in range(nq):
+            eng.dt_d.copy_(dts_d[q])
+            eng.md_step()
+        self._status_h.copy_(eng.status_d, non_blocking=True)
+        self._total_h.copy_(eng.total_d, non_blocking=True)
+        fin = []
+        for r in self.active:
+            if r.iters_done + nq < r.iters_total:
+                continue
+            sl = self.slot_of[id(r)]
+            a0 = eng.atom0[sl]
+            bufs = {}
+            names = ('gpos_d', 'pos_d', 'vel_d') if r.kind == 'md' else ('gpos_d',)
+            for name in names:
+                b = torch.empty((r.natom, 3), dtype=torch.float64, pin_memory=True)
+                b.copy_(getattr(eng, name)[a0:a0 + r.natom], non_blocking=True)
+                bufs[name] = b
+            fin.append((r, sl, bufs))
+        ev = torch.cuda.Event()
+        ev.record()
+        self.stats['iterations'] += nq
+        return dict(event=ev, nq=nq, snap=snap, npad=npad, fin=fin)
 
-    def _retire(self, now):
+    def _finish(self, fl, now, attempt=0):
+        """After a quantum's event: validates the neighbour lists (on an
+        overflow: grow, restore the quantum's start and run it again),
+        advances the members and retires the finished ones (their results
+        are already on the host)."""
         eng = self.engine
-        fin = [i for i, r in enumerate(self.active) if r.iters_done >= r.iters_total]
-        if not fin:
-            return False
-        tot = eng.total_d.cpu().numpy()
-        for i in fin:
-            r = self.active[i]
-            a0 = eng.atom0[i]
-            r.energy = float(tot[i, 0])
-            r.vtens = tot[i, 1:].reshape(3, 3).copy()
-            r.gpos = eng.gpos_d[a0:a0 + r.natom].cpu().numpy()
+        flags = int(self._status_h[0])
+        if flags:
+            if attempt >= 2:
+                raise RuntimeError('neighbour list capacities still exceeded')
+            longest = int(eng.cnt_d[:eng.npad].max().item()) if flags & 1 else 0
+            if flags & 4:
+                eng._fullest_bin = int(eng.status_d[1].item())
+            eng.status_d.zero_()
+            eng.grow(flags, longest)
+            for t, v in zip((eng.pos_d, eng.vel_d, eng.gpos_d), fl['snap']):
+                t[:fl['npad']].copy_(v)
+            self.stats['regrow'] += 1
+            self.stats['iterations'] -= fl['nq']
+            again = self._launch()
+            again['event'].synchronize()
+            return self._finish(again, now, attempt + 1)
+        for r in self.active:
+            r.iters_done += fl['nq']
+        tot = self._total_h.numpy()
+        retired = []
+        for r, sl, bufs in fl['fin']:
+            r.energy = float(tot[sl, 0])
+            r.vtens = tot[sl, 1:].reshape(3, 3).copy()
+            r.gpos = bufs['gpos_d'].numpy().copy()
             if r.kind == 'md':
-                r.pos = eng.pos_d[a0:a0 + r.natom].cpu().numpy()
-                r.vel_out = eng.vel_d[a0:a0 + r.natom].cpu().numpy()
+                r.pos = bufs['pos_d'].numpy().copy()
+                r.vel_out = bufs['vel_d'].numpy().copy()
             r.t_done = now()
             self.done.append(r)
-        keep = [r for i, r in enumerate(self.active) if i not in set(fin)]
-        self.active = keep
-        return True
+            eng.retire(sl)
+            del self.slot_of[id(r)]
+            retired.append(r)
+        if retired:
+            gone = set(id(r) for r in retired)
+            self.active = [r for r in self.active if id(r) not in gone]
+        return retired
 
     # -- driver ------------------------------------------------------------
     def run(self, requests, realtime=True):
         """Serves ``requests`` (arrival times in seconds from the start when
         realtime, else all at once) and returns them completed, in completion
         order.  Latency = completion - arrival, on the host clock, with the
-        results on the host."""
+        results on the host.  While a quantum runs on the device, the host
+        takes in arrivals and plans their tables, and polls the quantum's
+        event."""
+        self._engine()
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         t0 = time.perf_counter()
 
@@ -282,21 +326,37 @@ class Server(object):
             for r in pending:
                 r.arrival = 0.0
         self.done = []
-        while pending or self.waiting or self.active:
+        inflight = None
+        retired = []
+        pool = concurrent.futures.ThreadPoolExecutor(self.planners)
+
+        def arrivals():
             t = now()
             while pending and pending[0].arrival <= t:
-                self.waiting.append(pending.popleft())
+                r = pending.popleft()
+                r._plan = pool.submit(self.engine.prepare, r.nb)
+                self.waiting.append(r)
+        while pending or self.waiting or self.active or inflight is not None:
+            arrivals()
+            if inflight is not None:
+                t1 = time.perf_counter()
+                while not inflight['event'].query():
+                    arrivals()
+                    time.sleep(0)
+                self.stats['wait_s'] += time.perf_counter() - t1
+                retired = self._finish(inflight, now)
+                inflight = None
+                arrivals()
             if not self.active and not self.waiting:
-                wait = pending[0].arrival - now()
-                if wait > 0:
-                    time.sleep(min(wait, 0.001))
+                if pending:
+                    wait = pending[0].arrival - now()
+                    if wait > 0:
+                        time.sleep(min(wait, 0.001))
                 continue
-            self._admit(t)
-            if self.engine is None or [id(r) for r in self.engine_members] != [id(r) for r in self.active]:
-                self._form()
-            self._quantum()
-            self._retire(now)
-            if not self.active:
-                self.engine = None
-                self.engine_members = []
+            new = self._admit(now())
+            if new or retired:
+                self._place(new)
+                retired = []
+            inflight = self._launch()
+        pool.shutdown()
         return self.done
