@@ -405,9 +405,10 @@ def served(nreq, rates):
     rocksalt alternating, triclinic angles 80-100 degrees) served by the
     continuous-batching server, FP64 and mixed, at each arrival rate, and at
     saturation (every request waiting at the start): p50/p99 latency (arrival
-    to results on the host), throughput, and the host's share of the span
-    spent placing requests (admission, retirement and table uploads) and
-    waiting for quanta."""
+    to results on the host), throughput, the host's share of the span spent
+    placing requests (admission, retirement and table uploads, overlapped
+    with the quantum in flight) and the device's busy share (HIP-event time
+    of the quanta)."""
     from yaff_amd import serve
     out = {'workload': '%d requests per rate, log-uniform 100-50000 atoms, 50%% eval / 50%% 20-step MD' % nreq}
     warm = serve.poisson_requests(6, 1000.0, natom_range=(100, 3000), nsteps=2, seed=99)
@@ -423,12 +424,14 @@ def served(nreq, rates):
             summ.update({'rate_per_s': 'saturation' if rate is None else rate, 'commits': srv.stats['forms'],
                          'iterations': srv.stats['iterations'], 'form_s': srv.stats['form_s'],
                          'form_frac': srv.stats['form_s']/summ['span_s'], 'wait_s': srv.stats['wait_s'],
+                         'device_busy_frac': srv.stats['device_s']/summ['span_s'],
                          'regrow': srv.stats['regrow'], 'peak_est_bytes': srv.peak_bytes,
                          'engine_bytes': srv.engine.device_bytes()})
             res.append(summ)
-            progress('served %s rate %s: p50 %.1f ms p99 %.1f ms, %.3g atom-evals/s, placing %.1f%% of the span'
-                     % (prec, summ['rate_per_s'], summ['p50_ms'], summ['p99_ms'], summ['throughput'],
-                        100*summ['form_frac']))
+            progress('served %s rate %s: p50 %.1f ms p99 %.1f ms, %.3g atom-evals/s, host placing %.1f%% and '
+                     'device busy %.1f%% of the span' % (prec, summ['rate_per_s'], summ['p50_ms'], summ['p99_ms'],
+                                                        summ['throughput'], 100*summ['form_frac'],
+                                                        100*summ['device_busy_frac']))
         out[prec] = res
     return out
 

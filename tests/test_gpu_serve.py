@@ -92,3 +92,21 @@ def test_admission_budget_and_mixed_accuracy():
     est = serve.estimate_bytes([r.nb])
     print('estimate %d engine tables %d' % (est, used))
     assert 0.5*used <= est <= 2.0*used
+
+
+def test_overflow_restarts_requests_with_same_bits(monkeypatch):
+    """A neighbour-list overflow (row capacities forced far too small from
+    the host) is caught after its quantum: capacities grow and the affected
+    requests run again from their inputs, with the bits of a run without
+    the overflow."""
+    from yaff_amd import serve, slots
+    print('case serve overflow restart', flush=True)
+    ref = _requests(seed=13, n=6)
+    serve.Server(precision='fp64').run(ref, realtime=False)
+    monkeypatch.setattr(slots, '_nlist_kcap', lambda s: 8)
+    rs = _requests(seed=13, n=6)
+    srv = serve.Server(precision='fp64', max_quantum=2)
+    done = srv.run(rs, realtime=False)
+    assert len(done) == len(rs) and srv.stats['regrow'] >= 1
+    for a, b in zip(rs, ref):
+        _same(a, b)

@@ -68,7 +68,8 @@ class Request(object):
         if kind == 'md' and self.vel is None:
             self.vel = maxwell_boltzmann(system.masses, 120.0 if system.spec.kind == 'lj' else 300.0, rid)
         self.iters_total = 1 + self.nsteps
-        self.iters_done = 0
+        self.iters_sched = 0    # iterations enqueued on the device
+        self.iters_done = 0     # iterations completed
         # results
         self.energy = None
         self.gpos = None
@@ -156,8 +157,9 @@ class Server(object):
     persistent SlotEngine (created on the first run and kept)."""
 
     def __init__(self, precision='mixed', hbm_budget=64 << 30, max_systems=64, max_quantum=4,
-                 tuning=SERVER_TUNING, planners=4, reserve_atoms=1 << 16):
+                 tuning=SERVER_TUNING, planners=4, reserve_atoms=1 << 16, depth=2):
         self.planners = int(planners)
+        self.depth = int(depth)                   # quanta in flight
         self.reserve_atoms = int(reserve_atoms)   # starting atom capacity of the engine
         self.precision = precision
         self.hbm_budget = int(hbm_budget)
@@ -218,38 +220,43 @@ class Server(object):
         """Places newly admitted requests on the engine and commits every
         change since the last commit (their slices, the slots freed by
         retired requests)."""
-        t0 = time.perf_counter()
         eng = self.engine
         for r in new:
             self.slot_of[id(r)] = eng.admit(r.nb, vel=r.vel if r.kind == 'md' else None, prepared=self._planned(r))
         eng.commit()
         self.stats['forms'] += 1
-        self.stats['form_s'] += time.perf_counter() - t0
 
     def _launch(self):
         """Enqueues one quantum of iterations of every member, the copies of
-        the results of the members that finish in it and an event; returns
-        the in-flight record.  Nothing here waits for the device."""
+        the results of the members that finish in it, the neighbour-list
+        status and an event, then frees the slots of those members (their
+        tables are rewritten only by later commits, after the copies in
+        stream order).  Returns the in-flight record; nothing here waits for
+        the device."""
         eng = self.engine
-        nq = min(min(r.iters_total - r.iters_done for r in self.active), self.max_quantum)
-        dts = np.zeros((nq, eng.nsys))
+        nq = min(min(r.iters_total - r.iters_sched for r in self.active), self.max_quantum)
+        dts_h = torch.zeros((nq, eng.nsys), dtype=torch.float64, pin_memory=True)
+        dts = dts_h.numpy()
         for r in self.active:
             sl = self.slot_of[id(r)]
             for q in range(nq):
-                dts[q, sl] = r.dt if (r.kind == 'md' and r.iters_done + q > 0) else 0.0
-        dts_h = torch.empty(dts.shape, dtype=torch.float64, pin_memory=True)
-        dts_h.numpy()[:] = dts
+                dts[q, sl] = r.dt if (r.kind == 'md' and r.iters_sched + q > 0) else 0.0
         dts_d = dts_h.to(eng.device, non_blocking=True)
-        npad = eng.npad
-        snap = [t[:npad].clone() for t in (eng.pos_d, eng.vel_d, eng.gpos_d)]
+        start = torch.cuda.Event(enable_timing=True)
+        start.record()
         for q in range(nq):
             eng.dt_d.copy_(dts_d[q])
             eng.md_step()
-        self._status_h.copy_(eng.status_d, non_blocking=True)
-        self._total_h.copy_(eng.total_d, non_blocking=True)
+        status_h = torch.empty(4, dtype=torch.int32, pin_memory=True)
+        status_h.copy_(eng.status_d, non_blocking=True)
+        eng.status_d.zero_()      # flags of this quantum's builds only
+        total_h = torch.empty((eng.nsys, 10), dtype=torch.float64, pin_memory=True)
+        total_h.copy_(eng.total_d, non_blocking=True)
         fin = []
-        for r in self.active:
-            if r.iters_done + nq < r.iters_total:
+        members = list(self.active)
+        for r in members:
+            r.iters_sched += nq
+            if r.iters_sched < r.iters_total:
                 continue
             sl = self.slot_of[id(r)]
             a0 = eng.atom0[sl]
@@ -260,37 +267,28 @@ class Server(object):
                 b.copy_(getattr(eng, name)[a0:a0 + r.natom], non_blocking=True)
                 bufs[name] = b
             fin.append((r, sl, bufs))
-        ev = torch.cuda.Event()
+        ev = torch.cuda.Event(enable_timing=True)
         ev.record()
+        for r, sl, _ in fin:
+            eng.retire(sl)
+            del self.slot_of[id(r)]
+        if fin:
+            gone = set(id(r) for r, _, _ in fin)
+            self.active = [r for r in self.active if id(r) not in gone]
         self.stats['iterations'] += nq
-        return dict(event=ev, nq=nq, snap=snap, npad=npad, fin=fin)
+        return dict(event=ev, start=start, nq=nq, fin=fin, members=members, status=status_h, total=total_h)
 
-    def _finish(self, fl, now, attempt=0):
-        """After a quantum's event: validates the neighbour lists (on an
-        overflow: grow, restore the quantum's start and run it again),
-        advances the members and retires the finished ones (their results
-        are already on the host)."""
-        eng = self.engine
-        flags = int(self._status_h[0])
-        if flags:
-            if attempt >= 2:
-                raise RuntimeError('neighbour list capacities still exceeded')
-            longest = int(eng.cnt_d[:eng.npad].max().item()) if flags & 1 else 0
-            if flags & 4:
-                eng._fullest_bin = int(eng.status_d[1].item())
-            eng.status_d.zero_()
-            eng.grow(flags, longest)
-            for t, v in zip((eng.pos_d, eng.vel_d, eng.gpos_d), fl['snap']):
-                t[:fl['npad']].copy_(v)
-            self.stats['regrow'] += 1
-            self.stats['iterations'] -= fl['nq']
-            again = self._launch()
-            again['event'].synchronize()
-            return self._finish(again, now, attempt + 1)
-        for r in self.active:
+    def _complete(self, fl, now):
+        """After a quantum's event: its finished requests get their results
+        (already on the host).  Returns False when its neighbour-list builds
+        overflowed (the caller restarts the affected requests)."""
+        if int(fl['status'][0]):
+            return False
+        # device time of the quantum: from the stream reaching its start to its end
+        self.stats['device_s'] += 1e-3*fl['start'].elapsed_time(fl['event'])
+        for r in fl['members']:
             r.iters_done += fl['nq']
-        tot = self._total_h.numpy()
-        retired = []
+        tot = fl['total'].numpy()
         for r, sl, bufs in fl['fin']:
             r.energy = float(tot[sl, 0])
             r.vtens = tot[sl, 1:].reshape(3, 3).copy()
@@ -300,22 +298,51 @@ class Server(object):
                 r.vel_out = bufs['vel_d'].numpy().copy()
             r.t_done = now()
             self.done.append(r)
-            eng.retire(sl)
-            del self.slot_of[id(r)]
-            retired.append(r)
-        if retired:
-            gone = set(id(r) for r in retired)
-            self.active = [r for r in self.active if id(r) not in gone]
-        return retired
+        return True
+
+    def _recover(self, inflight, first):
+        """An overflow in the quantum `first` of the in-flight records: wait
+        for the device, grow the capacities the flags name, empty the batch
+        and put every request of that quantum and the later ones back at the
+        head of the queue, to run again from its inputs (every result is a
+        function of the request alone, so the rerun gives the bits a run
+        without the overflow gives)."""
+        eng = self.engine
+        torch.cuda.synchronize()
+        flags = 0
+        for fl in inflight:
+            flags |= int(fl['status'][0])
+        if flags & 1:
+            longest = int(eng.cnt_d[:eng.npad].max().item())
+            eng.grow(1, max(longest, 2*eng.kcap))
+        if flags & 4:
+            eng._fullest_bin = max(int(eng.status_d[1].item()), 2*eng.bin_cap)
+            eng.grow(4, 0)
+        if flags & 2:
+            eng.grow(2, 0)
+        eng.status_d.zero_()
+        redo = {}
+        for fl in inflight:
+            for r in fl['members']:
+                redo[id(r)] = r
+        for r in self.active:
+            eng.retire(self.slot_of.pop(id(r)))
+        self.active = []
+        for r in sorted(redo.values(), key=lambda r: (r.arrival, r.rid), reverse=True):
+            r.iters_sched = r.iters_done = 0
+            r.t_admit = None
+            self.waiting.appendleft(r)
+        eng.commit()
+        self.stats['regrow'] += 1
 
     # -- driver ------------------------------------------------------------
     def run(self, requests, realtime=True):
         """Serves ``requests`` (arrival times in seconds from the start when
         realtime, else all at once) and returns them completed, in completion
         order.  Latency = completion - arrival, on the host clock, with the
-        results on the host.  While a quantum runs on the device, the host
-        takes in arrivals and plans their tables, and polls the quantum's
-        event."""
+        results on the host.  Up to `depth` quanta are in flight: the next
+        quantum is placed and enqueued while the device runs the previous
+        one; arrivals are planned by a thread pool meanwhile."""
         self._engine()
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         t0 = time.perf_counter()
@@ -325,9 +352,10 @@ class Server(object):
         if not realtime:
             for r in pending:
                 r.arrival = 0.0
+        for r in pending:
+            r.iters_sched = r.iters_done = 0
         self.done = []
-        inflight = None
-        retired = []
+        inflight = collections.deque()
         pool = concurrent.futures.ThreadPoolExecutor(self.planners)
 
         def arrivals():
@@ -336,27 +364,34 @@ class Server(object):
                 r = pending.popleft()
                 r._plan = pool.submit(self.engine.prepare, r.nb)
                 self.waiting.append(r)
-        while pending or self.waiting or self.active or inflight is not None:
-            arrivals()
-            if inflight is not None:
-                t1 = time.perf_counter()
-                while not inflight['event'].query():
-                    arrivals()
-                    time.sleep(0)
-                self.stats['wait_s'] += time.perf_counter() - t1
-                retired = self._finish(inflight, now)
-                inflight = None
+        try:
+            while pending or self.waiting or self.active or inflight:
                 arrivals()
-            if not self.active and not self.waiting:
-                if pending:
-                    wait = pending[0].arrival - now()
-                    if wait > 0:
-                        time.sleep(min(wait, 0.001))
-                continue
-            new = self._admit(now())
-            if new or retired:
-                self._place(new)
-                retired = []
-            inflight = self._launch()
-        pool.shutdown()
+                if inflight and (len(inflight) >= self.depth or not (self.active or self.waiting)):
+                    t1 = time.perf_counter()
+                    while not inflight[0]['event'].query():
+                        arrivals()
+                        time.sleep(0)
+                    self.stats['wait_s'] += time.perf_counter() - t1
+                    if not self._complete(inflight[0], now):
+                        self._recover(inflight, 0)
+                        inflight.clear()
+                    else:
+                        inflight.popleft()
+                    continue
+                if not self.active and not self.waiting:
+                    if pending:
+                        wait = pending[0].arrival - now()
+                        if wait > 0:
+                            time.sleep(min(wait, 0.001))
+                    continue
+                t1 = time.perf_counter()
+                new = self._admit(now())
+                if new or self.engine.changed():
+                    self._place(new)
+                self.stats['form_s'] += time.perf_counter() - t1
+                if self.active:
+                    inflight.append(self._launch())
+        finally:
+            pool.shutdown()
         return self.done

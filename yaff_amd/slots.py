@@ -450,6 +450,10 @@ class SlotEngine(Engine):
         self._pending.append((slot, vel))
         return slot
 
+    def changed(self):
+        """Whether admissions or retirements wait for a commit()."""
+        return bool(self._pending or self._stage_free.ops)
+
     def retire(self, slot):
         """Frees slot's ranges and empties the slot (takes effect on the
         device at the next commit())."""
