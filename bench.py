@@ -58,6 +58,9 @@ def parse():
     ap.add_argument('--batch-steps', type=int, default=200)
     ap.add_argument('--batch-systems', type=int, default=64)
     ap.add_argument('--no-served', action='store_true')
+    ap.add_argument('--no-config0', action='store_true', help='skip the 864-atom LJ fluid line')
+    ap.add_argument('--no-cpu-aggregate', action='store_true',
+                    help='skip the all-core oracle runs on the batched and served workloads')
     ap.add_argument('--no-precision', action='store_true',
                     help='skip the 55,296-atom triclinic FP64-versus-mixed comparison')
     ap.add_argument('--serve-requests', type=int, default=150)
@@ -293,7 +296,128 @@ def precision_large(steps, warmup):
         'force_max_rel_err': float(np.sqrt(((gm - g64)**2).sum(axis=1)).max()/fn.max()),
         'virial_rel_err': float(np.abs(vm - v64).max()/np.abs(v64).max()),
         'reference': 'FP64 GPU evaluation of the same positions'}
+    # both against the oracle: molmod/yaff's own C kernels on the same system
+    # (tests/golden/rocksalt55296_ref.npz: energy, virial and the gradient of
+    # 8192 seeded atoms)
+    fx = os.path.join(ROOT, 'tests', 'golden', 'rocksalt55296_ref.npz')
+    if os.path.exists(fx):
+        ref = np.load(fx)
+        idx, gref = ref['sample'], ref['gpos_sample']
+        gn = np.sqrt((gref**2).sum(axis=1).mean())
+        for prec, (e, g, v) in res.items():
+            d = g[idx] - gref
+            out['vs_oracle_' + prec] = {
+                'energy_rel_err': float(abs(e - float(ref['e']))/abs(float(ref['e']))),
+                'force_rms_rel_err': float(np.sqrt((d**2).sum(axis=1).mean())/gn),
+                'force_max_rel_err': float(np.sqrt((d**2).sum(axis=1)).max()/np.sqrt((gref**2).sum(axis=1)).max()),
+                'virial_rel_err': float(np.abs(v - ref['vtens']).max()/np.abs(ref['vtens']).max()),
+                'reference': 'molmod/yaff C kernels compiled from the reference sources (fixture '
+                             'tests/golden/rocksalt55296_ref.npz; forces on its 8192 sampled atoms)'}
     return out
+
+
+def config0(steps, warmup, cpu_evals=20):
+    """The 864-atom LJ fluid (6x6x6 FCC cells, L = 34.7 A, jitter seed 0),
+    FP64: `steps` velocity-Verlet NVE steps at dt = 2 fs from Maxwell-Boltzmann
+    velocities at 120 K (seed 1), replayed from a HIP graph; the conserved
+    energy's drift over the run; the oracle's FP64 rate on the same system
+    (neighbour list and LJ with Switch3, one host core)."""
+    import torch
+    from yaff_amd import workloads, autotune
+    from yaff_amd.batch import nbsystem
+    from yaff_amd.engine import Engine
+    from yaff_amd.units import femtosecond
+    s = workloads.lj_fluid((6, 6, 6), seed=0)
+    n = s.natom
+    eng = Engine([nbsystem(s)], precision='fp64')
+    eng.ensure_nlist()
+    tun = autotune.tune(eng, cache=None)
+    eng.compute(build=False)
+    eng.set_velocities([maxwell_boltzmann(s.masses, 120.0, 1)])
+    eng.set_timestep(2.0*femtosecond)
+    torch.cuda.synchronize()
+
+    def conserved():
+        return float(eng.total_d[0, 0].item()) + 0.5*float(eng.ekin2_d[0].item())
+    run_steps(eng, max(warmup, 1))
+    torch.cuda.synchronize()
+    e0 = conserved()
+    dt = timed_graph(eng, steps, 0)
+    e1 = conserved()
+    ekin = 0.5*float(eng.ekin2_d[0].item())
+    out = {'workload': 'LJ fluid 864 atoms (6x6x6 FCC cells, L=34.7 A), sigma 3.405 A, eps 3.79e-4 Ha, rcut 12 A, '
+                       'Switch3 4 A, FP64, NVE dt 2 fs, 120 K', 'natom': n, 'steps': steps,
+           'value': n*steps/dt, 'unit': 'atom-force-evals/s', 'ms_per_step': dt/steps*1e3, 'tuning': tun,
+           'conserved_energy_drift_over_kinetic': abs(e1 - e0)/ekin}
+    try:
+        from oracle import oracle as orc
+        O = orc.Oracle()
+        sp = s.spec
+        cell = O.cell(s.cell.rvecs)
+        pot = orc.Pot(orc.LJ, sp.rcut, orc.TR_SWITCH3, sp.width, p0=np.full(n, sp.sigma), p1=np.full(n, sp.epsilon))
+        t0 = time.perf_counter()
+        for _ in range(cpu_evals):
+            rows = O.nlist_build(s.pos, cell, sp.rcut, orc.rmax_of(cell, sp.rcut))
+            O.pair_compute(rows, None, pot, n)
+        t = time.perf_counter() - t0
+        out['cpu_oracle'] = {'value': n*cpu_evals/t, 'cores': 1, 'kind': 'port',
+                             'sample': '%d FP64 evaluations (neighbour list + LJ), %.2f s' % (cpu_evals, t)}
+    except OSError as exc:
+        out['cpu_oracle'] = {'value': None, 'sample': 'unavailable: %s' % exc}
+    del eng
+    torch.cuda.empty_cache()
+    return out
+
+
+def _oracle_eval(s):
+    """One FP64 oracle evaluation of a workload system (neighbour list, its
+    pair potentials, and for rocksalt the Ewald reciprocal and correction):
+    (atoms, seconds).  Runs in a worker process."""
+    from oracle import oracle as orc
+    O = orc.Oracle()
+    sp = s.spec
+    n = s.natom
+    cell = O.cell(s.cell.rvecs)
+    t0 = time.perf_counter()
+    rows = O.nlist_build(s.pos, cell, sp.rcut, orc.rmax_of(cell, sp.rcut))
+    if sp.kind == 'lj':
+        O.pair_compute(rows, None, orc.Pot(orc.LJ, sp.rcut, orc.TR_SWITCH3, sp.width, p0=np.full(n, sp.sigma),
+                                           p1=np.full(n, sp.epsilon)), n)
+    else:
+        for p in (orc.Pot(orc.EXPREP, sp.rcut, ffatype_ids=s.ffatype_ids, amp_cross=np.full((2, 2), sp.exprep_a),
+                          b_cross=np.full((2, 2), sp.exprep_b)),
+                  orc.Pot(orc.LJ, sp.rcut, p0=np.full(n, sp.lj_sigma), p1=np.full(n, sp.lj_epsilon)),
+                  orc.Pot(orc.EI, sp.rcut, p0=s.charges, alpha=sp.alpha)):
+            O.pair_compute(rows, None, p, n)
+        O.ewald_reci(s.pos, s.charges, cell, sp.alpha, orc.gmax_of(cell, sp.gcut), sp.gcut)
+        O.ewald_corr(s.pos, s.charges, cell, sp.alpha, None)
+    return n, time.perf_counter() - t0
+
+
+def _oracle_warm(_):
+    from oracle import oracle as orc
+    orc.Oracle()
+    return 0
+
+
+def cpu_aggregate(systems, label):
+    """The oracle on every host core at once: one worker process per core
+    (spawned: the workers touch no GPU), each system evaluated once; the
+    aggregate rate is the atoms evaluated over the wall time."""
+    import multiprocessing as mp
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, 16)          # the host share of one GPU
+    ctx = mp.get_context('spawn')
+    with ctx.Pool(cores) as pool:
+        pool.map(_oracle_warm, range(cores))                # workers started, oracle loaded
+        t0 = time.perf_counter()
+        res = pool.map(_oracle_eval, systems, chunksize=1)
+        wall = time.perf_counter() - t0
+    natom = sum(r[0] for r in res)
+    return {'value': natom/wall, 'unit': 'atom-force-evals/s', 'cores': cores, 'kind': 'port',
+            'sample': '%s: %d systems, %d atoms, one FP64 evaluation each, %.1f s wall (%.1f s of one-core time)'
+                      % (label, len(systems), natom, wall, sum(r[1] for r in res)),
+            'one_core_value': natom/sum(r[1] for r in res)}
 
 
 def progress(msg):
@@ -562,6 +686,9 @@ def main():
                              'kernel': 'torch elementwise mul (out=)'}
         del src, dst
     progress('single-system FP64/mixed done')
+    if not args.no_config0:
+        out['config0'] = config0(100, W)
+        progress('config 0 (864-atom LJ, 100 NVE steps) done: %.3g atom-evals/s' % out['config0']['value'])
     if not args.no_precision:
         out['precision'] = precision_large(max(K//2, 1), 2)
         progress('precision (55,296 atoms) done: mixed RMS force error %.2e'
@@ -589,6 +716,18 @@ def main():
         except OSError as exc:
             out['cpu_baseline'] = {'value': None, 'unit': 'atom-force-evals/s', 'cores': 1, 'kind': 'port',
                                    'sample': 'unavailable: %s' % exc}
+        if not args.no_cpu_aggregate and out['cpu_baseline']['value'] is not None:
+            # all host cores on the batched workload and on a bounded sample of the served stream
+            from yaff_amd import workloads, serve
+            agg = {'batched': cpu_aggregate(workloads.batch_of_systems(args.batch_systems, (200, 6000),
+                                                                      (80.0, 100.0), seed=0),
+                                            'the batched workload (200-6000 atoms)')}
+            stream = [r.system for r in serve.poisson_requests(args.serve_requests, 400.0, seed=11)
+                      if r.natom <= 6000][:64]
+            agg['served'] = cpu_aggregate(stream, 'served-stream requests of at most 6000 atoms')
+            out['cpu_baseline']['all_cores'] = agg
+            progress('cpu aggregate: batched %.3g, served sample %.3g atom-evals/s on %d cores'
+                     % (agg['batched']['value'], agg['served']['value'], agg['batched']['cores']))
     print(json.dumps(out))
 
 

@@ -16,7 +16,7 @@ class Scalings(object):
                 raise ValueError('%s must be in the range [-1,1].' % name)
         self.scale1, self.scale2, self.scale3, self.scale4 = scale1, scale2, scale3, scale4
         rows = []
-        for i0 in range(system.natom):
+        for i0 in sorted(set(system.neighs1) | set(system.neighs2) | set(system.neighs3) | set(system.neighs4)):
             for scale, nb, neighs in ((scale1, 1, system.neighs1), (scale2, 2, system.neighs2),
                                       (scale3, 3, system.neighs3), (scale4, 4, system.neighs4)):
                 if scale < 1.0:

@@ -11,6 +11,14 @@ from .cell import Cell
 __all__ = ['System']
 
 
+class _Neighbours(dict):
+    """Neighbour sets by atom index; an atom without neighbours is absent and
+    reads as an empty set."""
+
+    def __missing__(self, key):
+        return frozenset()
+
+
 class System(object):
     def __init__(self, numbers, pos, ffatypes=None, ffatype_ids=None, bonds=None, rvecs=None,
                  charges=None, radii=None, masses=None, **kwargs):
@@ -76,33 +84,33 @@ class System(object):
         return cls(**kwargs)
 
     def _init_derived_bonds(self):
-        """yaff/system.py:199-234"""
-        n = self.natom
-        n1 = {i: set() for i in range(n)}
+        """yaff/system.py:199-234: the atoms 1, 2, 3 and 4 bonds away from each
+        atom (sets by atom; atoms without such neighbours give an empty set)."""
+        n1 = _Neighbours()
         for i0, i1 in (self.bonds if self.bonds is not None else []):
-            n1[int(i0)].add(int(i1))
-            n1[int(i1)].add(int(i0))
-        n2 = {i: set() for i in range(n)}
+            n1.setdefault(int(i0), set()).add(int(i1))
+            n1.setdefault(int(i1), set()).add(int(i0))
+        n2 = _Neighbours()
         for i0, s0 in n1.items():
             for i1 in s0:
                 for i2 in n1[i1]:
                     if i2 > i0 and i2 not in n1[i0]:
-                        n2[i0].add(i2)
-                        n2[i2].add(i0)
-        n3 = {i: set() for i in range(n)}
+                        n2.setdefault(i0, set()).add(i2)
+                        n2.setdefault(i2, set()).add(i0)
+        n3 = _Neighbours()
         for i0, s0 in n1.items():
             for i1 in s0:
                 for i3 in n2[i1]:
                     if i3 != i0 and i3 not in n1[i0] and i3 not in n2[i0]:
-                        n3[i0].add(i3)
-                        n3[i3].add(i0)
-        n4 = {i: set() for i in range(n)}
+                        n3.setdefault(i0, set()).add(i3)
+                        n3.setdefault(i3, set()).add(i0)
+        n4 = _Neighbours()
         for i0, s0 in n1.items():
             for i1 in s0:
                 for i4 in n3[i1]:
                     if i4 != i0 and i4 not in n1[i0] and i4 not in n2[i0] and i4 not in n3[i0]:
-                        n4[i0].add(i4)
-                        n4[i4].add(i0)
+                        n4.setdefault(i0, set()).add(i4)
+                        n4.setdefault(i4, set()).add(i0)
         self.neighs1, self.neighs2, self.neighs3, self.neighs4 = n1, n2, n3, n4
 
     def supercell(self, *reps):
