@@ -3,9 +3,11 @@
  *
  * Every entry point is asynchronous on the HIP stream it is given, takes
  * device pointers (allocated by the caller) plus sizes, allocates nothing and
- * never synchronises, except the two that say so (ynb_nlist_rows and
- * ynb_sys_plan, which is host-only).  Return value: 0 on success, a negative
- * YNB_E* code otherwise; ynb_last_error() gives the message.
+ * never synchronises, except those that say so: ynb_nlist_rows
+ * (synchronises) and the host-only planning functions (ynb_sys_plan,
+ * ynb_ewald_kvectors, ynb_row_blocks, ynb_ewald_records, ynb_ewald_work).
+ * Return value: 0 on success, a negative YNB_E* code otherwise;
+ * ynb_last_error() gives the message.
  *
  * Work is batched: a launch covers every system of a ynb_batch.  Atoms of
  * system s live at [sys.atom0, sys.atom0 + sys.natom) of the padded atom
@@ -29,6 +31,11 @@
  *                                         (batched: every system of a ynb_batch)
  *   ynb_verlet_*                       <- VerletIntegrator.propagate
  *                                         yaff/sampling/verlet.py:339-367
+ *   ynb_ewald_kvectors                 <- the k loop of compute_ewald_reci
+ *                                         yaff/pes/ewald.c:38-58 (host planning)
+ * No counterpart in the reference (batches and their tables):
+ *   ynb_row_blocks, ynb_ewald_records, ynb_ewald_work(_device), ynb_scatter,
+ *   ynb_total, ynb_pair_rows_static
  */
 #ifndef YAFF_NB_H
 #define YAFF_NB_H
@@ -273,7 +280,9 @@ int ynb_ewald_work_device(int32_t *work, int64_t nblk, int64_t nchunk, int32_t s
 
 /* Workspace of ynb_nlist_build in bytes, for bins of bin_cap atoms
  * (ynb_batch.max_bin_atoms).  The caller zero-initialises it once; the build
- * leaves its bin counters at zero. */
+ * leaves its bin counters at zero.  The counters are its first 4 * nbin
+ * bytes, so a batch whose npad or nbin changes keeps them; when nbin grows,
+ * the caller zeroes the counters of the new bins. */
 size_t ynb_nlist_workspace_bytes(int64_t npad, int32_t nbin, int32_t bin_cap);
 
 /* Workspace of ynb_nlist_rows for a system of natom atoms and nrows rows. */

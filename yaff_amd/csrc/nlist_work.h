@@ -29,8 +29,10 @@ inline Work carve(void *base, int64_t npad, int32_t nbin, int32_t cb) {
   };
   const size_t nb = (size_t)(nbin > 0 ? nbin : 1);
   const size_t slots = nb * (size_t)(cb > 0 ? cb : 1);
-  w.slot_idx = (int32_t *)take(4 * slots);
+  // the bin counters first: their place does not depend on npad, nbin or cb,
+  // so a batch whose extents change keeps them (zero between builds)
   w.bin_count = (int32_t *)take(4 * nb);
+  w.slot_idx = (int32_t *)take(4 * slots);
   w.xw = (double4 *)take(32 * npad);
   w.wr = (int4 *)take(16 * npad);
   w.sxw = (double4 *)take(32 * slots);

@@ -513,8 +513,11 @@ class SlotEngine(Engine):
         if cb != self.bin_cap:
             self.bin_cap = cb
             self._alloc_nlist_work()
-        elif npad != self.npad or nbin != self.nbin:
-            self.nl_work_d.zero_()
+        elif nbin > self.nbin:
+            # the workspace starts with one counter per bin (zero between
+            # builds); the counters of bins past the old extent lay in the
+            # region that followed them
+            self.nl_work_d[4*self.nbin:4*nbin].zero_()
         if npad != self.npad:
             # par is [MAX_POT][3][npad]: move it to the new stride
             m = min(npad, self.npad)
