@@ -42,7 +42,7 @@ __global__ void k_bin(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
 // does not depend on the order of the atomics in k_bin.
 __global__ __launch_bounds__(256) void k_bin_order(int32_t cb, int32_t *bin_count, const int32_t *slot_idx,
                                                    const double4 *xw, const int4 *wr, double4 *sxw, int4 *sw,
-                                                   int32_t *bin_start, int32_t *bin_end, const int32_t *bin_sys,
+                                                   int32_t *bin_end, const int32_t *bin_sys,
                                                    const int32_t *rebuild) {
   __shared__ int32_t l_idx[1024];
   const int b = blockIdx.x;
@@ -60,7 +60,6 @@ __global__ __launch_bounds__(256) void k_bin_order(int32_t cb, int32_t *bin_coun
     sw[o] = wr[g];
   }
   if (threadIdx.x == 0) {
-    bin_start[b] = b * cb;
     bin_end[b] = b * cb + n;
     bin_count[b] = 0;  // every thread has read the count (barrier above): ready for the next build
   }
@@ -107,8 +106,7 @@ template <bool EXACT, int NB>
 __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restrict__ sys,
                                                           const int32_t *__restrict__ bin_sys, int64_t npad,
                                                           const double *__restrict__ pos,
-                                                          const int32_t *__restrict__ bin_start,
-                                                          const int32_t *__restrict__ bin_end,
+                                                          const int32_t *__restrict__ bin_end, int32_t cb,
                                                           const double4 *__restrict__ sxw,
                                                           const int4 *__restrict__ sw, ynb_nlist nl) {
   __shared__ float l_x[CAP], l_y[CAP], l_z[CAP];
@@ -124,8 +122,10 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
   const int sid = bin_sys[b];
   if (sid < 0 || (nl.rebuild && !nl.rebuild[sid])) return;
   const ynb_sys &S = sys[sid];
-  const int start = bin_start[b], end = bin_end[b];
+  // bin b's slots start at b * cb (k_bin_order); only its end is read
+  const int start = b * cb, end = bin_end[b];
   if (start + (int)blockIdx.y * BW >= end) return;
+
   const int lb = b - S.bin0;
   const int c2 = lb % S.grid[2], c1 = (lb / S.grid[2]) % S.grid[1], c0 = lb / (S.grid[1] * S.grid[2]);
   const int R0 = S.range[0], R1 = S.range[1], R2 = S.range[2];
@@ -136,30 +136,32 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
   const double X1 = f0 * S.rvecs[1] + f1 * S.rvecs[4] + f2 * S.rvecs[7];
   const double X2 = f0 * S.rvecs[2] + f1 * S.rvecs[5] + f2 * S.rvecs[8];
 
-  // Partner-bin table and the exclusive prefix of the partner-bin sizes
+  // Partner-bin table: one thread per partner bin, all bins at once (one
+  // round trip for their sizes), then the exclusive prefix of the sizes
   // (64 bins per step in wave 0).
+  for (int k = tid; k < nnb; k += BW * 64) {
+    const int i2 = k % n2, i1 = (k / n2) % n1, i0 = k / (n1 * n2);
+    int nb0 = c0 + i0 - R0, nb1 = c1 + i1 - R1, nb2 = c2 + i2 - R2;
+    const int s0 = floor_div(nb0, S.grid[0]), s1 = floor_div(nb1, S.grid[1]), s2 = floor_div(nb2, S.grid[2]);
+    nb0 -= s0 * S.grid[0];
+    nb1 -= s1 * S.grid[1];
+    nb2 -= s2 * S.grid[2];
+    const int nbin = S.bin0 + (nb0 * S.grid[1] + nb1) * S.grid[2] + nb2;
+    const int bs = nbin * cb;
+    l_pre[k + 1] = bin_end[nbin] - bs;  // the size, made a prefix below
+    l_bs[k] = bs;
+    l_sh[0][k] = s0 * S.rvecs[0] + s1 * S.rvecs[3] + s2 * S.rvecs[6] - X0;
+    l_sh[1][k] = s0 * S.rvecs[1] + s1 * S.rvecs[4] + s2 * S.rvecs[7] - X1;
+    l_sh[2][k] = s0 * S.rvecs[2] + s1 * S.rvecs[5] + s2 * S.rvecs[8] - X2;
+    const bool far = s0 < -511 || s0 > 511 || s1 < -511 || s1 > 511 || s2 < -511 || s2 > 511;
+    l_s[k] = far ? INT32_MIN : pack_img(s0, s1, s2);
+  }
+  __syncthreads();
   if (wv == 0) {
     int carry = 0;
     for (int k0 = 0; k0 < nnb; k0 += 64) {
       const int k = k0 + lane;
-      int c = 0;
-      if (k < nnb) {
-        const int i2 = k % n2, i1 = (k / n2) % n1, i0 = k / (n1 * n2);
-        int nb0 = c0 + i0 - R0, nb1 = c1 + i1 - R1, nb2 = c2 + i2 - R2;
-        const int s0 = floor_div(nb0, S.grid[0]), s1 = floor_div(nb1, S.grid[1]), s2 = floor_div(nb2, S.grid[2]);
-        nb0 -= s0 * S.grid[0];
-        nb1 -= s1 * S.grid[1];
-        nb2 -= s2 * S.grid[2];
-        const int nbin = S.bin0 + (nb0 * S.grid[1] + nb1) * S.grid[2] + nb2;
-        const int bs = bin_start[nbin];
-        c = bin_end[nbin] - bs;
-        l_bs[k] = bs;
-        l_sh[0][k] = s0 * S.rvecs[0] + s1 * S.rvecs[3] + s2 * S.rvecs[6] - X0;
-        l_sh[1][k] = s0 * S.rvecs[1] + s1 * S.rvecs[4] + s2 * S.rvecs[7] - X1;
-        l_sh[2][k] = s0 * S.rvecs[2] + s1 * S.rvecs[5] + s2 * S.rvecs[8] - X2;
-        const bool far = s0 < -511 || s0 > 511 || s1 < -511 || s1 > 511 || s2 < -511 || s2 > 511;
-        l_s[k] = far ? INT32_MIN : pack_img(s0, s1, s2);
-      }
+      const int c = k < nnb ? l_pre[k + 1] : 0;  // each lane reads its own entry before writing it
       int incl = c;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
@@ -715,10 +717,10 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
       YNB_LAUNCH("nlist_bin", k_bin, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, cb, w.bin_count,
                  w.slot_idx, w.xw, w.wr, nl->status, nl->rebuild);
     YNB_LAUNCH("nlist_bin_order", k_bin_order, dim3(b->nbin), dim3(256), 0, st, cb, w.bin_count, w.slot_idx, w.xw,
-               w.wr, w.sxw, w.sw, w.bin_start, w.bin_end, b->bin_sys, nl->rebuild);
+               w.wr, w.sxw, w.sw, w.bin_end, b->bin_sys, nl->rebuild);
 #define YNB_BUILD(EX, NBB)                                                                                    \
   YNB_LAUNCH("nlist_build_binned", (k_build_binned<EX, NBB>), dim3(b->nbin, ychunks), dim3(BW * 64), 0, st,        \
-             b->sys, b->bin_sys, npad, pos, w.bin_start, w.bin_end, w.sxw, w.sw, *nl)
+             b->sys, b->bin_sys, npad, pos, w.bin_end, cb, w.sxw, w.sw, *nl)
     // Bins hold a few atoms on average; the block's wavefronts loop over the
     // atoms beyond BW.  Large batches take one block per bin (a block per BW
     // atoms of the largest bin would mostly launch empty blocks); small ones

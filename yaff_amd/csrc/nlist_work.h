@@ -12,7 +12,7 @@ struct Work {
   int32_t *bin_count;  // [nbin]
   double4 *xw, *sxw;   // wrapped coordinates: per atom, and per bin slot
   int4 *wr, *sw;       // wrap counts (+ atom index in .w)
-  int32_t *bin_start, *bin_end;
+  int32_t *bin_end;    // [nbin] end of each bin's slots (they start at bin * cb)
   size_t total;
 };
 
@@ -37,7 +37,6 @@ inline Work carve(void *base, int64_t npad, int32_t nbin, int32_t cb) {
   w.wr = (int4 *)take(16 * npad);
   w.sxw = (double4 *)take(32 * slots);
   w.sw = (int4 *)take(16 * slots);
-  w.bin_start = (int32_t *)take(4 * nb);
   w.bin_end = (int32_t *)take(4 * nb);
   w.total = off;
   return w;
