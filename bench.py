@@ -110,12 +110,15 @@ def run_steps(eng, K):
 
 def timed(eng, K, profile=True):
     """K eager MD steps issued as the graphs of timed_graph hold them (K/S
-    sequences of S joined steps); per-kernel HIP events when profile."""
+    sequences of S joined steps); per-kernel HIP events when profile, with
+    the Ewald chain run after the pair pass on one stream then, so that each
+    kernel's duration is its own (the graphs overlap the two chains)."""
     import torch
     from yaff_amd import _lib
     torch.cuda.synchronize()
     if profile:
         _lib.profile_enable(True)
+        eng.overlap = False   # kernel durations measured with each kernel alone on the chip
     t0 = time.perf_counter()
     S = graph_steps(K)
     for _ in range(K//S):
@@ -125,6 +128,7 @@ def timed(eng, K, profile=True):
     prof = _lib.profile_read() if profile else []
     if profile:
         _lib.profile_enable(False)
+        eng.overlap = True
     return t1 - t0, prof
 
 
@@ -483,12 +487,15 @@ def batched(nsys, steps, warmup):
         nst = steps if prec == 'mixed' else max(steps//4, 1)
         md.capture(graph_steps(nst))
         dt = md.run(nst)
-        # per-kernel durations of a few eager steps of the same batch
+        # per-kernel durations of a few eager steps of the same batch (one
+        # stream: each kernel alone on the chip)
         from yaff_amd import _lib
         torch.cuda.synchronize()
         _lib.profile_enable(True)
+        md.engine.overlap = False
         md.engine.md_steps(3)
         kt = kernel_table(_lib.profile_read(), 3)
+        md.engine.overlap = True
         _lib.profile_enable(False)
         progress('batched %s %.3g atom-evals/s' % (prec, natom*nst/dt))
         out['batched_' + prec] = {'value': natom*nst/dt, 'ms_per_step': dt/nst*1e3, 'steps': nst,
@@ -506,8 +513,10 @@ def batched(nsys, steps, warmup):
     from yaff_amd import _lib
     torch.cuda.synchronize()
     _lib.profile_enable(True)
+    md.engine.overlap = False
     md.engine.md_steps(10)
     kt = kernel_table(_lib.profile_read(), 10)
+    md.engine.overlap = True
     _lib.profile_enable(False)
     out['batched_mixed_skin1A'] = {'value': natom*steps/dt, 'ms_per_step': dt/steps*1e3, 'steps': steps,
                                    'skin_angstrom': 1.0,

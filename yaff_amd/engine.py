@@ -389,6 +389,9 @@ def _ceil_div(a, b):
 class Engine(object):
     """Device state and launches for a batch of NBSystem objects."""
 
+    overlap = True   # MD steps run the Ewald chain on a second stream (see _md_forces)
+    _side = None
+
     def __init__(self, systems, precision='fp64', apl=4, skb=None, fkb=None, fapl=4, kcap=None, device='cuda',
                  pair_lanes=32, upload=True):
         self.lib = _lib.load()
@@ -906,6 +909,19 @@ class Engine(object):
                                              self.nlist if prebin else None, self.stream()))
 
     def _md_forces(self):
+        # The Ewald chain (structure factors, k reduction, forces) needs only
+        # the positions and the set-up the drift made, not the list or the
+        # pair pass: with `overlap` it runs on a second stream (forked after
+        # the drift, joined before the correction and the kick; recorded as
+        # a fork and a join in a captured graph), so the two chains share the
+        # chip.  Their buffers are disjoint; results are the same bits.
+        overlap = self.has_ewald and self.overlap
+        if overlap:
+            main = torch.cuda.current_stream()
+            side = self._side_stream()
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
+                self._md_reci()
         if self._md_prebin():
             self.nlist.flags = _lib.NLIST_PREBINNED
             try:
@@ -915,23 +931,36 @@ class Engine(object):
         else:
             self.build_nlist(force=False)
         self.pair(gpos=True, accumulate=False)
-        # the Ewald partial gradients stay in their slots and are summed by
-        # the kick, which leaves the total gradient in gpos_d ...
         if self.has_ewald:
-            ew = self.ewald
-            ew.flags = _lib.EWALD_PREPARED
-            try:
-                self.ewald_reci(gpos=False)
-            finally:
-                ew.flags = 0
-            # the correction adds its gradient to gpos_d and its energy/virial
-            # to ewald_out_d (after the reciprocal pass has written it)
+            if overlap:
+                main.wait_stream(side)
+            else:
+                self._md_reci()
+            # the correction adds its gradient to gpos_d (after the pair pass
+            # has written it) and its energy/virial to ewald_out_d (after the
+            # reciprocal pass has written it)
             self.ewald_corr()
 
+    def _md_reci(self):
+        # the Ewald partial gradients stay in their slots and are summed by
+        # the kick, which leaves the total gradient in gpos_d
+        ew = self.ewald
+        ew.flags = _lib.EWALD_PREPARED
+        try:
+            self.ewald_reci(gpos=False)
+        finally:
+            ew.flags = 0
+
+    def _side_stream(self):
+        """The second stream of the MD step (default priority), made once."""
+        if getattr(self, '_side', None) is None:
+            self._side = torch.cuda.Stream(device=self.device)
+        return self._side
+
     def _md_kick(self, join=False):
-        # ... and the kick also totals energy and virial (in place of
-        # total()); with join, the next step's kick and drift follow in the
-        # same kernel
+        # the kick sums the pair gradient and the Ewald slots and totals the
+        # energy and virial (in place of total()); with join, the next step's
+        # kick and drift follow in the same kernel
         common = (self.vel_d.data_ptr(), self.gpos_d.data_ptr(),
                   self.ew_gpart_d.data_ptr() if self.has_ewald else None,
                   self.minv_d.data_ptr(), self.masses_d.data_ptr(), self.dt_d.data_ptr(),
