@@ -59,6 +59,9 @@ def parse():
     ap.add_argument('--batch-systems', type=int, default=64)
     ap.add_argument('--no-served', action='store_true')
     ap.add_argument('--no-config0', action='store_true', help='skip the 864-atom LJ fluid line')
+    ap.add_argument('--no-overlap', action='store_true',
+                    help='MD steps on one stream (the Ewald chain after the pair pass): kernel traces whose averages '
+                         'are each kernel alone on the chip')
     ap.add_argument('--no-cpu-aggregate', action='store_true',
                     help='skip the all-core oracle runs on the batched and served workloads')
     ap.add_argument('--no-precision', action='store_true',
@@ -128,7 +131,7 @@ def timed(eng, K, profile=True):
     prof = _lib.profile_read() if profile else []
     if profile:
         _lib.profile_enable(False)
-        eng.overlap = True
+        del eng.overlap   # back to the class setting
     return t1 - t0, prof
 
 
@@ -495,7 +498,7 @@ def batched(nsys, steps, warmup):
         md.engine.overlap = False
         md.engine.md_steps(3)
         kt = kernel_table(_lib.profile_read(), 3)
-        md.engine.overlap = True
+        del md.engine.overlap
         _lib.profile_enable(False)
         progress('batched %s %.3g atom-evals/s' % (prec, natom*nst/dt))
         out['batched_' + prec] = {'value': natom*nst/dt, 'ms_per_step': dt/nst*1e3, 'steps': nst,
@@ -516,7 +519,7 @@ def batched(nsys, steps, warmup):
     md.engine.overlap = False
     md.engine.md_steps(10)
     kt = kernel_table(_lib.profile_read(), 10)
-    md.engine.overlap = True
+    del md.engine.overlap
     _lib.profile_enable(False)
     out['batched_mixed_skin1A'] = {'value': natom*steps/dt, 'ms_per_step': dt/steps*1e3, 'steps': steps,
                                    'skin_angstrom': 1.0,
@@ -580,6 +583,8 @@ def main():
     from yaff_amd.engine import Engine
     from yaff_amd.units import femtosecond
     K, W = args.steps, args.warmup
+    if args.no_overlap:
+        Engine.overlap = False
 
     s, nb = rocksalt_system()
     n = s.natom
