@@ -13,6 +13,8 @@
 //   a block of whole rows; per k a lane sums its atoms, and every 8 k the
 //   wavefront transposes its per-lane sums through LDS so that each k's sum
 //   over the chunk is one fixed-order tree.  Partial S(k) per (chunk, k).
+//   Mixed precision (k_ewald_sk_mixed): the terms in FP32 between FP64
+//   phase anchors every 8 k of a row, the sums from the LDS tile on in FP64.
 // Pass 2 (k_ewald_kreduce): one thread per k sums its chunks in a fixed
 //   order per group (atoms below nlow, between, above nhigh), forms the
 //   energy and virial terms of ewald.c:82-127 and the coefficients 2 c S(k).
@@ -81,7 +83,8 @@ __device__ inline void sincos_turns(double t, double *s, double *c) {
 
 // Seed of a recurrence: sin and cos of 2 pi t for a reduced phase t in
 // [-1/2, 1/2] turns.  Mixed mode evaluates it in FP32 on the FP64-reduced
-// argument; the recurrences and all sums stay in FP64 in both modes.
+// argument; the force pass's recurrences and sums stay FP64 in both modes
+// (the mixed structure-factor pass is k_ewald_sk_mixed).
 template <int PREC>
 __device__ inline void seed_sincos(double t, double *s, double *c) {
   if (PREC == YNB_FP64) {
@@ -128,6 +131,39 @@ __device__ inline void continue_seed(double &zr, double &zi, double c1, double s
   }
 }
 
+// LDS tile of the structure-factor passes: T = 8 k rows of RS = 66 entries
+// (one per lane), padded so that both the row writes and the strided reads
+// are free of bank conflicts.
+constexpr int SK_T = 8, SK_RS = 66;
+
+// Sums the m filled rows of the tile: lane (part p, row r) adds entries p,
+// p + 8, ..., p + 56 of row r, three xor-shuffle levels combine the 8 parts,
+// and part 0 writes S(k) of row r to out[2 r], out[2 r + 1].  Fixed order.
+__device__ inline void sk_flush_tile(double2 *l_t, int m, double *out) {
+  const int lane = threadIdx.x;
+  const int m_r = lane & (SK_T - 1), part = lane >> 3;
+  __syncthreads();
+  double ar = 0.0, ai = 0.0;
+  if (m_r < m) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double2 v = l_t[m_r * SK_RS + part + 8 * u];
+      ar += v.x;
+      ai += v.y;
+    }
+  }
+#pragma unroll
+  for (int off = 8; off < 64; off <<= 1) {
+    ar += __shfl_xor(ar, off, 64);
+    ai += __shfl_xor(ai, off, 64);
+  }
+  if (part == 0 && m_r < m) {
+    out[2 * m_r] = ar;
+    out[2 * m_r + 1] = ai;
+  }
+  __syncthreads();
+}
+
 // Structure factors.  One wavefront per work item (row block, atom chunk):
 // lane l holds atoms l, l + 64, ... of the chunk (P per lane).  Per k the
 // lane adds its atoms' q exp(i k.r) and stores the sum in row m of an LDS
@@ -145,10 +181,9 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
                                                  const double *__restrict__ frac, const double *__restrict__ step,
                                                  double *__restrict__ spart, int nwork, int64_t npad) {
 #pragma clang fp contract(fast)
-  constexpr int T = 8, RS = 66;
+  constexpr int T = SK_T, RS = SK_RS;
   __shared__ double2 l_t[T * RS];
   const int lane = threadIdx.x;
-  const int m_r = lane & (T - 1), part = lane >> 3;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const int4 wk = *(const int4 *)(work + 4 * w);
     if (wk.x < 0) continue;  // an item of no system (the same for the wavefront)
@@ -207,26 +242,119 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
         j += n;
         m += n;
         if (m == T || (r + 1 == re && j == row.w)) {  // flush the tile
-          __syncthreads();
-          double ar = 0.0, ai = 0.0;
-          if (m_r < m) {
+          sk_flush_tile(l_t, m, out + 2 * kt);
+          kt += m;
+          m = 0;
+        }
+      }
+    }
+  }
+}
+
+// Structure factors in mixed precision: the terms q exp(i k.r) in FP32,
+// their phases anchored in FP64.  Along a row, every 8th term (j = 0, 8, 16,
+// ... from the row start) is taken from an FP64 anchor: at the row start the
+// anchor is q exp(2 pi i t) of the FP64-reduced phase t (FP32 sine and cosine
+// of t), then each anchor is the previous one times exp(2 pi i 8 f0) in FP64
+// (exp(2 pi i f0) squared three times per atom).  Between anchors a term
+// steps in FP32 by exp(2 pi i f0): at most 7 FP32 complex products, so the
+// phase error of a term stays below ~5e-7 rad instead of growing along the
+// whole row.  A lane sums its atoms' terms in FP32; from the LDS tile on
+// every sum is FP64, in the same fixed order as the FP64 pass.
+template <int P>
+__global__ __launch_bounds__(64) void k_ewald_sk_mixed(const ynb_sys *__restrict__ sys,
+                                                       const int32_t *__restrict__ work,
+                                                       const int32_t *__restrict__ chunk,
+                                                       const int32_t *__restrict__ skblk,
+                                                       const int32_t *__restrict__ krow,
+                                                       const int32_t *__restrict__ krow_k,
+                                                       const double *__restrict__ frac,
+                                                       const double *__restrict__ step,
+                                                       double *__restrict__ spart, int nwork, int64_t npad) {
+#pragma clang fp contract(fast)
+  constexpr int T = SK_T, RS = SK_RS, SEG = 8;
+  __shared__ double2 l_t[T * RS];
+  const int lane = threadIdx.x;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int4 wk = *(const int4 *)(work + 4 * w);
+    if (wk.x < 0) continue;  // an item of no system (the same for the wavefront)
+    const ynb_sys &S = sys[wk.x];
+    const int4 ch = *(const int4 *)(chunk + 4 * wk.z);
+    const int rb = skblk[2 * wk.y], re = skblk[2 * wk.y + 1];
+    double f0[P], f1[P], f2[P], q[P], ar[P], ai[P], w8r[P], w8i[P];
+    float cr[P], ci[P], zr[P], zi[P];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              const double2 v = l_t[m_r * RS + part + 8 * u];
-              ar += v.x;
-              ai += v.y;
-            }
-          }
+    for (int p = 0; p < P; ++p) {
+      const int la = lane + 64 * p;
+      const bool valid = la < ch.z;
+      const int64_t g = S.atom0 + ch.y + (valid ? la : 0);
+      f0[p] = frac[g];
+      f1[p] = frac[npad + g];
+      f2[p] = frac[2 * npad + g];
+      q[p] = valid ? frac[3 * npad + g] : 0.0;
+      double c = step[g], sn = step[npad + g];  // exp(2 pi i f0)
+      cr[p] = (float)c;
+      ci[p] = (float)sn;
 #pragma unroll
-          for (int off = 8; off < 64; off <<= 1) {
-            ar += __shfl_xor(ar, off, 64);
-            ai += __shfl_xor(ai, off, 64);
+      for (int e = 0; e < 3; ++e) {  // exp(2 pi i 8 f0)
+        const double nc = c * c - sn * sn;
+        sn = 2.0 * c * sn;
+        c = nc;
+      }
+      w8r[p] = c;
+      w8i[p] = sn;
+      ar[p] = ai[p] = 0.0;
+      zr[p] = zi[p] = 0.0f;
+    }
+    double *out = spart + 2 * (S.spart0 + (int64_t)(wk.z - S.chunk0) * S.nk - S.k0);
+    int kt = krow_k[rb];  // first k of the current tile
+    int m = 0;            // k stored in the tile
+    int4 rown = *(const int4 *)(krow + 4 * rb);
+    for (int r = rb; r < re; ++r) {
+      const int4 row = rown;
+      if (r + 1 < re) rown = *(const int4 *)(krow + 4 * (r + 1));
+#pragma unroll
+      for (int p = 0; p < P; ++p) {  // row start: the anchor q exp(2 pi i g.f)
+        float sf, cf;
+        sincospif(2.0f * (float)reduced_phase(row.x, row.y, row.z, f0[p], f1[p], f2[p]), &sf, &cf);
+        ar[p] = q[p] * (double)cf;
+        ai[p] = q[p] * (double)sf;
+        zr[p] = (float)ar[p];
+        zi[p] = (float)ai[p];
+      }
+      for (int j = 0; j < row.w;) {
+        // k up to the end of the row, of the tile or of the anchor's segment
+        const int n = min(min(row.w - j, T - m), SEG - (j & (SEG - 1)));
+#pragma unroll 2
+        for (int i = 0; i < n; ++i) {
+          float sr = zr[0], si = zi[0];
+#pragma unroll
+          for (int p = 1; p < P; ++p) {
+            sr += zr[p];
+            si += zi[p];
           }
-          if (part == 0 && m_r < m) {
-            out[2 * (kt + m_r)] = ar;
-            out[2 * (kt + m_r) + 1] = ai;
+          l_t[(m + i) * RS + lane] = make_double2((double)sr, (double)si);
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            const float nr = zr[p] * cr[p] - zi[p] * ci[p];
+            zi[p] = zr[p] * ci[p] + zi[p] * cr[p];
+            zr[p] = nr;
           }
-          __syncthreads();
+        }
+        j += n;
+        m += n;
+        if ((j & (SEG - 1)) == 0 && j < row.w) {  // next anchor: 8 steps in FP64
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            const double nr = ar[p] * w8r[p] - ai[p] * w8i[p];
+            ai[p] = ar[p] * w8i[p] + ai[p] * w8r[p];
+            ar[p] = nr;
+            zr[p] = (float)ar[p];
+            zi[p] = (float)ai[p];
+          }
+        }
+        if (m == T || (r + 1 == re && j == row.w)) {  // flush the tile
+          sk_flush_tile(l_t, m, out + 2 * kt);
           kt += m;
           m = 0;
         }
@@ -773,15 +901,19 @@ extern "C" int ynb_ewald_reci(const ynb_batch *b, const ynb_ewald *ew, const dou
 #define YNB_SK(PR, PP)                                                                                        \
   YNB_LAUNCH("ewald_sk", (k_ewald_sk<PR, PP>), grid, dim3(64), 0, st, b->sys, ew->work, ew->chunk, ew->skblk,  \
              ew->krow, ew->krow_k, ew->frac, ew->step, ew->spart, ew->nwork, npad)
+#define YNB_SKM(PP)                                                                                           \
+  YNB_LAUNCH("ewald_sk", (k_ewald_sk_mixed<PP>), grid, dim3(64), 0, st, b->sys, ew->work, ew->chunk, ew->skblk, \
+             ew->krow, ew->krow_k, ew->frac, ew->step, ew->spart, ew->nwork, npad)
     if (precision == YNB_FP64) {
       if (ew->apl == 2) YNB_SK(YNB_FP64, 2);
       else if (ew->apl == 4) YNB_SK(YNB_FP64, 4);
       else YNB_SK(YNB_FP64, 8);
     } else {
-      if (ew->apl == 2) YNB_SK(YNB_MIXED, 2);
-      else if (ew->apl == 4) YNB_SK(YNB_MIXED, 4);
-      else YNB_SK(YNB_MIXED, 8);
+      if (ew->apl == 2) YNB_SKM(2);
+      else if (ew->apl == 4) YNB_SKM(4);
+      else YNB_SKM(8);
     }
+#undef YNB_SKM
 #undef YNB_SK
     YNB_LAUNCHED();
   }
