@@ -149,8 +149,10 @@ typedef struct ynb_batch {
 #define YNB_BATCH_FORM_GENERIC 4 /* some system has pair_form YNB_FORM_GENERIC */
 #define YNB_BATCH_FORM_LJ 8      /* some system has pair_form YNB_FORM_LJ */
 #define YNB_BATCH_FORM_IONIC 16  /* some system has pair_form YNB_FORM_IONIC */
-#define YNB_BATCH_WIDE_SEARCH 32 /* some binned system searches more than 125 partner bins
-                                    ((2 range0 + 1)(2 range1 + 1)(2 range2 + 1) > 125) */
+#define YNB_BATCH_WIDE_SEARCH 32 /* some binned system stages more than YNB_BUILD_NB_SMALL partner
+                                    bins per home group (ynb_build_search_bins) */
+#define YNB_BUILD_NB_SMALL 160   /* partner bins of the build's small tables (two 5x5x5 search boxes
+                                    one bin apart: 150) */
 
 /* Full neighbour list, one row of kcap entries per atom: entry k of atom i is
  * at i*kcap + k.
@@ -160,7 +162,8 @@ typedef struct ynb_nlist {
   int32_t *j;
   int32_t *img;
   int32_t *cnt;        /* [npad] entries per atom (also counted past kcap) */
-  int32_t *status;     /* [4]: [0] flags: 1 row > kcap, 2 image shift > 511 cells, 4 bin > max_bin_atoms;
+  int32_t *status;     /* [4]: [0] flags: 1 row > kcap, 2 image shift > 511 cells, 4 bin > max_bin_atoms,
+                          8 partner bins > the tables (YNB_BATCH_WIDE_SEARCH missing);
                           [1] after flag 4, the fullest bin's atom count.
                           Flags are sticky (builds OR into them, the caller clears them), so
                           a check after many graph-replayed steps sees every overflow.  On a
@@ -244,6 +247,13 @@ int ynb_profile_read(int i, const char **name, float *ms);
  * sys->rc and sys->natom.  Systems with nvec < 3 use the all-pairs path
  * (nbin = 0). */
 int ynb_sys_plan(ynb_sys *sys);
+
+/* Host only.  Partner bins the cell-binned build stages per home group of a
+ * planned system (a group: up to two bins adjacent along the third cell
+ * vector whose search boxes are staged together); a batch holding a system
+ * above YNB_BUILD_NB_SMALL needs YNB_BATCH_WIDE_SEARCH.  0 for a system
+ * without bins. */
+int ynb_build_search_bins(const ynb_sys *sys);
 
 /* Host only.  The k vectors of compute_ewald_reci (yaff/pes/ewald.c:38-58)
  * for reciprocal rows gvecs [3][3] (no 2 pi), cell volume, alpha, gcut and

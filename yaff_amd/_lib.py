@@ -18,6 +18,7 @@ FP64, MIXED = 0, 1
 BATCH_ALLPAIRS, BATCH_EXACT_BUILD = 1, 2
 BATCH_FORM_GENERIC, BATCH_FORM_LJ, BATCH_FORM_IONIC = 4, 8, 16
 BATCH_WIDE_SEARCH = 32
+BUILD_NB_SMALL = 160   # YNB_BUILD_NB_SMALL
 NLIST_PREBINNED = 1
 EWALD_PREPARED = 1
 FORM_GENERIC, FORM_LJ, FORM_IONIC = 0, 1, 2
@@ -90,7 +91,7 @@ class YnbCorr(C.Structure):
                 ('nrow_tot', C.c_int32), ('nblk_tot', C.c_int32)]
 
 
-EXPORTS = ['ynb_version', 'ynb_last_error', 'ynb_sys_plan', 'ynb_nlist_workspace_bytes',
+EXPORTS = ['ynb_version', 'ynb_last_error', 'ynb_sys_plan', 'ynb_build_search_bins', 'ynb_nlist_workspace_bytes',
            'ynb_rows_workspace_bytes', 'ynb_nlist_build', 'ynb_nlist_rows', 'ynb_nlist_recompute',
            'ynb_pair_compute', 'ynb_pair_rows', 'ynb_pair_rows_static', 'ynb_ewald_reci', 'ynb_ewald_corr', 'ynb_verlet_kick_drift', 'ynb_verlet_kick',
            'ynb_verlet_kick_kick_drift',
@@ -118,6 +119,7 @@ def load(path=LIB_PATH):
     L.ynb_version.restype = C.c_char_p
     L.ynb_last_error.restype = C.c_char_p
     L.ynb_sys_plan.argtypes = [P(YnbSys)]
+    L.ynb_build_search_bins.argtypes = [P(YnbSys)]
     L.ynb_ewald_kvectors.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp, i64, vp, vp, vp, vp, P(i64),
                                      P(i64)]
     L.ynb_row_blocks.argtypes = [vp, i64, i32, vp, P(i64)]

@@ -125,6 +125,21 @@ __device__ inline bool yaff_row_vector(const double *pA, const double *pB, bool 
   return inside;
 }
 
+// Home bins per block of the cell-binned build: runs of 2 bins along the
+// first cell vector when the union of their search boxes (range[0..2] bins
+// on each side) fits YNB_BUILD_NB_SMALL partner bins, else single bins.  A
+// function of the system alone.
+__host__ __device__ inline int build_group_len(const int32_t range[3]) {
+  const int n12 = (2 * range[1] + 1) * (2 * range[2] + 1), w0 = 2 * range[0] + 1;
+  return n12 * (w0 + 1) <= YNB_BUILD_NB_SMALL ? 2 : 1;
+}
+
+// Partner bins the build stages for one home group of the system.
+__host__ __device__ inline int build_search_bins(const int32_t range[3]) {
+  const int n12 = (2 * range[1] + 1) * (2 * range[2] + 1), w0 = 2 * range[0] + 1;
+  return n12 * (w0 + build_group_len(range) - 1);
+}
+
 __device__ inline bool nlow_nhigh_ok(int a, int b, int nlow, int nhigh) {
   return (a >= nlow && b < nhigh) || (b >= nlow && a < nhigh);  // nlist.c:57
 }

@@ -40,7 +40,7 @@ __global__ void k_bin(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
 // One block per bin: orders the bin's atoms by index (rank among the bin's
 // atoms) and writes their wrapped data to the bin's slots, so the layout
 // does not depend on the order of the atomics in k_bin.
-__global__ __launch_bounds__(256) void k_bin_order(int32_t cb, int32_t *bin_count, const int32_t *slot_idx,
+__global__ __launch_bounds__(64) void k_bin_order(int32_t cb, int32_t *bin_count, const int32_t *slot_idx,
                                                    const double4 *xw, const int4 *wr, double4 *sxw, int4 *sw,
                                                    int32_t *bin_end, const int32_t *bin_sys,
                                                    const int32_t *rebuild) {
@@ -74,23 +74,30 @@ __device__ inline void store_entry(const ynb_nlist &nl, int64_t npad, int64_t gi
 }
 
 constexpr int BW = 8;         // atoms (wavefronts) per block of the binned build
-constexpr int CAP = 1024;     // partner candidates staged in LDS at once
-constexpr int MAXNB = 512;    // partner bins per home bin (ynb_sys_plan keeps (2R+1)^3 below)
-constexpr int NB_SMALL = 128; // partner-bin tables of the common search box (5 x 5 x 5 bins)
+constexpr int CAP = 1280;     // partner candidates staged in LDS at once
+constexpr int MAXNB = 512;    // partner bins per home group (ynb_sys_plan keeps (2R+1)^3 below)
+constexpr int NB_SMALL = YNB_BUILD_NB_SMALL;  // partner-bin tables of the common search boxes
 
 // Lanes of the wave below this one that hold a set bit of mask.
 __device__ inline int lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
 }
 
-// Block (bin b, part y of its atoms), one atom per wavefront, the atoms
-// BW gridDim.y apart.  All
-// partner bins of the home bin are staged into LDS at once (CAP candidates
-// per stage) as FP32 coordinates relative to the home-bin centre with the
-// image shift applied.  The wave scans 64 candidates per step; accepted
-// candidates are appended to the atom's row with a ballot (stream
+// Block (home group of bin b, part y of its atoms), one atom per wavefront,
+// the atoms BW gridDim.y apart.  A home group is a run of up to
+// build_group_len(S) bins consecutive along the first cell vector (the run
+// starts at a first-axis bin index divisible by the length), so
+// neighbouring home bins share most partner bins: the union of their search
+// boxes is staged once.  The blocks of the other bins of a run exit at
+// once.  Partner bins are staged in (first, second, third) axis order, the
+// first axis slowest, so the search box of home bin h of the run is one
+// contiguous range of the staged candidates, in the order a single home
+// bin would stage them; as FP32 coordinates relative to the run's centre
+// with the image shift applied.  The wave scans 64 candidates per step;
+// accepted candidates are appended to the atom's row with a ballot (stream
 // compaction), so rows are written coalesced and in a fixed order (partner
-// bins in offset order, partner atoms by index).
+// bins in offset order, partner atoms by index) that depends on the system
+// alone.
 //
 // Acceptance.  Relative coordinates stay below ~150 bohr, so the FP32
 // distance squared is within 1e-5 relative of the exact one; below a band of
@@ -113,7 +120,7 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
   __shared__ int32_t l_m[CAP], l_g[CAP];
   __shared__ int32_t l_pre[NB + 1];
   __shared__ int32_t l_bs[NB];       // slot start of each partner bin
-  __shared__ double l_sh[3][NB];     // image shift of each partner bin minus the home-bin centre
+  __shared__ double l_sh[3][NB];     // image shift of each partner bin minus the run's centre
   __shared__ int32_t l_s[NB];        // packed shift (s0, s1, s2)
   __shared__ int16_t l_bin[CAP];     // partner bin of each staged candidate
   const int b = blockIdx.x;
@@ -122,25 +129,40 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
   const int sid = bin_sys[b];
   if (sid < 0 || (nl.rebuild && !nl.rebuild[sid])) return;
   const ynb_sys &S = sys[sid];
-  // bin b's slots start at b * cb (k_bin_order); only its end is read
-  const int start = b * cb, end = bin_end[b];
-  if (start + (int)blockIdx.y * BW >= end) return;
-
   const int lb = b - S.bin0;
   const int c2 = lb % S.grid[2], c1 = (lb / S.grid[2]) % S.grid[1], c0 = lb / (S.grid[1] * S.grid[2]);
   const int R0 = S.range[0], R1 = S.range[1], R2 = S.range[2];
-  const int n1 = 2 * R1 + 1, n2 = 2 * R2 + 1;
-  const int nnb = (2 * R0 + 1) * n1 * n2;
-  const double f0 = (c0 + 0.5) / S.grid[0], f1 = (c1 + 0.5) / S.grid[1], f2 = (c2 + 0.5) / S.grid[2];
+  const int LG = build_group_len(S.range);
+  if (c0 % LG != 0) return;  // built by the run's first bin
+  const int nh = min(LG, S.grid[0] - c0);  // home bins b + h hstride, h < nh
+  const int hstride = S.grid[1] * S.grid[2];
+  // atoms of the run: home bin h holds n_h of them from slot (b + h hstride) cb
+  int hend[2];  // prefix of the home bins' atom counts (uniform)
+  int natoms = 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (h < nh) natoms += bin_end[b + h * hstride] - (b + h * hstride) * cb;
+    hend[h] = natoms;
+  }
+  if ((int)blockIdx.y * BW >= natoms) return;
+
+  const int n1 = 2 * R1 + 1, n2 = 2 * R2 + 1, n12 = n1 * n2;
+  const int w0 = 2 * R0 + 1;             // first-axis slices searched per home bin
+  const int nnb = n12 * (w0 + nh - 1);   // partner bins of the run
+  if (nnb > NB) {  // a batch flag that does not match its systems (ynb_build_search_bins)
+    if (tid == 0) atomicOr(nl.status, 8);
+    return;
+  }
+  const double f0 = (c0 + 0.5 * nh) / S.grid[0], f1 = (c1 + 0.5) / S.grid[1], f2 = (c2 + 0.5) / S.grid[2];
   const double X0 = f0 * S.rvecs[0] + f1 * S.rvecs[3] + f2 * S.rvecs[6];
   const double X1 = f0 * S.rvecs[1] + f1 * S.rvecs[4] + f2 * S.rvecs[7];
   const double X2 = f0 * S.rvecs[2] + f1 * S.rvecs[5] + f2 * S.rvecs[8];
 
   // Partner-bin table: one thread per partner bin, all bins at once (one
   // round trip for their sizes), then the exclusive prefix of the sizes
-  // (64 bins per step in wave 0).
+  // (64 bins per step in wave 0).  Bin k = (i0 n1 + i1) n2 + i2.
   for (int k = tid; k < nnb; k += BW * 64) {
-    const int i2 = k % n2, i1 = (k / n2) % n1, i0 = k / (n1 * n2);
+    const int i2 = k % n2, i1 = (k / n2) % n1, i0 = k / n12;
     int nb0 = c0 + i0 - R0, nb1 = c1 + i1 - R1, nb2 = c2 + i2 - R2;
     const int s0 = floor_div(nb0, S.grid[0]), s1 = floor_div(nb1, S.grid[1]), s2 = floor_div(nb2, S.grid[2]);
     nb0 -= s0 * S.grid[0];
@@ -227,9 +249,14 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
   // [-511, 511].  The unpacked test covers the rest.
   const bool no_groups = nlow <= 0 && nhigh >= S.natom;
   const int32_t zero_code = pack_img(0, 0, 0);
-  for (int ib = start + (int)blockIdx.y * BW; ib < end; ib += (int)gridDim.y * BW) {
-    const int p = ib + wv;
-    const bool active = p < end;  // uniform in the wave
+  for (int ia = (int)blockIdx.y * BW; ia < natoms; ia += (int)gridDim.y * BW) {
+    const int a = ia + wv;
+    const bool active = a < natoms;  // uniform in the wave
+    // home bin h of the atom and its slot p
+    const int h = a >= hend[0] ? 1 : 0;
+    const int p = (b + h * hstride) * cb + (a - (h > 0 ? hend[0] : 0));
+    // its search box: first-axis slices h .. h + 2 R0 of the run
+    const int rlo = l_pre[h * n12], rhi = l_pre[(h + w0) * n12];
     double4 xi = make_double4(0, 0, 0, 0);
     int4 wi = make_int4(0, 0, 0, 0);
     if (active) {
@@ -257,6 +284,9 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
       }
       if (!active) continue;
       const int lim = min(T - base, CAP);
+      // the atom's candidates in this stage: [q_lo, q_hi)
+      const int q_lo = max(rlo - base, 0), q_hi = min(rhi - base, lim);
+      if (q_lo >= q_hi) continue;
       const bool wi_small = wi.x >= -255 && wi.x <= 255 && wi.y >= -255 && wi.y <= 255 && wi.z >= -255 && wi.z <= 255;
       if (!EXACT && !any_wide && wi_small) {  // uniform in the wave
         const int32_t wlin = wi.x + (wi.y << 10) + (wi.z << 20);
@@ -265,47 +295,80 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
         const int32_t gi_u = __builtin_amdgcn_readfirstlane((int32_t)gi);
         int32_t *const rj = nl.j + (int64_t)gi_u * nl.kcap;
         int32_t *const ri = nl.img + (int64_t)gi_u * nl.kcap;
-        // one step of 64 candidates; TAIL: the last, partial step; GROUPS:
-        // the nlow/nhigh test is needed
-        auto scan_step = [&](int q0, auto tail_c, auto groups_c) {
+        // the test of candidate q (qc: the slot read, q clamped for a tail)
+        auto test = [&](int q, int qc, auto tail_c, auto groups_c, int32_t &gj, int32_t &code) {
           constexpr bool TAIL = decltype(tail_c)::value, GROUPS = decltype(groups_c)::value;
-          const int q = q0 + lane;
-          const int qc = TAIL ? min(q, lim - 1) : q;
           const float dx = l_x[qc] - xr, dy = l_y[qc] - yr, dz = l_z[qc] - zr;
           const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          const int32_t gj = l_g[qc];
-          const int32_t code = l_m[qc] + wlin;
+          gj = l_g[qc];
+          code = l_m[qc] + wlin;
           bool acc = (d2 < hi32) & ((gj != gi_u) | (code != zero_code));
-          if (TAIL) acc = acc & (q < lim);
+          if (TAIL) acc = acc & (q < q_hi);
           if (GROUPS) {
             const int lj = gj - atom0;
-            acc = acc && nlow_nhigh_ok(li > lj ? li : lj, li > lj ? lj : li, nlow, nhigh);
+            acc = acc & nlow_nhigh_ok(li > lj ? li : lj, li > lj ? lj : li, nlow, nhigh);
           }
-          const uint64_t mask = __ballot(acc);
-          // with room for a whole step in the row (uniform) no lane can pass
-          // the capacity
-          const bool room = cnt + 64 <= nl.kcap;
-          const uint32_t o = (uint32_t)(cnt + lanes_below(mask));
-          if (acc && (room || o < (uint32_t)nl.kcap)) {
+          return acc;
+        };
+        // appends the accepted candidates of one ballot at offset base of the
+        // row; ROOM: the row has room for 64 more entries (uniform), so no
+        // lane can pass the capacity
+        auto append = [&](bool acc, uint64_t mask, int base, int32_t gj, int32_t code, auto room_c) {
+          const uint32_t o = (uint32_t)(base + lanes_below(mask));
+          if (acc & (decltype(room_c)::value || o < (uint32_t)nl.kcap)) {
             const uint32_t ob = o * 4u;  // bytes; a row is far below 4 GiB
             *(int32_t *)((char *)rj + ob) = gj;
             *(int32_t *)((char *)ri + ob) = code;
           }
+        };
+        // one step of 64 candidates; TAIL: the last, partial step; GROUPS:
+        // the nlow/nhigh test is needed
+        auto scan_step = [&](int q0, auto tail_c, auto groups_c) {
+          const int q = q0 + lane;
+          const int qc = decltype(tail_c)::value ? min(q, q_hi - 1) : q;
+          int32_t gj, code;
+          const bool acc = test(q, qc, tail_c, groups_c, gj, code);
+          const uint64_t mask = __ballot(acc);
+          if (cnt + 64 <= nl.kcap)
+            append(acc, mask, cnt, gj, code, std::true_type{});
+          else
+            append(acc, mask, cnt, gj, code, std::false_type{});
           cnt += __popcll(mask);
         };
-        const int nfull = lim & ~63;
+        // two full steps at once (128 candidates, entries in candidate order):
+        // the loop and count bookkeeping once per 128
+        auto scan_step2 = [&](int q0, auto groups_c) {
+          const int qa = q0 + lane, qb = q0 + 64 + lane;
+          int32_t gja, codea, gjb, codeb;
+          const bool acca = test(qa, qa, std::false_type{}, groups_c, gja, codea);
+          const bool accb = test(qb, qb, std::false_type{}, groups_c, gjb, codeb);
+          const uint64_t ma = __ballot(acca), mb = __ballot(accb);
+          const int cb2 = cnt + __popcll(ma);
+          if (cnt + 128 <= nl.kcap) {  // uniform: the common case
+            append(acca, ma, cnt, gja, codea, std::true_type{});
+            append(accb, mb, cb2, gjb, codeb, std::true_type{});
+          } else {
+            append(acca, ma, cnt, gja, codea, std::false_type{});
+            append(accb, mb, cb2, gjb, codeb, std::false_type{});
+          }
+          cnt = cb2 + __popcll(mb);
+        };
+        const int n = q_hi - q_lo;
+        const int n128 = q_lo + (n & ~127), nfull = q_lo + (n & ~63);
         if (no_groups) {
-          for (int q0 = 0; q0 < nfull; q0 += 64) scan_step(q0, std::false_type{}, std::false_type{});
-          if (nfull < lim) scan_step(nfull, std::true_type{}, std::false_type{});
+          for (int q0 = q_lo; q0 < n128; q0 += 128) scan_step2(q0, std::false_type{});
+          if (n128 < nfull) scan_step(n128, std::false_type{}, std::false_type{});
+          if (nfull < q_hi) scan_step(nfull, std::true_type{}, std::false_type{});
         } else {
-          for (int q0 = 0; q0 < nfull; q0 += 64) scan_step(q0, std::false_type{}, std::true_type{});
-          if (nfull < lim) scan_step(nfull, std::true_type{}, std::true_type{});
+          for (int q0 = q_lo; q0 < n128; q0 += 128) scan_step2(q0, std::true_type{});
+          if (n128 < nfull) scan_step(n128, std::false_type{}, std::true_type{});
+          if (nfull < q_hi) scan_step(nfull, std::true_type{}, std::true_type{});
         }
         continue;
       }
-      for (int q0 = 0; q0 < lim; q0 += 64) {
+      for (int q0 = q_lo; q0 < q_hi; q0 += 64) {
         const int q = q0 + lane;
-        const int qc = q < lim ? q : lim - 1;
+        const int qc = q < q_hi ? q : q_hi - 1;
         // branch-free test of the candidate (all lanes read a valid slot)
         const float dx = l_x[qc] - xr, dy = l_y[qc] - yr, dz = l_z[qc] - zr;
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -318,7 +381,7 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
         const bool self = (gj == gi);
         const int lj = gj - atom0;
         const int A = li > lj ? li : lj, B = li > lj ? lj : li;
-        bool acc = (q < lim) && (d2 < hi32) && !(self && n[0] == 0 && n[1] == 0 && n[2] == 0) &&
+        bool acc = (q < q_hi) && (d2 < hi32) && !(self && n[0] == 0 && n[1] == 0 && n[2] == 0) &&
                    nlow_nhigh_ok(A, B, nlow, nhigh);
         if (EXACT && acc && !(fast_ok && d2 < lo32)) {
           const bool i_is_a = (li == A);
@@ -688,6 +751,11 @@ int ynb_sys_plan(ynb_sys *s) {
   return YNB_OK;
 }
 
+int ynb_build_search_bins(const ynb_sys *s) {
+  if (!s || s->nbin <= 0) return 0;
+  return build_search_bins(s->range);
+}
+
 size_t ynb_nlist_workspace_bytes(int64_t npad, int32_t nbin, int32_t bin_cap) {
   return carve(nullptr, npad, nbin, bin_cap).total;
 }
@@ -716,7 +784,7 @@ int ynb_nlist_build(const ynb_batch *b, const double *pos, void *work, size_t wo
     if (!(nl->flags & YNB_NLIST_PREBINNED))
       YNB_LAUNCH("nlist_bin", k_bin, dim3(nb), dim3(tb), 0, st, b->sys, b->wave_sys, npad, pos, cb, w.bin_count,
                  w.slot_idx, w.xw, w.wr, nl->status, nl->rebuild);
-    YNB_LAUNCH("nlist_bin_order", k_bin_order, dim3(b->nbin), dim3(256), 0, st, cb, w.bin_count, w.slot_idx, w.xw,
+    YNB_LAUNCH("nlist_bin_order", k_bin_order, dim3(b->nbin), dim3(64), 0, st, cb, w.bin_count, w.slot_idx, w.xw,
                w.wr, w.sxw, w.sw, w.bin_end, b->bin_sys, nl->rebuild);
 #define YNB_BUILD(EX, NBB)                                                                                    \
   YNB_LAUNCH("nlist_build_binned", (k_build_binned<EX, NBB>), dim3(b->nbin, ychunks), dim3(BW * 64), 0, st,        \

@@ -638,7 +638,8 @@ class Engine(object):
     def _batch_flags(self):
         """Build paths the batch needs: all-pairs (systems without bins), the
         exact in-build re-test (binned systems that are not box_safe) and the
-        large partner-bin tables (search boxes of more than 125 bins)."""
+        large partner-bin tables (more than BUILD_NB_SMALL partner bins per
+        home group)."""
         f = 0
         for i in range(self.nsys):
             h = self.hsys[i]
@@ -646,7 +647,7 @@ class Engine(object):
                 f |= _lib.BATCH_ALLPAIRS
             if h.nbin > 0 and not h.box_safe:
                 f |= _lib.BATCH_EXACT_BUILD
-            if h.nbin > 0 and (2*h.range[0] + 1)*(2*h.range[1] + 1)*(2*h.range[2] + 1) > 125:
+            if self.lib.ynb_build_search_bins(_lib.C.byref(h)) > _lib.BUILD_NB_SMALL:
                 f |= _lib.BATCH_WIDE_SEARCH
             f |= {_lib.FORM_GENERIC: _lib.BATCH_FORM_GENERIC, _lib.FORM_LJ: _lib.BATCH_FORM_LJ,
                   _lib.FORM_IONIC: _lib.BATCH_FORM_IONIC}[h.pair_form]
@@ -773,8 +774,9 @@ class Engine(object):
     def nlist_status(self):
         """(flags, longest row) of the builds since the last call; synchronises
         and clears the (sticky) flags.  Flags: 1 a row exceeded kcap, 2 an
-        image shift exceeded 511 cells, 4 a bin exceeded its capacity.  The
-        longest row is read only after a flag."""
+        image shift exceeded 511 cells, 4 a bin exceeded its capacity, 8 the
+        partner-bin tables were too small for a system (the batch flags do
+        not match its systems).  The longest row is read only after a flag."""
         flags = int(self.status_d[0].item())
         longest = 0
         if flags & 1:
@@ -789,6 +791,8 @@ class Engine(object):
         """Enlarges the capacities named by the flags of a failed build."""
         if flags & 2:
             raise RuntimeError('neighbour list: an image translation exceeds 511 cells')
+        if flags & 8:
+            raise RuntimeError('neighbour list: partner-bin tables smaller than a system needs (batch flags)')
         if flags & 4:
             cap = _round_up(int(self._fullest_bin*1.25) + 16, 16)
             if cap > 1024:

@@ -318,8 +318,8 @@ class Server(object):
         if flags & 4:
             eng._fullest_bin = max(int(eng.status_d[1].item()), 2*eng.bin_cap)
             eng.grow(4, 0)
-        if flags & 2:
-            eng.grow(2, 0)
+        if flags & 10:
+            eng.grow(flags & 10, 0)
         eng.status_d.zero_()
         redo = {}
         for fl in inflight:

@@ -594,7 +594,7 @@ class SlotEngine(Engine):
             f |= _lib.BATCH_ALLPAIRS
         if h.nbin > 0 and not h.box_safe:
             f |= _lib.BATCH_EXACT_BUILD
-        if h.nbin > 0 and (2*h.range[0] + 1)*(2*h.range[1] + 1)*(2*h.range[2] + 1) > 125:
+        if h.nbin > 0 and _lib.load().ynb_build_search_bins(_lib.C.byref(h)) > _lib.BUILD_NB_SMALL:
             f |= _lib.BATCH_WIDE_SEARCH
         f |= {_lib.FORM_GENERIC: _lib.BATCH_FORM_GENERIC, _lib.FORM_LJ: _lib.BATCH_FORM_LJ,
               _lib.FORM_IONIC: _lib.BATCH_FORM_IONIC}[h.pair_form]
@@ -678,6 +678,8 @@ class SlotEngine(Engine):
         """Enlarges the capacities named by the flags of a failed build."""
         if flags & 2:
             raise RuntimeError('neighbour list: an image translation exceeds 511 cells')
+        if flags & 8:
+            raise RuntimeError('neighbour list: partner-bin tables smaller than a system needs (batch flags)')
         if flags & 4:
             cap = _round_up(int(self._fullest_bin*1.25) + 16, 16)
             if cap > 1024:
