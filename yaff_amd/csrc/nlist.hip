@@ -129,22 +129,34 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
   const int sid = bin_sys[b];
   if (sid < 0 || (nl.rebuild && !nl.rebuild[sid])) return;
   const ynb_sys &S = sys[sid];
-  const int lb = b - S.bin0;
-  const int c2 = lb % S.grid[2], c1 = (lb / S.grid[2]) % S.grid[1], c0 = lb / (S.grid[1] * S.grid[2]);
-  const int R0 = S.range[0], R1 = S.range[1], R2 = S.range[2];
-  const int LG = build_group_len(S.range);
-  if (c0 % LG != 0) return;  // built by the run's first bin
-  const int nh = min(LG, S.grid[0] - c0);  // home bins b + h hstride, h < nh
   const int hstride = S.grid[1] * S.grid[2];
-  // atoms of the run: home bin h holds n_h of them from slot (b + h hstride) cb
+  const int LG = build_group_len(S.range);
+  int lb = b - S.bin0;
+  const int c2 = lb % S.grid[2], c1 = (lb / S.grid[2]) % S.grid[1];
+  int c0 = lb / hstride;
+  // The run's blocks: with one block row (gridDim.y == 1, many bins) the
+  // run's first bin builds it alone and the other bin's block exits; with
+  // several rows the blocks of both bins split the run's atoms (parts
+  // 2 y + 0 and 2 y + 1), so the first row holds the work of every run.
+  const int off = c0 % LG;
+  c0 -= off;
+  lb -= off * hstride;
+  const int nh = min(LG, S.grid[0] - c0);  // home bins bh + h hstride, h < nh
+  const int bh = S.bin0 + lb;               // the run's first bin
+  const bool split = nh > 1 && gridDim.y > 1;
+  if (off != 0 && !split) return;
+  const int part = split ? 2 * (int)blockIdx.y + off : (int)blockIdx.y;
+  const int nparts = split ? 2 * (int)gridDim.y : (int)gridDim.y;
+  const int R0 = S.range[0], R1 = S.range[1], R2 = S.range[2];
+  // atoms of the run: home bin h holds n_h of them from slot (bh + h hstride) cb
   int hend[2];  // prefix of the home bins' atom counts (uniform)
   int natoms = 0;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    if (h < nh) natoms += bin_end[b + h * hstride] - (b + h * hstride) * cb;
+    if (h < nh) natoms += bin_end[bh + h * hstride] - (bh + h * hstride) * cb;
     hend[h] = natoms;
   }
-  if ((int)blockIdx.y * BW >= natoms) return;
+  if (part * BW >= natoms) return;
 
   const int n1 = 2 * R1 + 1, n2 = 2 * R2 + 1, n12 = n1 * n2;
   const int w0 = 2 * R0 + 1;             // first-axis slices searched per home bin
@@ -196,7 +208,17 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
     if (lane == 0) l_pre[0] = 0;
   }
   __syncthreads();
-  const int T = l_pre[nnb];
+  // Candidates staged: the search boxes of the home bins this block's atoms
+  // belong to (with one pass of the block over the run's atoms, its parts
+  // stage only what their atoms search), from sb on.
+  int h_lo = 0, h_hi = nh - 1;
+  if (nparts * BW >= natoms) {
+    const int a0 = part * BW, a1 = min(a0 + BW, natoms) - 1;
+    h_lo = a0 >= hend[0] ? 1 : 0;
+    h_hi = a1 >= hend[0] ? 1 : 0;
+  }
+  const int sb = l_pre[h_lo * n12];
+  const int T = l_pre[(h_hi + w0) * n12] - sb;
   const double rc2 = S.rc * S.rc;
   // per-system values used in the scan, held in registers: the scan stores
   // rows through pointers the compiler cannot separate from sys, so reads of
@@ -206,19 +228,20 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
   bool bad = false;
   bool wide = false;  // a staged image shift outside [-255, 255] (this thread)
 
-  // Stage the candidates [base, base + CAP): candidate f belongs to the
-  // partner bin k with l_pre[k] <= f < l_pre[k + 1] (a table written bin by
-  // bin, then one lookup per candidate).
+  // Stage the candidates sb + [base, base + CAP): candidate f belongs to
+  // the partner bin k with l_pre[k] <= f < l_pre[k + 1] (a table written bin
+  // by bin, then one lookup per candidate).
   auto stage = [&](int base) {
     const int lim = min(T - base, CAP);
+    const int ab = sb + base;
     // partner bin of each staged slot: thread k writes bin k over its range
     for (int k = tid; k < nnb; k += BW * 64) {
-      const int f1 = min(l_pre[k + 1] - base, lim);
-      for (int f = max(l_pre[k] - base, 0); f < f1; ++f) l_bin[f] = (int16_t)k;
+      const int f1 = min(l_pre[k + 1] - ab, lim);
+      for (int f = max(l_pre[k] - ab, 0); f < f1; ++f) l_bin[f] = (int16_t)k;
     }
     __syncthreads();
     for (int q = tid; q < lim; q += BW * 64) {
-      const int f = base + q;
+      const int f = ab + q;
       const int lo = l_bin[q];
       const int j = l_bs[lo] + (f - l_pre[lo]);
       const double4 x = sxw[j];
@@ -249,14 +272,14 @@ __global__ __launch_bounds__(BW * 64) void k_build_binned(const ynb_sys *__restr
   // [-511, 511].  The unpacked test covers the rest.
   const bool no_groups = nlow <= 0 && nhigh >= S.natom;
   const int32_t zero_code = pack_img(0, 0, 0);
-  for (int ia = (int)blockIdx.y * BW; ia < natoms; ia += (int)gridDim.y * BW) {
+  for (int ia = part * BW; ia < natoms; ia += nparts * BW) {
     const int a = ia + wv;
     const bool active = a < natoms;  // uniform in the wave
     // home bin h of the atom and its slot p
     const int h = a >= hend[0] ? 1 : 0;
-    const int p = (b + h * hstride) * cb + (a - (h > 0 ? hend[0] : 0));
+    const int p = (bh + h * hstride) * cb + (a - (h > 0 ? hend[0] : 0));
     // its search box: first-axis slices h .. h + 2 R0 of the run
-    const int rlo = l_pre[h * n12], rhi = l_pre[(h + w0) * n12];
+    const int rlo = l_pre[h * n12] - sb, rhi = l_pre[(h + w0) * n12] - sb;
     double4 xi = make_double4(0, 0, 0, 0);
     int4 wi = make_int4(0, 0, 0, 0);
     if (active) {
