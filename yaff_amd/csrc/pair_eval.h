@@ -1,5 +1,5 @@
-// Pair potentials evaluated per pair, shared by the list-driven pair kernels
-// (pair.hip) and the fused neighbour-search + pair kernel (nlist.hip):
+// Pair potentials evaluated per pair, for the list-driven pair kernels
+// (pair.hip):
 // Yaff's pair functions (yaff/pes/pair_pot.c:277-352, 473-489, 711-751) and
 // truncations (yaff/pes/truncation.c:30-95) in FP64, their FP32 forms for
 // mixed precision, and one entry of a specialised potential set
@@ -274,14 +274,12 @@ __device__ inline FormConst form_const(const ynb_sys &S, const double *cross) {
 // rebuilt exactly as nlist_build_low does (yaff_row_vector, positions from
 // pos), so the cutoff tests d < rcut see Yaff's bits, and u is replaced by
 // it.  Returns false when the pair is beyond every cutoff; otherwise the
-// energy and Yaff's vg (dE/dd / d) summed over the set's potentials.  FP64
-// mode: one reciprocal distance per pair (device rsqrt, a few ulp);
-// ExpRep, LJ (eps from sqrt(eps) per atom) and EI (alpha > 0, point charges,
-// q / sqrt(dielectric) per atom) as PotFast.  Mixed mode: squared distance
-// and cutoff decisions in FP64, the potentials in FP32 (native reciprocal
-// square root and exp2; erfc = exp(-x^2) erfcx(x) with x^2 reduced in FP64
-// and erfcx an FP32 polynomial).  ri/rj: per-atom records (LJ sigma,
-// sqrt(LJ eps), charge / sqrt(dielectric), ExpRep type).
+// energy and Yaff's vg (dE/dd / d) summed over the set's potentials, in
+// FP64: one reciprocal distance per pair (device rsqrt, a few ulp); ExpRep,
+// LJ (eps from sqrt(eps) per atom) and EI (alpha > 0, point charges,
+// q / sqrt(dielectric) per atom) as PotFast.  ri/rj: per-atom records (LJ
+// sigma, sqrt(LJ eps), charge / sqrt(dielectric), ExpRep type).  Mixed
+// precision: form_geom_mixed and form_pot_mixed below.
 template <int PREC, int FORM>
 __device__ inline bool form_pair(const ynb_sys &S, const FormConst &C, int li, int64_t gi, int64_t gj,
                                  const double pi[3], const double *pos, const int nn[3], const double4 &ri,
@@ -302,7 +300,8 @@ __device__ inline bool form_pair(const ynb_sys &S, const FormConst &C, int li, i
   };
   e_sum = 0.0;
   vg_sum = 0.0;
-  if constexpr (PREC == YNB_FP64) {
+  static_assert(PREC == YNB_FP64, "form_pair evaluates FP64 entries");
+  {
     // 1/d from the device rsqrt (a few ulp) and d = r2 / d: every decision
     // that depends on the last bits of d is made on the exact path
     const double r2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
@@ -351,74 +350,119 @@ __device__ inline bool form_pair(const ynb_sys &S, const FormConst &C, int li, i
       }
     }
     return true;
-  } else {
-    double r2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    bool near_cut = fabs(r2 - C.rc0sq) <= C.tol0sq;
-    if (FORM == YNB_FORM_IONIC)
-      near_cut = near_cut || fabs(r2 - C.rc1sq) <= C.tol1sq || fabs(r2 - C.rc2sq) <= C.tol2sq;
-    bool in0, in1, in2;
-    if (near_cut) {
-      double v[3];
-      exact(v);
-      r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-      const double dd = sqrt(r2);
-      in0 = dd < C.rc0;
-      in1 = dd < C.rc1;
-      in2 = dd < C.rc2;
-    } else {
-      in0 = r2 < C.rc0sq;
-      in1 = r2 < C.rc1sq;
-      in2 = r2 < C.rc2sq;
-    }
-    if (!(in0 || (FORM == YNB_FORM_IONIC && (in1 || in2)))) return false;
-    const float r2f = (float)r2;
-    const float inv = __builtin_amdgcn_rsqf(r2f);
-    const float inv2 = inv * inv;
-    const float df = r2f * inv;
-    float ef = 0.0f, gf = 0.0f;
-    if (FORM == YNB_FORM_LJ) {
-      float g;
-      float e = lj32((float)ri.x, (float)rj.x, (float)(ri.y * rj.y), inv, inv2, g);
-      if (C.trunc0 == YNB_TR_SWITCH3) {
-        float hg;
-        const float h = switch3_32(df, C.rc0f, C.w0f, hg);
-        g = g * h + e * hg * inv;
-        e *= h;
-      }
-      ef = e;
-      gf = g;
-    } else {
-      if (in0) {  // ExpRep
-        const int ci = ti * C.nf + (int)rj.w;
-        const float a = C.lds_cross ? C.l_ampf[ci] : (float)C.amp[ci];
-        const float b = C.lds_cross ? C.l_bexf[ci] : (float)C.bex[ci];
-        if (a != 0.0f && b != 0.0f) {
-          const float e = a * __builtin_amdgcn_exp2f(-b * df * 1.44269504088896341f);
-          ef += e;
-          gf -= e * b * inv;
-        }
-      }
-      if (in1) {  // LJ
-        float g;
-        ef += lj32((float)ri.x, (float)rj.x, (float)(ri.y * rj.y), inv, inv2, g);
-        gf += g;
-      }
-      if (in2) {  // EI, point charges, alpha > 0: exp(-x^2) with x^2 = alpha^2 r^2 reduced in FP64
-        const double a2r2 = C.alpha2 * r2;
-        const double tq = a2r2 * -1.4426950408889634074;
-        const double kq = rint(tq);
-        const float e2 = __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f((float)(tq - kq)), (int)kq);
-        const float x = fminf(__builtin_amdgcn_sqrtf((float)a2r2), ERFCX32_XMAX);
-        const float qq = (float)(ri.z * rj.z);
-        const float e = qq * (e2 * erfcx32(x)) * inv;
-        ef += e;
-        gf += (-1.12837916709551257f * C.alphaf * e2 * qq - e) * inv2;
-      }
-    }
-    e_sum = (double)ef;
-    vg_sum = (double)gf;
-    return true;
   }
+}
+
+// One mixed-precision entry of a specialised set in two phases, so that a
+// lane can evaluate two entries with their arithmetic interleaved (the FP64
+// kernels use form_pair, one entry at a time):
+//
+// form_geom_mixed: u holds pos_j - pos_i + n.R (FP64); where a squared
+// distance lies within a relative 4e-12 of a squared cutoff the vector is
+// rebuilt exactly as nlist_build_low does (yaff_row_vector) and u replaced
+// by it.  Fills r^2 (FP64) and the flags of the set's cutoffs, decided in
+// FP64 as form_pair decides them; returns false when the pair is beyond
+// every cutoff.
+//
+// form_pot_mixed: the energy and Yaff's vg (dE/dd / d) summed over the set's
+// potentials in FP32 (the arithmetic of form_pair's mixed branch), as
+// straight-line code: every potential is evaluated and those beyond their
+// cutoff (or with a zero ExpRep parameter) are selected away, which gives
+// form_pair's sums bit for bit.
+struct PairGeom {
+  double r2;  // squared distance (FP64)
+  bool in0, in1, in2;
+};
+
+template <int FORM>
+__device__ inline bool form_geom_mixed(const ynb_sys &S, const FormConst &C, int li, int64_t gi, int64_t gj,
+                                       const double pi[3], const double *pos, const int nn[3], double u[3],
+                                       PairGeom &Gm) {
+  double r2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  bool near_cut = fabs(r2 - C.rc0sq) <= C.tol0sq;
+  if (FORM == YNB_FORM_IONIC)
+    near_cut = near_cut || fabs(r2 - C.rc1sq) <= C.tol1sq || fabs(r2 - C.rc2sq) <= C.tol2sq;
+  if (near_cut) {  // Yaff's own row vector (no scaling rows in a form)
+    const int lj = (int)(gj - S.atom0);
+    const bool self = (gj == gi);
+    const bool i_is_a = li > lj || (li == lj);
+    const int nab[3] = {i_is_a ? nn[0] : -nn[0], i_is_a ? nn[1] : -nn[1], i_is_a ? nn[2] : -nn[2]};
+    const double pj[3] = {pos[3 * gj], pos[3 * gj + 1], pos[3 * gj + 2]};
+    double v[3];
+    int r[3];
+    yaff_row_vector(i_is_a ? pi : pj, i_is_a ? pj : pi, self, nab, S.rvecs, S.gvecs, S.nvec, S.rmax, v, r);
+    u[0] = i_is_a ? v[0] : -v[0];
+    u[1] = i_is_a ? v[1] : -v[1];
+    u[2] = i_is_a ? v[2] : -v[2];
+    r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double dd = sqrt(r2);
+    Gm.in0 = dd < C.rc0;
+    Gm.in1 = dd < C.rc1;
+    Gm.in2 = dd < C.rc2;
+  } else {
+    Gm.in0 = r2 < C.rc0sq;
+    Gm.in1 = r2 < C.rc1sq;
+    Gm.in2 = r2 < C.rc2sq;
+  }
+  Gm.r2 = r2;
+  return Gm.in0 || (FORM == YNB_FORM_IONIC && (Gm.in1 || Gm.in2));
+}
+
+template <int FORM>
+__device__ inline void form_pot_mixed(const FormConst &C, const PairGeom &Gm, const double4 &ri, const double4 &rj,
+                                      int ti, double &e_sum, double &vg_sum) {
+  const double r2 = Gm.r2;
+  const float r2f = (float)r2;
+  const float inv = __builtin_amdgcn_rsqf(r2f);
+  const float inv2 = inv * inv;
+  const float df = r2f * inv;
+  float ef = 0.0f, gf = 0.0f;
+  if (FORM == YNB_FORM_LJ) {
+    float g;
+    float e = lj32((float)ri.x, (float)rj.x, (float)(ri.y * rj.y), inv, inv2, g);
+    if (C.trunc0 == YNB_TR_SWITCH3) {  // switch3_32 as selects
+      const float xr = C.rc0f - df;
+      const float rw = __builtin_amdgcn_rcpf(C.w0f);
+      const float x = xr * rw;
+      const float hgp = -6.0f * x * (1.0f - x) * rw;
+      const float hp = (3.0f - 2.0f * x) * x * x;
+      const bool flat = xr > C.w0f;
+      const float h = flat ? 1.0f : hp, hg = flat ? 0.0f : hgp;
+      g = g * h + e * hg * inv;
+      e *= h;
+    }
+    ef = e;
+    gf = g;
+  } else {
+    {  // ExpRep
+      const int ci = ti * C.nf + (int)rj.w;
+      const float a = C.lds_cross ? C.l_ampf[ci] : (float)C.amp[ci];
+      const float b = C.lds_cross ? C.l_bexf[ci] : (float)C.bex[ci];
+      const float e = a * __builtin_amdgcn_exp2f(-b * df * 1.44269504088896341f);
+      const bool on = Gm.in0 && a != 0.0f && b != 0.0f;
+      ef += on ? e : 0.0f;
+      gf -= on ? e * b * inv : 0.0f;
+    }
+    {  // LJ
+      float g;
+      const float e = lj32((float)ri.x, (float)rj.x, (float)(ri.y * rj.y), inv, inv2, g);
+      ef += Gm.in1 ? e : 0.0f;
+      gf += Gm.in1 ? g : 0.0f;
+    }
+    {  // EI, point charges, alpha > 0: exp(-x^2) with x^2 = alpha^2 r^2 reduced in FP64
+      const double a2r2 = C.alpha2 * r2;
+      const double tq = a2r2 * -1.4426950408889634074;
+      const double kq = rint(tq);
+      const float e2 = __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f((float)(tq - kq)), (int)kq);
+      const float x = fminf(__builtin_amdgcn_sqrtf((float)a2r2), ERFCX32_XMAX);
+      const float qq = (float)(ri.z * rj.z);
+      const float e = qq * (e2 * erfcx32(x)) * inv;
+      ef += Gm.in2 ? e : 0.0f;
+      gf += Gm.in2 ? (-1.12837916709551257f * C.alphaf * e2 * qq - e) * inv2 : 0.0f;
+    }
+  }
+  e_sum = (double)ef;
+  vg_sum = (double)gf;
 }
 
 }  // namespace
