@@ -224,7 +224,7 @@ def fp64_instance(kernel, tuning):
     if kernel == 'ewald_force':
         return '%s<0, %d>' % (base, tuning['fapl'])
     if kernel == 'nlist_build_binned':
-        return '%s<false, 128>' % base
+        return '%s<false, 160>' % base
     return base
 
 

@@ -6,8 +6,13 @@
 // exp(2 pi i f0).  The k vectors are grouped in rows (runs of consecutive g0);
 // at a row start each lane seeds exp(i k.r) of its atoms from the reduced
 // phase g.f - rint(g.f) (FP64 in both precisions) and steps it along the row.
-// Fractional coordinates are wrapped to [0, 1), which leaves every phase
-// unchanged modulo 2 pi.
+// The FP64 passes step by the three-term recurrence
+// z_{j+1} = 2 cos(2 pi f0) z_j - z_{j-1} (one FMA per component instead of a
+// complex product; z_{-1} = z_0 exp(-2 pi i f0) at the row start).  Its
+// rounding error after j steps is below ~j^2 eps whatever f0 (1e-13 for
+// the 45-k rows of the 6912-atom rocksalt, against ~j eps for the complex
+// product).  Fractional coordinates are wrapped to [0, 1), which leaves
+// every phase unchanged modulo 2 pi.
 //
 // Pass 1 (k_ewald_sk): lanes hold atoms (apl per lane) of one chunk and walk
 //   a block of whole rows; per k a lane sums its atoms, and every 8 k the
@@ -18,7 +23,7 @@
 // Pass 2 (k_ewald_kreduce): one thread per k sums its chunks in a fixed
 //   order per group (atoms below nlow, between, above nhigh), forms the
 //   energy and virial terms of ewald.c:82-127 and the coefficients 2 c S(k).
-// Pass 3 (k_ewald_force): lanes hold 2 or 4 atoms each; the per-k
+// Pass 3 (k_ewald_force): lanes hold 1, 2 or 4 atoms each; the per-k
 //   coefficients are uniform and come through scalar loads; the k vector
 //   weighting is carried by two running sums per row; the 4 wavefronts of a
 //   block walk 4 row blocks of the system and are summed in LDS.
@@ -131,6 +136,11 @@ __device__ inline void continue_seed(double &zr, double &zi, double c1, double s
   }
 }
 
+// Pointers into the constant address space (data that no kernel of the
+// launch writes: loads at uniform addresses become scalar loads).
+typedef const __attribute__((address_space(4))) double cdouble;
+__device__ inline double4 load_c4(const cdouble *p) { return make_double4(p[0], p[1], p[2], p[3]); }
+
 // LDS tile of the structure-factor passes: T = 8 k rows of RS = 66 entries
 // (one per lane), padded so that both the row writes and the strided reads
 // are free of bank conflicts.
@@ -190,7 +200,7 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
     const ynb_sys &S = sys[wk.x];
     const int4 ch = *(const int4 *)(chunk + 4 * wk.z);
     const int rb = skblk[2 * wk.y], re = skblk[2 * wk.y + 1];
-    double f0[P], f1[P], f2[P], q[P], cr[P], ci[P], zr[P], zi[P];
+    double f0[P], f1[P], f2[P], q[P], cr[P], ci[P], tc[P], zr[P], zi[P], pr[P], pi[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       const int la = lane + 64 * p;
@@ -202,7 +212,8 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
       q[p] = valid ? frac[3 * npad + g] : 0.0;
       cr[p] = step[g];
       ci[p] = step[npad + g];
-      zr[p] = zi[p] = 0.0;
+      tc[p] = 2.0 * cr[p];
+      zr[p] = zi[p] = pr[p] = pi[p] = 0.0;
     }
     double *out = spart + 2 * (S.spart0 + (int64_t)(wk.z - S.chunk0) * S.nk - S.k0);
     int kt = krow_k[rb];  // first k of the current tile
@@ -214,11 +225,13 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
       // Row blocks of this pass hold few rows (many work items keep the
       // chip busy), so every row is seeded from its reduced phase.
 #pragma unroll
-      for (int p = 0; p < P; ++p) {  // row start: seed q exp(2 pi i g.f)
+      for (int p = 0; p < P; ++p) {  // row start: seed q exp(2 pi i g.f), the term before it z exp(-2 pi i f0)
         double sn, cs;
         seed_sincos<PREC>(reduced_phase(row.x, row.y, row.z, f0[p], f1[p], f2[p]), &sn, &cs);
         zr[p] = q[p] * cs;
         zi[p] = q[p] * sn;
+        pr[p] = zr[p] * cr[p] + zi[p] * ci[p];
+        pi[p] = zi[p] * cr[p] - zr[p] * ci[p];
       }
       for (int j = 0; j < row.w;) {
         // k up to the end of the row or of the tile: a loop without branches
@@ -232,11 +245,16 @@ __global__ __launch_bounds__(64) void k_ewald_sk(const ynb_sys *__restrict__ sys
             si += zi[p];
           }
           l_t[(m + i) * RS + lane] = make_double2(sr, si);
+          // the next term by the three-term recurrence
+          // z_{j+1} = 2 cos(2 pi f0) z_j - z_{j-1} (2 FP64 instructions)
 #pragma unroll
           for (int p = 0; p < P; ++p) {
-            const double nr = zr[p] * cr[p] - zi[p] * ci[p];
-            zi[p] = zr[p] * ci[p] + zi[p] * cr[p];
+            const double nr = fma(tc[p], zr[p], -pr[p]);
+            const double ni = fma(tc[p], zi[p], -pi[p]);
+            pr[p] = zr[p];
+            pi[p] = zi[p];
             zr[p] = nr;
+            zi[p] = ni;
           }
         }
         j += n;
@@ -563,8 +581,8 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
   if (sid < 0) return;
   const ynb_sys &S = sys[sid];
   if (S.nk == 0 || (int)blockIdx.y >= S.nslot) return;  // the same for the whole block
-  double f0[P], f1[P], f2[P], cr[P], ci[P], c1[P], s1[P], zr[P], zi[P], sr0[P], si0[P], T[P], U[P], G0[P], G1[P],
-      G2[P];
+  double f0[P], f1[P], f2[P], cr[P], ci[P], tc[P], c1[P], s1[P], zr[P], zi[P], pr[P], pi[P], sr0[P], si0[P], T[P],
+      U[P], G0[P], G1[P], G2[P];
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     const int64_t g = blk * NA + lane + 64 * p;
@@ -573,6 +591,7 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
     f2[p] = frac[2 * npad + g];
     cr[p] = step[g];
     ci[p] = step[npad + g];
+    tc[p] = 2.0 * cr[p];
     c1[p] = step[2 * npad + g];
     s1[p] = step[3 * npad + g];
     G0[p] = G1[p] = G2[p] = 0.0;
@@ -582,8 +601,10 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
   if (fb < S.nfrb) {
     const double b0 = TWO_PI * S.gvecs[0], b1 = TWO_PI * S.gvecs[1], b2 = TWO_PI * S.gvecs[2];
     const int rb = fblk[2 * (S.frb0 + fb)], re = fblk[2 * (S.frb0 + fb) + 1];
-    const double *rp = frec + 2 * (int64_t)krow_f[rb];
-    double4 na = *(const double4 *)rp, nb = *(const double4 *)(rp + 4);  // next 4 records
+    // the records are read-only for the whole launch: read through the
+    // constant address space, so the loads stay scalar loads
+    const cdouble *rp = (const cdouble *)(frec + 2 * (int64_t)krow_f[rb]);
+    double4 na = load_c4(rp), nb = load_c4(rp + 4);  // next 4 records
     int4 rown = *(const int4 *)(krow + 4 * rb);
     int4 prev = make_int4(0, 0, 0, 0);
     for (int r = rb; r < re; ++r) {
@@ -603,6 +624,9 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
       for (int p = 0; p < P; ++p) {
         zr[p] = sr0[p];
         zi[p] = si0[p];
+        // the term before the row start, z exp(-2 pi i f0)
+        pr[p] = zr[p] * cr[p] + zi[p] * ci[p];
+        pi[p] = zi[p] * cr[p] - zr[p] * ci[p];
         T[p] = U[p] = 0.0;
       }
       const int ng = (row.w + 3) >> 2;
@@ -611,28 +635,51 @@ __global__ __launch_bounds__(256) void k_ewald_force(const ynb_sys *__restrict__
       // leaves nothing in flight at the loop head, so the only wait in the
       // loop falls after its arithmetic.
       __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-      for (int q = 0; q < ng; ++q) {
-        const double4 ca = na, cb = nb;
-        rp += 8;
-        na = *(const double4 *)rp;
-        nb = *(const double4 *)(rp + 4);
-        // keeps the request ahead of this group's arithmetic (the scheduler
-        // would otherwise sink it into the group, shortening its cover)
-        __builtin_amdgcn_sched_barrier(0);
+      // one group of 4 records: T += cs zr - cc zi as two fused steps,
+      // U += T, and the next term by the three-term recurrence
+      // z_{j+1} = 2 cos(2 pi f0) z_j - z_{j-1} (5 FP64 instructions per
+      // (k, atom)); z and p swap roles every step
+      auto group = [&](const double4 ca, const double4 cb) {
         const double cs[4] = {ca.x, ca.z, cb.x, cb.z}, cc[4] = {ca.y, ca.w, cb.y, cb.w};
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
+        for (int m = 0; m < 4; m += 2) {
 #pragma unroll
           for (int p = 0; p < P; ++p) {
-            // T += cs zr - cc zi as two fused steps (7 FP64 instructions
-            // per (k, atom) with the running sum and the complex step)
             T[p] = fma(cs[m], zr[p], fma(-cc[m], zi[p], T[p]));
             U[p] += T[p];
-            const double nr = zr[p] * cr[p] - zi[p] * ci[p];
-            zi[p] = zr[p] * ci[p] + zi[p] * cr[p];
-            zr[p] = nr;
+            pr[p] = fma(tc[p], zr[p], -pr[p]);
+            pi[p] = fma(tc[p], zi[p], -pi[p]);
+            T[p] = fma(cs[m + 1], pr[p], fma(-cc[m + 1], pi[p], T[p]));
+            U[p] += T[p];
+            zr[p] = fma(tc[p], pr[p], -zr[p]);
+            zi[p] = fma(tc[p], pi[p], -zi[p]);
           }
         }
+      };
+      // Two groups per trip, the records alternating between two register
+      // sets (no copies): the request for the next group is issued before a
+      // group's arithmetic (a scheduling barrier keeps it there) and waited
+      // for after it.
+      int q = 0;
+      for (; q + 2 <= ng; q += 2) {
+        const double4 nc = load_c4(rp + 8), nd = load_c4(rp + 12);
+        __builtin_amdgcn_sched_barrier(0);
+        group(na, nb);
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        rp += 16;
+        na = load_c4(rp);
+        nb = load_c4(rp + 4);
+        __builtin_amdgcn_sched_barrier(0);
+        group(nc, nd);
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+      }
+      if (q < ng) {
+        const double4 ca = na, cb = nb;
+        rp += 8;
+        na = load_c4(rp);
+        nb = load_c4(rp + 4);
+        __builtin_amdgcn_sched_barrier(0);
+        group(ca, cb);
       }
       // k_0 of the row and the row's contribution
       const double g0 = (double)row.x, g1 = (double)row.y, g2 = (double)row.z;
