@@ -390,7 +390,9 @@ class Engine(object):
     """Device state and launches for a batch of NBSystem objects."""
 
     overlap = True   # MD steps run the Ewald chain on a second stream (see _md_forces)
+    reci_first = False  # with overlap: issue the Ewald chain before the list and pair chain
     _side = None
+    _fork = None
 
     def __init__(self, systems, precision='fp64', apl=4, skb=None, fkb=None, fapl=4, kcap=None, device='cuda',
                  pair_lanes=32, upload=True):
@@ -922,13 +924,19 @@ class Engine(object):
         # the drift, joined before the correction and the kick; recorded as
         # a fork and a join in a captured graph), so the two chains share the
         # chip.  Their buffers are disjoint; results are the same bits.
+        # The list and pair chain is issued first (unless reci_first): a
+        # captured graph replays the first-issued branch of a fork on the
+        # queue of the node before it, and a dependency within one queue
+        # costs less than one across queues, so the longer chain goes there.
         overlap = self.has_ewald and self.overlap
         if overlap:
             main = torch.cuda.current_stream()
             side = self._side_stream()
-            side.wait_stream(main)
-            with torch.cuda.stream(side):
-                self._md_reci()
+            if self._fork is None:
+                self._fork = torch.cuda.Event()
+            self._fork.record(main)
+            if self.reci_first:
+                self._md_side(side)
         if self._md_prebin():
             self.nlist.flags = _lib.NLIST_PREBINNED
             try:
@@ -940,6 +948,8 @@ class Engine(object):
         self.pair(gpos=True, accumulate=False)
         if self.has_ewald:
             if overlap:
+                if not self.reci_first:
+                    self._md_side(side)
                 main.wait_stream(side)
             else:
                 self._md_reci()
@@ -947,6 +957,12 @@ class Engine(object):
             # has written it) and its energy/virial to ewald_out_d (after the
             # reciprocal pass has written it)
             self.ewald_corr()
+
+    def _md_side(self, side):
+        # the Ewald chain on the side stream, after the fork event
+        side.wait_event(self._fork)
+        with torch.cuda.stream(side):
+            self._md_reci()
 
     def _md_reci(self):
         # the Ewald partial gradients stay in their slots and are summed by
