@@ -79,9 +79,15 @@ def tune(engine, cache=DEFAULT_CACHE, verbose=False):
     for name, values, kernels in CANDIDATES:
         if name != 'pair_lanes' and not engine.has_ewald:
             continue
-        if values is None:  # row-block sizes: around the current choice
+        if values is None:
+            # row-block sizes around the current choice, in steps of 8 k: the
+            # number of blocks (whole rows each) changes in steps, and a launch
+            # that ends just past a full round of resident blocks idles most
+            # of the chip in its last round, so the coarse factors are
+            # refined between 1/2 and 2
             k0 = getattr(engine, name)
-            values = sorted(set(max(16, int(k0*f)) for f in (0.25, 0.5, 1.0, 2.0)))
+            values = sorted(set(max(16, 8*int(round(k0*f/8)))
+                                for f in (0.25, 0.5, 0.625, 0.75, 0.875, 1.0, 1.25, 1.5, 1.625, 1.75, 2.0)))
         results = []
         for v in values:
             engine.set_tuning(**{name: v})
