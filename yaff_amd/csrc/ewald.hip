@@ -426,6 +426,7 @@ __global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_e
   const ynb_sys &S = sys[sid];
   const int lk = k - S.k0;
   const double4 kv = *(const double4 *)(ew.kvec + 4 * (int64_t)k);
+  const int kp = ew.kpos[k];  // requested with kv, used at the end
   __shared__ double l_s[6][8][32];
   // chunk groups: [0, lo) low (1), [lo, nchunk - hi) mid (0), [nchunk - hi, nchunk) high (2)
   const int lo = S.nchunk_low, hi = S.nchunk_high, nmid_end = S.nchunk - hi;
@@ -433,21 +434,23 @@ __global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_e
 #pragma unroll
   for (int grp = 0; grp < 3; ++grp) {
     double ac[4] = {0, 0, 0, 0}, as[4] = {0, 0, 0, 0};
-    if (kv.w != 0.0) {
-      for (int c = gb[grp] + part; c < ge[grp]; c += 32) {
+    // the partial sums are read whether or not k is a padding vector (the
+    // addresses are valid; a padding vector's entries are never written), so
+    // the loads do not wait for kv; a padding vector's sums are dropped below
+    for (int c = gb[grp] + part; c < ge[grp]; c += 32) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int cu = c + 8 * u;
-          if (cu < ge[grp]) {
-            const double2 v = *(const double2 *)(ew.spart + 2 * (S.spart0 + (int64_t)cu * S.nk + lk));
-            ac[u] += v.x;
-            as[u] += v.y;
-          }
+      for (int u = 0; u < 4; ++u) {
+        const int cu = c + 8 * u;
+        if (cu < ge[grp]) {
+          const double2 v = *(const double2 *)(ew.spart + 2 * (S.spart0 + (int64_t)cu * S.nk + lk));
+          ac[u] += v.x;
+          as[u] += v.y;
         }
       }
     }
-    l_s[2 * grp][part][kl] = (ac[0] + ac[1]) + (ac[2] + ac[3]);
-    l_s[2 * grp + 1][part][kl] = (as[0] + as[1]) + (as[2] + as[3]);
+    const bool real_k = kv.w != 0.0;
+    l_s[2 * grp][part][kl] = real_k ? (ac[0] + ac[1]) + (ac[2] + ac[3]) : 0.0;
+    l_s[2 * grp + 1][part][kl] = real_k ? (as[0] + as[1]) + (as[2] + as[3]) : 0.0;
   }
   __syncthreads();
   double e = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
@@ -488,23 +491,18 @@ __global__ __launch_bounds__(256) void k_ewald_kreduce(const ynb_sys *sys, ynb_e
     double *o = ew.coef + 6 * (int64_t)k;
 #pragma unroll
     for (int c = 0; c < 6; ++c) o[c] = cf[c];
-    const int kp = ew.kpos[k];
     if (kp >= 0) *(double2 *)(ew.frec + 2 * (int64_t)kp) = make_double2(cf[1], cf[0]);
-  }
-  __syncthreads();
-  __shared__ double red[7][32];
-  if (part == 0) {
-    red[0][kl] = e;
+    // energy and virial sums of the block's 32 k: a tree over lanes 0-31 of
+    // the first wavefront (lane l adds lane l + off for off = 16, 8, ..., 1)
+    double r[7] = {e, v[0], v[1], v[2], v[3], v[4], v[5]};
 #pragma unroll
-    for (int c = 0; c < 6; ++c) red[1 + c][kl] = v[c];
+    for (int off = 16; off > 0; off >>= 1)
+#pragma unroll
+      for (int c = 0; c < 7; ++c) r[c] += __shfl_down(r[c], off, 32);
+    if (kl == 0)
+#pragma unroll
+      for (int c = 0; c < 7; ++c) ew.kpart[8 * (int64_t)blockIdx.x + c] = r[c];
   }
-  __syncthreads();
-  for (int off = 16; off > 0; off >>= 1) {
-    if (part == 0 && kl < off)
-      for (int c = 0; c < 7; ++c) red[c][kl] += red[c][kl + off];
-    __syncthreads();
-  }
-  if (t < 7) ew.kpart[8 * (int64_t)blockIdx.x + t] = red[t][0];
 }
 
 // Gradient of one atom block (na atoms from g0): the sum of the system's
