@@ -60,6 +60,25 @@ def _time_kernels(engine, names, reps=5):
     return min(times)
 
 
+def _row_block_sizes(k0):
+    """Row-block sizes (k per block) tried around the current k0: every
+    size of a ladder (steps of 8 k up to 256, then 16, 32, 64, ...) from
+    k0/8 to max(8 k0, 512).  The number of blocks (whole rows each) changes
+    in steps, and a launch that ends just past a full round of resident
+    blocks idles most of the chip in its last round, so the best size sits
+    in a narrow window (the force pass of the 6912-atom rocksalt: 31 us at
+    200-216 k per row block, 33-34 us at 192 or 224, from a default of 64)."""
+    lo, hi = k0/8.0, max(8*k0, 512)
+    out, v, step = [], 16, 8
+    while v <= hi:
+        if v >= lo:
+            out.append(v)
+        if v >= 256 and v % (32*step) == 0:
+            step *= 2
+        v += step
+    return out or [k0]
+
+
 def tune(engine, cache=DEFAULT_CACHE, verbose=False):
     """Coordinate descent over CANDIDATES; applies and returns the best
     parameters (loaded from the cache when the key is known)."""
@@ -80,14 +99,7 @@ def tune(engine, cache=DEFAULT_CACHE, verbose=False):
         if name != 'pair_lanes' and not engine.has_ewald:
             continue
         if values is None:
-            # row-block sizes around the current choice, in steps of 8 k: the
-            # number of blocks (whole rows each) changes in steps, and a launch
-            # that ends just past a full round of resident blocks idles most
-            # of the chip in its last round, so the coarse factors are
-            # refined between 1/2 and 2
-            k0 = getattr(engine, name)
-            values = sorted(set(max(16, 8*int(round(k0*f/8)))
-                                for f in (0.25, 0.5, 0.625, 0.75, 0.875, 1.0, 1.25, 1.5, 1.625, 1.75, 2.0)))
+            values = _row_block_sizes(getattr(engine, name))
         results = []
         for v in values:
             engine.set_tuning(**{name: v})

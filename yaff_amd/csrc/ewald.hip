@@ -525,7 +525,14 @@ __device__ void ewald_gsum_block(const ynb_sys &S, int64_t g0, int na, const dou
       a2 += gather(src + (int64_t)(sl + 2) * stride);
       a3 += gather(src + (int64_t)(sl + 3) * stride);
     }
-    for (; sl < ns; ++sl) a0 += gather(src + (int64_t)sl * stride);
+    if (sl < ns) {  // the last 1-3 slots into a0 in slot order, their loads issued together
+      double v[3];
+#pragma unroll
+      for (int u = 0; u < 3; ++u) v[u] = gather(src + (int64_t)min(sl + u, ns - 1) * stride);
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+        if (sl + u < ns) a0 += v[u];
+    }
     const double G = (a0 + a1) + (a2 + a3);
     if (accumulate)
       gpos[e] += G;

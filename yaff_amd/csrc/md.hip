@@ -74,6 +74,23 @@ __global__ __launch_bounds__(256) void k_kick_drift(const ynb_sys *sys, const in
   if (bin_count && S) bin_atom(*S, active, g, p[0], p[1], p[2], cb, bin_count, slot_idx, xw, wr, status);
 }
 
+// Sum of ns > 0 gradient slots src[0], src[stride], ...: slot l goes to
+// accumulator l mod 4, the accumulators are added in a fixed order.  Eight
+// loads per round (slots past ns read slot ns - 1, a valid address, and add
+// nothing), so a partial last round costs one round trip, not one per slot.
+__device__ inline double sum_slots(const double *src, int64_t stride, int ns) {
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int sl = 0; sl < ns; sl += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src[(int64_t)min(sl + u, ns - 1) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (sl + u < ns) a[u & 3] += v[u];
+  }
+  return (a[0] + a[1]) + (a[2] + a[3]);
+}
+
 // Term c of system s's total: its potentials' pair terms in order, then the
 // Ewald reciprocal term.
 __device__ inline void system_total(const ynb_sys &S, int s, int c, const double *pair_out, const double *ewald_out,
@@ -90,8 +107,8 @@ __device__ inline void system_total(const ynb_sys &S, int s, int c, const double
 // multiples of 64 atoms, so a block belongs to the system of its wave entry);
 // each block's sum of m v^2 goes to `partial` and the system's last block
 // adds them in a fixed order.  With gpart, the gradient is gpos + the
-// system's nslot Ewald partial gradients (slots of 3 npad doubles, eight
-// loads in flight, four interleaved sums in a fixed order) and is written
+// system's nslot Ewald partial gradients (slots of 3 npad doubles,
+// sum_slots) and is written
 // back to gpos.  With total, the last block also totals the system's energy
 // and virial terms (system_total).
 __global__ __launch_bounds__(192) void k_kick(const ynb_sys *sys, const int32_t *wave_sys, int64_t npad,
@@ -113,17 +130,7 @@ __global__ __launch_bounds__(192) void k_kick(const ynb_sys *sys, const int32_t 
     if (gpart && ns > 0) {
       const double *src = gpart + e;
       const int64_t stride = 3 * npad;
-      double a[4] = {0.0, 0.0, 0.0, 0.0};
-      int sl = 0;
-      for (; sl + 8 <= ns; sl += 8) {
-        double v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = src[(int64_t)(sl + u) * stride];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) a[u & 3] += v[u];
-      }
-      for (; sl < ns; ++sl) a[sl & 3] += src[(int64_t)sl * stride];
-      gr += (a[0] + a[1]) + (a[2] + a[3]);
+      gr += sum_slots(src, stride, ns);
       gpos[e] = gr;
     }
     const double v = vel[e] + 0.5 * (-gr * minv[g]) * dt[sid];
@@ -178,17 +185,7 @@ __global__ __launch_bounds__(192) void k_kick_kick_drift(
       if (gpart && ns > 0) {
         const double *src = gpart + e;
         const int64_t stride = 3 * npad;
-        double a[4] = {0.0, 0.0, 0.0, 0.0};
-        int sl = 0;
-        for (; sl + 8 <= ns; sl += 8) {
-          double v[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) v[u] = src[(int64_t)(sl + u) * stride];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) a[u & 3] += v[u];
-        }
-        for (; sl < ns; ++sl) a[sl & 3] += src[(int64_t)sl * stride];
-        gr += (a[0] + a[1]) + (a[2] + a[3]);
+        gr += sum_slots(src, stride, ns);
         gpos[e] = gr;
       }
       const double h = dt[sid];
