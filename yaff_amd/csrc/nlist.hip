@@ -74,7 +74,7 @@ __device__ inline void store_entry(const ynb_nlist &nl, int64_t npad, int64_t gi
 }
 
 constexpr int BW = 8;         // atoms (wavefronts) per block of the binned build
-constexpr int CAP = 1280;     // partner candidates staged in LDS at once
+constexpr int CAP = 1536;     // partner candidates staged in LDS at once
 constexpr int MAXNB = 512;    // partner bins per home group (ynb_sys_plan keeps (2R+1)^3 below)
 constexpr int NB_SMALL = YNB_BUILD_NB_SMALL;  // partner-bin tables of the common search boxes
 
