@@ -106,7 +106,13 @@ def tune(engine, cache=DEFAULT_CACHE, verbose=False):
             results.append((_time_kernels(engine, kernels), v))
             if verbose:
                 print('autotune %s=%s: %.1f us' % (name, v, results[-1][0]*1e3), flush=True)
-        t, v = min(results)
+        # the three fastest again with more runs: a single noisy minimum over
+        # many candidates would otherwise decide
+        finals = []
+        for t, v in sorted(results)[:3]:
+            engine.set_tuning(**{name: v})
+            finals.append((min(t, _time_kernels(engine, kernels, reps=11)), v))
+        t, v = min(finals)
         engine.set_tuning(**{name: v})
         best[name] = v
     if cache:
