@@ -297,6 +297,7 @@ class Server(object):
                 r.pos = bufs['pos_d'].numpy().copy()
                 r.vel_out = bufs['vel_d'].numpy().copy()
             r.t_done = now()
+            r._plan = None    # its planning (page-locked copies of its arrays) is no longer needed
             self.done.append(r)
         return True
 

@@ -101,6 +101,16 @@ class _Stager(object):
 
     def __init__(self):
         self.ops = []
+        self.blobs = []
+
+    def put_blob(self, buf, items):
+        """Copies out of one page-locked float64 host tensor buf: items are
+        (attr, element offset in attr, element offset in buf, elements).
+        The buffer goes to the device staging tensor with one asynchronous
+        copy of its own, after the staged data, and the scatter launch
+        places its pieces."""
+        if buf.numel():
+            self.blobs.append((buf, list(items)))
 
     def put(self, attr, off, arr):
         a = np.ascontiguousarray(arr)
@@ -113,8 +123,9 @@ class _Stager(object):
 
     def flush(self, eng):
         """One pinned buffer (segment table, then data) -> one copy to the
-        engine's staging tensor -> one ynb_scatter launch."""
-        if not self.ops:
+        engine's staging tensor, each put_blob buffer -> one copy after it ->
+        one ynb_scatter launch."""
+        if not self.ops and not self.blobs:
             return
         segs, blobs = [], []
         off = 0
@@ -140,6 +151,19 @@ class _Stager(object):
                 bits = int(np.array(value, dtype=_NP_DTYPE[t.dtype]).view({4: np.uint32, 8: np.uint64}[es]))
                 for c in range(0, nbytes, SEG_BYTES):
                     segs.append((dst + c, -1, min(SEG_BYTES, nbytes - c), bits, elem))
+        # put_blob buffers after the staged data (offsets relative to the data start)
+        boff = _round_up(off, 256)
+        placed = []
+        for buf, items in self.blobs:
+            for attr, o, so, n in items:
+                t = getattr(eng, attr)
+                if t.dtype != torch.float64 or o + n > t.numel():
+                    raise ValueError('blob copy into %s out of range or not float64' % attr)
+                dst, nbytes = t.data_ptr() + 8*o, 8*n
+                for c in range(0, nbytes, SEG_BYTES):
+                    segs.append((dst + c, boff + 8*so + c, min(SEG_BYTES, nbytes - c), 0, 8))
+            placed.append((boff, buf))
+            boff += _round_up(8*buf.numel(), 256)
         table = np.zeros(len(segs), _lib.SEGMENT_DTYPE)
         cols = list(zip(*segs))
         for k, name in enumerate(('dst', 'src', 'nbytes', 'fill', 'elem')):
@@ -155,12 +179,43 @@ class _Stager(object):
         buf[:table.nbytes] = table.view(np.uint8)
         for o, data in blobs:
             buf[head + o:head + o + len(data)] = data
-        if eng._stage_d.numel() < len(buf):
-            eng._stage_d = torch.empty(_round_up(int(len(buf)*GROWTH), 256), dtype=torch.uint8, device=eng.device)
+        if eng._stage_d.numel() < head + boff:
+            eng._stage_d = torch.empty(_round_up(int((head + boff)*GROWTH), 256), dtype=torch.uint8,
+                                       device=eng.device)
         eng._stage_d[:len(buf)].copy_(pinned, non_blocking=True)
+        for o, b in placed:
+            eng._stage_d[head + o:head + o + 8*b.numel()].view(torch.float64).copy_(b, non_blocking=True)
         base = eng._stage_d.data_ptr()
         _lib.check(eng.lib.ynb_scatter(base, len(table), base + head, eng.stream()))
         self.ops = []
+        self.blobs = []
+
+
+def _pin_member_arrays(s, entry, plan):
+    """The arrays of a member that do not depend on its placement
+    (positions, per-atom records and parameter rows, charges, masses and
+    their inverses, k vectors), copied into one page-locked buffer: the
+    planning threads pay for the host copies, and the commit sends the
+    buffer with one asynchronous copy.  Returns the float64 buffer and
+    name -> (element offset, elements)."""
+    n = s.natom
+    masses = np.ones(n) if s.masses is None else np.asarray(s.masses, dtype=np.float64)
+    items = [('pos', s.pos), ('rec', entry.rec)]
+    par = entry.par.reshape(MAX_POT*3, n)
+    items += [('par%d' % row, par[row]) for row in range(3*len(s.pots))]
+    items += [('charges', np.zeros(n) if s.charges is None else s.charges), ('masses', masses),
+              ('minv', 1.0/masses)]
+    if plan is not None:
+        items.append(('kvec', plan['kpad']))
+    arrs = [(name, np.ascontiguousarray(a, dtype=np.float64).reshape(-1)) for name, a in items]
+    buf = torch.empty(sum(a.size for _, a in arrs), dtype=torch.float64, pin_memory=True)
+    host = buf.numpy()
+    where, off = {}, 0
+    for name, a in arrs:
+        host[off:off + a.size] = a
+        where[name] = (off, a.size)
+        off += a.size
+    return buf, where
 
 
 def _nlist_kcap(s):
@@ -408,7 +463,7 @@ class SlotEngine(Engine):
             raise ValueError('Ewald reciprocal needs a 3D periodic cell')
         entry = system_entry(self.lib, s, 0)
         plan = _ewald_local_plan(s, self.apl, self.skb, self.fkb) if s.ewald is not None and s.natom > 0 else None
-        return entry, plan, _nlist_kcap(s)
+        return entry, plan, _nlist_kcap(s), _pin_member_arrays(s, entry, plan)
 
     def admit(self, s, vel=None, prepared=None):
         """Places NBSystem s (planned by prepare(s), which admit calls when
@@ -420,7 +475,7 @@ class SlotEngine(Engine):
         if not free:
             raise RuntimeError('no free slot')
         slot = free[0]
-        entry, plan, kcap = self.prepare(s) if prepared is None else prepared
+        entry, plan, kcap, pinned = self.prepare(s) if prepared is None else prepared
         ranges = {}
         napad = _round_up(max(s.natom, 1), ALIGN)
         ranges['atoms'] = (self._take('atoms', napad), napad)
@@ -444,6 +499,7 @@ class SlotEngine(Engine):
             h.spart0 = ranges['spart'][0]
             h.nchunk_low, h.nchunk_high = plan['nchunk_low'], plan['nchunk_high']
         m = _Member(s, entry, ranges, plan, kcap)
+        m.pinned = pinned
         self.members[slot] = m
         self.systems[slot] = s
         self.atom0[slot] = ranges['atoms'][0]
@@ -608,21 +664,20 @@ class SlotEngine(Engine):
         st = self._stage
         a0, napad = m.ranges['atoms']
         n = s.natom
-        st.put('pos_d', 3*a0, s.pos)
+        pbuf, pin = m.pinned   # the slot-independent arrays, page-locked by prepare()
+        m.pinned = None
+        blob = [('pos_d', 3*a0) + pin['pos']]
         if vel is not None:
             st.put('vel_d', 3*a0, np.asarray(vel, dtype=np.float64))
         else:
             st.fill('vel_d', 3*a0, 3*napad, 0.0)
         st.fill('gpos_d', 3*a0, 3*napad, 0.0)
         st.fill('pos_ref_d', 3*a0, 3*napad, 1e150)
-        st.put('rec_d', 4*a0, e.rec)
-        par = e.par.reshape(MAX_POT*3, n)
+        blob.append(('rec_d', 4*a0) + pin['rec'])
         for row in range(3*len(s.pots)):   # the kernels read no row of a potential the system lacks
-            st.put('par_d', row*self.npad + a0, par[row])
-        st.put('charges_d', a0, np.zeros(n) if s.charges is None else s.charges)
-        masses = np.ones(n) if s.masses is None else np.asarray(s.masses, dtype=np.float64)
-        st.put('masses_d', a0, masses)
-        st.put('minv_d', a0, 1.0/masses)
+            blob.append(('par_d', row*self.npad + a0) + pin['par%d' % row])
+        blob += [('charges_d', a0) + pin['charges'], ('masses_d', a0) + pin['masses'],
+                 ('minv_d', a0) + pin['minv']]
         ws = np.full(napad//64, -1, np.int32)
         ws[:(max(n, 1) + 63)//64] = slot
         st.put('wave_sys_d', a0//64, ws)
@@ -641,7 +696,7 @@ class SlotEngine(Engine):
             w0 = m.ranges['work'][0]
             rec0, nrec = m.ranges['rec']
             nkp = p['nkp']
-            st.put('kvec_d', 4*k0, p['kpad'])
+            blob.append(('kvec_d', 4*k0) + pin['kvec'])
             st.put('kpos_d', k0, np.where(p['kpos'] >= 0, p['kpos'] + rec0, -1).astype(np.int32))
             st.fill('kblk_sys_d', k0//KBLOCK, nkp//KBLOCK, slot)
             if p['nrow']:
@@ -655,6 +710,7 @@ class SlotEngine(Engine):
             st.put('chunk_d', 4*c0, ch)
             self._work_gen.append((w0, len(p['skblk']), len(p['chunks']), slot, sb0, c0))
             st.fill('ew_frec_d', 2*rec0, 2*nrec, 0.0)
+        st.put_blob(pbuf, blob)
 
     def _commit_structs(self):
         """The ctypes structures of the launches, pointing at the current tables."""
