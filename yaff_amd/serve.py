@@ -100,27 +100,42 @@ def _kcap_estimate(nb):
     return int(1.3*n) + 32
 
 
+def _estimate_terms(s, chunk, fkb):
+    """Per-system terms of estimate_bytes (padded atoms, row capacity, Ewald
+    bytes, force slots), kept on the system for its launch parameters: an
+    admission re-estimates the whole batch, so the terms are computed once
+    per system, not once per admission."""
+    key = (chunk, fkb)
+    memo = getattr(s, '_estimate_terms', None)
+    if memo is not None and memo[0] == key:
+        return memo[1]
+    ew_bytes = nslot = 0
+    if s.ewald is not None and s.natom > 0:
+        nk = len(ewald_kvectors_cached(s)[1])
+        nkp = _round_up(max(nk, 1), 256)
+        nchunk = _ceil_div(s.natom, chunk)
+        ew_bytes = nkp*(32 + 16 + 48 + 64) + 16*nchunk*nkp
+        # greedy row blocks: two consecutive blocks hold more than fkb k vectors
+        nslot = _ceil_div(2*_ceil_div(nk, fkb) + 1, 4)
+    terms = (_round_up(max(s.natom, 1), ALIGN), _kcap_estimate(s), ew_bytes, nslot)
+    s._estimate_terms = (key, terms)
+    return terms
+
+
 def estimate_bytes(nbs, tuning=SERVER_TUNING):
     """Device memory of a batch engine over ``nbs`` (the dominant allocations
     of Engine: per-atom state, the ELL neighbour list at the batch's row
     capacity, the Ewald tables and partial sums)."""
     if not nbs:
         return 0
-    npad = sum(_round_up(max(s.natom, 1), ALIGN) for s in nbs)
-    kcap = max(_kcap_estimate(s) for s in nbs)
-    total = npad*(8*kcap + 8*3*3 + 8*3*_lib.MAX_POT + 8*3 + 4 + 2*_lib.MAX_POT*8 + 64)
     chunk = 64*(tuning.get('apl') or 4)
     fkb = tuning.get('fkb') or 64
-    nslot = 0
-    for s in nbs:
-        if s.ewald is None or s.natom == 0:
-            continue
-        nk = len(ewald_kvectors_cached(s)[1])
-        nkp = _round_up(max(nk, 1), 256)
-        nchunk = _ceil_div(s.natom, chunk)
-        total += nkp*(32 + 16 + 48 + 64) + 16*nchunk*nkp
-        # greedy row blocks: two consecutive blocks hold more than fkb k vectors
-        nslot = max(nslot, _ceil_div(2*_ceil_div(nk, fkb) + 1, 4))
+    terms = [_estimate_terms(s, chunk, fkb) for s in nbs]
+    npad = sum(t[0] for t in terms)
+    kcap = max(t[1] for t in terms)
+    total = npad*(8*kcap + 8*3*3 + 8*3*_lib.MAX_POT + 8*3 + 4 + 2*_lib.MAX_POT*8 + 64)
+    total += sum(t[2] for t in terms)
+    nslot = max(t[3] for t in terms)
     if nslot:
         total += npad*(nslot*24 + 64)
     return int(total)
